@@ -1,0 +1,166 @@
+"""Benchmark of the FDTD time-stepping hot path on one MI355X.
+
+A step is one iteration of the reference caller loop (fdtd.py:162-175):
+E update (CPML psi_E), soft source, H update (CPML psi_H) with the Poynting
+vector S.  The default workload is the README-scale photonic-crystal slab,
+395 x 345 x 73 cells, FP32, 8-cell CPML on all faces, Poynting on.
+
+Prints one JSON line.  value = Gcell-updates/s over the whole grid (PML
+included) with all fields resident in HBM.  roofline = the dominant kernel's
+algorithmic HBM bytes per launch / its average launch duration (HIP events on
+the launch stream).  cpu_baseline = the oracle (C++/OpenMP restatement of the
+reference's three-pass dataflow, FP32) on a bounded sample of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(shape, pmls, kind, dtype_bytes=4, poynting=True):
+    """Algorithmic bytes of one E or H launch: E kernel = E r+w, H r, eps r
+    (12 words/cell); H kernel = E r, H r, H w, S w (12 words/cell with
+    Poynting, 9 without); plus psi r+w (4 words) per CPML slab cell and face."""
+    n = float(numpy.prod(shape))
+    words = 12 if (kind == "E" or poynting) else 9
+    slab = 0.0
+    for p in pmls:
+        r = "xyz".find(p["axis"])
+        slab += min(int(p.get("thickness", 8)), shape[r]) * n / shape[r]
+    return dtype_bytes * (words * n + 4 * slab)
+
+
+def cpu_baseline(w, seconds: float):
+    from oracle.oracle import OracleSimulation
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        cores = min(cores, int(env))
+    orc = OracleSimulation(w.epsilon, [dict(p) for p in w.pmls], dt=w.dt, nthreads=cores)
+    orc.step(None, 0)   # warm-up: page in all buffers
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.step(None, steps)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 400:
+            break
+    rate = w.cells * steps / el / 1e9
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d steps of the full %s grid (FP32, three-pass reference dataflow: E, H+oS, S), "
+                      "%.1f s on %s" % (steps, "x".join(map(str, w.shape)), el, platform.processor() or "host")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="phc_slab")
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--kernel-steps", type=int, default=200, help="steps timed kernel by kernel")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from opencl_fdtd_amd import Simulation
+    from opencl_fdtd_amd import workloads
+
+    total = args.warmup + args.steps
+    w = workloads.BUILDERS[args.workload](steps=max(total, args.kernel_steps))
+    dtype = numpy.float32 if args.dtype == "f32" else numpy.float64
+    sim = Simulation(w.epsilon, [dict(p) for p in w.pmls], dt=w.dt, float_type=dtype,
+                     bloch_boundaries=w.bloch_boundaries, do_poynting=w.do_poynting)
+    stream = torch.cuda.current_stream()
+    wave = torch.as_tensor(w.src_wave, dtype=torch.float32 if dtype == numpy.float32 else torch.float64,
+                           device="cuda")
+    print("bench: %s %s cells=%d steps=%d warmup=%d" % (w.name, args.dtype, w.cells, args.steps, args.warmup),
+          file=sys.stderr, flush=True)
+
+    sim.step_n(args.warmup, w.src_index, wave[:args.warmup])
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    sim.step_n(args.steps, w.src_index, wave[args.warmup:args.warmup + args.steps])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    gcell = w.cells * args.steps / (ms / 1e3) / 1e9
+    print("bench: timed %.1f ms (wall %.3f s) -> %.2f Gcell/s" % (ms, wall, gcell), file=sys.stderr, flush=True)
+
+    # kernel-by-kernel timing (HIP events on the launch stream)
+    dur = {"E": [], "H": []}
+    evs = []
+    for _ in range(args.kernel_steps):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        sim.update_E([])
+        b.record(stream)
+        sim.update_H([])
+        c.record(stream)
+        evs.append((a, b, c))
+    torch.cuda.synchronize()
+    for a, b, c in evs:
+        dur["E"].append(a.elapsed_time(b))
+        dur["H"].append(b.elapsed_time(c))
+    dbytes = 4 if dtype == numpy.float32 else 8
+    kern = {}
+    for k in ("E", "H"):
+        avg_ms = float(numpy.mean(dur[k]))
+        byts = kernel_bytes(w.shape, w.pmls, k, dbytes, w.do_poynting)
+        kern[k] = {"avg_us": avg_ms * 1e3, "bytes": byts, "GBs": byts / (avg_ms / 1e3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    print("bench: kernels %s" % json.dumps(kern), file=sys.stderr, flush=True)
+
+    step_bytes = kern["E"]["bytes"] + kern["H"]["bytes"]
+    result = {
+        "metric": "Gcell-updates/s + %HBM roofline; learned vs 16-cell CPML at -40 dB reflection",
+        "value": gcell,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": w.name, "shape": list(w.shape), "pml": "CPML-8 on 6 faces",
+                   "poynting": w.do_poynting, "cells": w.cells,
+                   "bytes_per_cell_update": step_bytes / w.cells,
+                   "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "kernel": "h_march (H+S)" if dom == "H" else "e_march (E)",
+                     "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "avg_us": kern[dom]["avg_us"], "bytes_per_launch": kern[dom]["bytes"],
+                     "other_kernel": {k: v for k, v in kern.items() if k != dom}},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,147 @@
+/*
+ * fdtd.h -- C-ABI of the MI355X FDTD time-stepping library (libfdtd_hip.so).
+ *
+ * Plain C: device pointers, sizes and a stream handle; no torch types.  The
+ * caller (the Python Simulation class, through ctypes) owns every device
+ * buffer; the library allocates no device memory.
+ *
+ * Each entry point replaces one piece of the reference's device boundary
+ * (opencl_fdtd, where every update is a pyopencl ElementwiseKernel built from
+ * a Jinja2 template and bound with its argument list at
+ * opencl_fdtd/simulation.py:321-326):
+ *
+ *   fdtd_plan_create   <- template rendering + ElementwiseKernel construction,
+ *                         simulation.py:170-271 (shape baked in at
+ *                         kernels/common.cl:16-19, PML list at :161-168,
+ *                         coefficients at :274-319)
+ *   fdtd_update_e      <- sim.update_E (+ sim.update_F with Bloch),
+ *                         kernels/update_e.cl, bound at simulation.py:252,263
+ *   fdtd_update_h      <- sim.update_H (+ sim.update_G), kernels/update_h.cl,
+ *                         bound at simulation.py:253,264; with Poynting fused
+ *                         it also does the work of kernels/update_s.cl
+ *   fdtd_update_s      <- sim.update_S, kernels/update_s.cl, simulation.py:260-261
+ *   fdtd_update_j      <- sim.update_J, kernels/update_j.cl, simulation.py:265-271
+ *   fdtd_add_point     <- the caller's soft source  sim.E[ind] += f(t)
+ *                         (fdtd.py:168-169), with f(t) read from device memory
+ *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
+ *                         (E, F, source, H, G, S) without host round trips
+ *
+ * Field layout (kernels/common.cl:34-61, fdfd_tools.vec at simulation.py:347):
+ * a vector field is 3 contiguous components of N = sx*sy*sz values each, and
+ * inside a component index i = z + sz*(y + sy*x).  The domain wraps
+ * periodically on every axis (kernels/common.cl:75-85).
+ *
+ * Every call returns 0 on success or a nonzero status; fdtd_last_error()
+ * then describes the failure (thread-local).  Calls are asynchronous on the
+ * given stream (a hipStream_t; NULL = the null stream).  A plan is immutable
+ * after creation and may be shared by streams; buffers may not.
+ */
+#ifndef FDTD_MI355X_H
+#define FDTD_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_ABI_VERSION 1
+#define FDTD_MAX_FACES 12
+
+enum { FDTD_F32 = 0, FDTD_F64 = 1 };
+
+/* Poynting modes (do_poynting in the reference, simulation.py:87, 227-235) */
+enum {
+    FDTD_POYNTING_OFF = 0,
+    FDTD_POYNTING_FUSED = 1,     /* S computed inside fdtd_update_h; needs H != H_out */
+    FDTD_POYNTING_TWO_PASS = 2   /* reference dataflow: H writes oS (6N), fdtd_update_s reads it */
+};
+
+/* Bloch modes (reference: simulation.py:157-159, 187-224) */
+enum {
+    FDTD_BLOCH_PHASE = 0,        /* neighbour across the seam multiplied by the phase factor */
+    FDTD_BLOCH_DECOUPLED = 1     /* reference as shipped: F/G a second periodic system */
+};
+
+/* One CPML face, in list order (simulation.py:161-168, 274-319). */
+typedef struct {
+    int32_t axis;        /* 0 = x, 1 = y, 2 = z */
+    int32_t polarity;    /* 0 = 'n' (low side), 1 = 'p' (high side) */
+    int32_t thickness;   /* layers T */
+    int32_t reserved;
+} fdtd_face;
+
+typedef struct {
+    int64_t shape[3];        /* sx, sy, sz */
+    int32_t dtype;           /* FDTD_F32 or FDTD_F64 */
+    int32_t n_face;          /* <= FDTD_MAX_FACES */
+    fdtd_face face[FDTD_MAX_FACES];
+    double dt;               /* passed to the kernels as the field type */
+    double inv_dx;           /* uniform 1/dx (computed in double, rounded to the field type) */
+    int32_t nonuniform;      /* 1: per-index 1/dx arrays in fdtd_fields.inv_d */
+    int32_t poynting;        /* FDTD_POYNTING_* */
+    int32_t bloch_mask;      /* bit r set: Bloch boundary on axis r (needs F, G) */
+    int32_t bloch_mode;      /* FDTD_BLOCH_* */
+    double bloch_re[3];      /* phase factor real(exp(i phi)) per axis */
+    double bloch_im[3];      /* phase factor imag(exp(i phi)) per axis */
+} fdtd_desc;
+
+/* Device buffers of one simulation.  Vector fields hold 3N values. */
+typedef struct {
+    void* E;          /* E field, updated in place by fdtd_update_e */
+    void* H;          /* H field read by the H step (H^{n-1/2}) */
+    void* H_out;      /* H field written by the H step; == H for in-place.
+                         FDTD_POYNTING_FUSED needs H_out != H (ping-pong) */
+    void* eps;        /* relative permittivity, 3N; +inf marks PEC */
+    void* S;          /* Poynting vector, 3N (poynting != OFF) */
+    void* oS;         /* 6N partial products (FDTD_POYNTING_TWO_PASS only) */
+    void* F;          /* Bloch partner of E (bloch_mask != 0) */
+    void* G;          /* Bloch partner of H, read */
+    void* G_out;      /* Bloch partner of H, written (same rule as H_out) */
+    void* psi_e;      /* CPML psi_E block, fdtd_psi_elems() values, zero-initialised */
+    void* psi_h;      /* CPML psi_H block read */
+    void* psi_h_out;  /* CPML psi_H block written (== psi_h when H_out == H) */
+    void* psi_f;      /* psi blocks of the Bloch partner system */
+    void* psi_g;
+    void* psi_g_out;
+    void* coef;       /* fdtd_coef_elems() values: per face p0e,p1e,p2e,p0h,p1h,p2h (T each) */
+    void* inv_d;      /* nonuniform only: 1/dx_e along x,y,z then 1/dx_h along x,y,z,
+                         lengths sx,sy,sz,sx,sy,sz (simulation.py:136-142, 179-182) */
+} fdtd_fields;
+
+int fdtd_abi_version(void);
+const char* fdtd_last_error(void);
+
+/* Validates the description and precomputes launch shapes.  Host memory only. */
+int fdtd_plan_create(const fdtd_desc* desc, void** plan_out);
+int fdtd_plan_destroy(void* plan);
+
+/* Sizes (in elements of the field type) of the caller-owned blocks. */
+int64_t fdtd_psi_elems(const void* plan);
+int64_t fdtd_coef_elems(const void* plan);
+
+/* One E half-step (E, and F when Bloch is on). */
+int fdtd_update_e(const void* plan, const fdtd_fields* f, void* stream);
+/* One H half-step (H, and G when Bloch is on); writes S when poynting is FUSED,
+   oS when it is TWO_PASS. */
+int fdtd_update_h(const void* plan, const fdtd_fields* f, void* stream);
+/* S from oS (TWO_PASS); a no-op returning 0 for FUSED. */
+int fdtd_update_s(const void* plan, const fdtd_fields* f, void* stream);
+/* E += c*Jr + s*Ji for xmin<=x<xmax, ymin<=y<ymax, zmin<=z<zmax; box = {xmin,xmax,ymin,ymax,zmin,zmax}. */
+int fdtd_update_j(const void* plan, const fdtd_fields* f, const void* Jr, const void* Ji,
+                  double c, double s, const int64_t* box, void* stream);
+/* field[index] += amp[0]; amp is a device pointer to one value of the field type. */
+int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp, void* stream);
+
+/* n full steps in the order of the reference caller loop (fdtd.py:162-175):
+ * E (+F), E[src_index] += src_wave[k] (skipped when src_wave is NULL),
+ * H (+G) with S.  With H_out != H the H (and G, psi_h, psi_g) buffers swap
+ * roles every step: after an odd n the newest H is in f->H_out. */
+int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
+                const void* src_wave, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTD_MI355X_H */

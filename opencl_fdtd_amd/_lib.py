@@ -1,0 +1,100 @@
+"""ctypes binding of ``libfdtd_hip.so`` (declared in ``include/fdtd.h``).
+
+The structures below mirror the C declarations field for field.  Loading
+fails loudly: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
+
+ABI_VERSION = 1
+MAX_FACES = 12
+
+F32, F64 = 0, 1
+POYNTING_OFF, POYNTING_FUSED, POYNTING_TWO_PASS = 0, 1, 2
+BLOCH_PHASE, BLOCH_DECOUPLED = 0, 1
+
+# Every symbol include/fdtd.h declares.
+EXPORTS = (
+    "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy",
+    "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
+    "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
+)
+
+
+class Face(ctypes.Structure):
+    _fields_ = [("axis", ctypes.c_int32), ("polarity", ctypes.c_int32),
+                ("thickness", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("shape", ctypes.c_int64 * 3),
+        ("dtype", ctypes.c_int32), ("n_face", ctypes.c_int32),
+        ("face", Face * MAX_FACES),
+        ("dt", ctypes.c_double), ("inv_dx", ctypes.c_double),
+        ("nonuniform", ctypes.c_int32), ("poynting", ctypes.c_int32),
+        ("bloch_mask", ctypes.c_int32), ("bloch_mode", ctypes.c_int32),
+        ("bloch_re", ctypes.c_double * 3), ("bloch_im", ctypes.c_double * 3),
+    ]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "E", "H", "H_out", "eps", "S", "oS", "F", "G", "G_out",
+        "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d")]
+
+
+class FdtdError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FdtdError(
+            "libfdtd_hip.so is not built (expected at %s); run __graft_entry__.build() "
+            "or python -m opencl_fdtd_amd.build" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    F = ctypes.POINTER(Fields)
+    lib.fdtd_abi_version.restype = ctypes.c_int
+    lib.fdtd_abi_version.argtypes = []
+    lib.fdtd_last_error.restype = ctypes.c_char_p
+    lib.fdtd_last_error.argtypes = []
+    lib.fdtd_plan_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_void_p)]
+    lib.fdtd_plan_destroy.argtypes = [P]
+    lib.fdtd_psi_elems.argtypes = [P]
+    lib.fdtd_psi_elems.restype = ctypes.c_int64
+    lib.fdtd_coef_elems.argtypes = [P]
+    lib.fdtd_coef_elems.restype = ctypes.c_int64
+    for name in ("fdtd_update_e", "fdtd_update_h", "fdtd_update_s"):
+        getattr(lib, name).argtypes = [P, F, P]
+    lib.fdtd_update_j.argtypes = [P, F, P, P, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_int64), P]
+    lib.fdtd_add_point.argtypes = [P, P, ctypes.c_int64, P, P]
+    lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
+    for name in EXPORTS:
+        if name not in ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems"):
+            getattr(lib, name).restype = ctypes.c_int
+    v = lib.fdtd_abi_version()
+    if v != ABI_VERSION:
+        raise FdtdError("libfdtd_hip.so ABI version %d, expected %d" % (v, ABI_VERSION))
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().fdtd_last_error().decode(errors="replace")
+        raise FdtdError("%s failed (status %d): %s" % (what, rc, msg))

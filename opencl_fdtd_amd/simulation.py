@@ -1,0 +1,397 @@
+"""The ``Simulation`` class: the reference host API over MI355X HIP kernels.
+
+Mirrors ``opencl_fdtd/simulation.py:25-357``: the same constructor arguments,
+field attributes (``E``, ``H``, ``S``, ``F``, ``G``, ``eps``, ``dt``,
+``shape``, ``arg_type``) and ``update_*`` callables that take a list of events
+to wait for and return an event.  Buffers are PyTorch-ROCm device tensors in
+the reference layout (3N values, component-major, C order inside a
+component), so ``sim.E[ind] += value`` works as in ``fdtd.py:168-169``.  The
+updates run as hand-written gfx950 kernels through the C-ABI of
+``libfdtd_hip.so``; nothing here computes field values on the host.
+
+Differences from the reference, all documented behaviour:
+
+* Poynting ``S`` is computed inside ``update_H`` (``poynting_mode='fused'``,
+  the default); ``update_S`` is then an event-only no-op kept for call-shape
+  parity.  In this mode ``H`` alternates between two device buffers, so read
+  ``sim.H`` after each update rather than holding on to an old tensor.
+  ``poynting_mode='two_pass'`` keeps the reference dataflow (6N ``oS`` buffer
+  and a separate S kernel).
+* With Bloch boundaries, ``update_E`` advances E and F together and
+  ``update_H`` advances H and G together (the F/G steps read only H/G and
+  E/F, which the paired step leaves untouched); ``update_F``/``update_G`` are
+  event-only.  ``bloch_mode='phase'`` (default) applies the phase factor at
+  the seam; ``'decoupled'`` reproduces the reference as shipped.
+* An unstable ``dt`` warns and is then used (the reference warns and leaves
+  ``dt`` unset); malformed initial fields raise (the reference builds the
+  exception without raising it).
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from typing import Dict, List, Optional, Sequence
+
+import numpy
+import torch
+
+from . import _lib
+from .pml import face_coefficients, with_defaults
+
+__all__ = ["Simulation", "Event", "DeviceField", "type_to_C"]
+
+
+def type_to_C(float_type) -> str:
+    """numpy float type -> C type name, simulation.py:360-376."""
+    if float_type == numpy.float16:
+        arg_type = "half"
+    elif float_type == numpy.float32:
+        arg_type = "float"
+    elif float_type == numpy.float64:
+        arg_type = "double"
+    else:
+        raise Exception("Unsupported type")
+    return arg_type
+
+
+class DeviceField(torch.Tensor):
+    """A device tensor with the pyopencl-style ``get()`` used by callers
+    (``fdtd.py:186-190``)."""
+
+    def get(self) -> numpy.ndarray:
+        with torch._C.DisableTorchFunctionSubclass():
+            return self.detach().cpu().numpy()
+
+
+class Event:
+    """Completion marker of an enqueued update (pyopencl.Event stand-in)."""
+
+    def __init__(self, stream: torch.cuda.Stream):
+        self._event = torch.cuda.Event()
+        self._event.record(stream)
+
+    def wait(self) -> None:
+        self._event.synchronize()
+
+    @property
+    def torch_event(self) -> torch.cuda.Event:
+        return self._event
+
+
+def _vec(fields, dtype, device) -> torch.Tensor:
+    parts = []
+    for f in fields:
+        t = f if isinstance(f, torch.Tensor) else torch.from_numpy(numpy.ascontiguousarray(f))
+        parts.append(t.reshape(-1).to(dtype=dtype))
+    return torch.cat(parts).to(device).contiguous()
+
+
+class Simulation:
+    """FDTD fields and update operations on one MI355X GPU.
+
+    After constructing this object, call ``update_E``, ``update_H`` (and
+    ``update_S``) each time step, as in the reference (simulation.py:28-51):
+
+        for t in range(max_t):
+            sim.update_E([]).wait()
+            sim.E[ind] += numpy.sin(omega * t * sim.dt)
+            event = sim.update_H([])
+            if sim.update_S:
+                event = sim.update_S([event])
+            event.wait()
+    """
+
+    def __init__(self,
+                 epsilon: List,
+                 pmls: List[Dict],
+                 bloch_boundaries: Sequence[Dict] = (),
+                 dxes=None,
+                 dt: Optional[float] = None,
+                 initial_fields: Optional[Dict[str, List]] = None,
+                 context=None,
+                 queue: Optional[torch.cuda.Stream] = None,
+                 float_type=numpy.float32,
+                 do_poynting: bool = True,
+                 do_fieldsrc: bool = False,
+                 poynting_mode: str = "fused",
+                 bloch_mode: str = "phase"):
+        if initial_fields is None:
+            initial_fields = {}
+        self._lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.FdtdError("Simulation needs a ROCm GPU (torch.cuda.is_available() is False)")
+        if context is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        else:
+            self.device = torch.device(context) if not isinstance(context, int) else torch.device("cuda", context)
+        if self.device.type != "cuda":
+            raise _lib.FdtdError("context must name a GPU device, got {}".format(self.device))
+        self.queue = queue
+        self.context = self.device
+
+        if len(epsilon) != 3:
+            raise Exception("Epsilon must be a list with length of 3")
+        shapes = [tuple(e.shape) for e in epsilon]
+        if not all(s == shapes[0] for s in shapes[1:]):
+            raise Exception("All epsilon grids must have the same shape. Shapes are {}".format(shapes))
+        if len(shapes[0]) != 3:
+            raise Exception("Epsilon grids must be 3-dimensional, got shape {}".format(shapes[0]))
+        self.shape = shapes[0]
+        self.arg_type = float_type
+        ctype = type_to_C(float_type)
+        if ctype == "half":
+            raise Exception("float16 is not supported (the reference never enables cl_khr_fp16)")
+        self._tdtype = torch.float32 if ctype == "float" else torch.float64
+        self._ndtype = numpy.float32 if ctype == "float" else numpy.float64
+        self.N = int(numpy.prod(self.shape))
+        self.sources = {}
+
+        # --- epsilon, dxes, dt (simulation.py:127-153, 340-347) -------------------
+        self.eps = _vec(epsilon, self._tdtype, self.device)
+        if dxes is None:
+            dxes = 1.0
+        if isinstance(dxes, (float, int)):
+            uniform_dx = float(dxes)
+            min_dx = dxes
+            self.inv_dxes = None
+        else:
+            uniform_dx = None
+            lists = list(dxes[0]) + list(dxes[1])
+            for r, d in enumerate(lists):
+                if len(d) != self.shape[r % 3]:
+                    raise Exception("dxes[{}][{}] has length {}, expected {}".format(
+                        r // 3, r % 3, len(d), self.shape[r % 3]))
+            self.inv_dxes = [torch.from_numpy((1 / numpy.asarray(d, dtype=numpy.float64)).astype(self._ndtype))
+                             .to(self.device) for d in lists]
+            min_dx = min(min(d) for d in lists)
+        max_dt = min_dx * .99 / numpy.sqrt(3)
+        if dt is None:
+            self.dt = max_dt
+        elif dt <= 0:
+            raise Exception("Invalid dt: {}".format(dt))
+        else:
+            if dt > max_dt:
+                warnings.warn("Warning: unstable dt: {}".format(dt))
+            self.dt = dt
+
+        # --- fields (simulation.py:155-159, 349-357) ---------------------------
+        self.bloch_boundaries = [dict(b) for b in bloch_boundaries]
+        for b in self.bloch_boundaries:
+            if b.get("axis") not in ("x", "y", "z"):
+                raise Exception("Bloch axis must be one of 'x', 'y', 'z'")
+        if poynting_mode not in ("fused", "two_pass"):
+            raise Exception("poynting_mode must be 'fused' or 'two_pass'")
+        if bloch_mode not in ("phase", "decoupled"):
+            raise Exception("bloch_mode must be 'phase' or 'decoupled'")
+        self.do_poynting = bool(do_poynting)
+        self.poynting_mode = poynting_mode if do_poynting else "off"
+        self.bloch_mode = bloch_mode
+        pingpong = self.poynting_mode == "fused"
+        self._E = self._create_field(initial_fields.get("E"))
+        self._H = [self._create_field(initial_fields.get("H"))]
+        if pingpong:
+            self._H.append(torch.empty_like(self._H[0]))
+        self._h = 0
+        self._F = self._G = None
+        if self.bloch_boundaries:
+            self._F = self._create_field(initial_fields.get("F"))
+            self._G = self._create_field(initial_fields.get("G"))
+        self._S = self._oS = None
+        if self.do_poynting:
+            self._S = torch.zeros_like(self._E)
+            if self.poynting_mode == "two_pass":
+                self._oS = torch.zeros(6 * self.N, dtype=self._tdtype, device=self.device)
+        self.Jr = self.Ji = None
+        if do_fieldsrc:
+            self.Jr = torch.zeros_like(self._E)
+            self.Ji = torch.zeros_like(self._E)
+
+        # --- PML faces and coefficients (simulation.py:161-168, 274-319) -------
+        self.pmls = [with_defaults(p) for p in pmls]
+        desc = _lib.Desc()
+        desc.shape[:] = list(self.shape)
+        desc.dtype = _lib.F32 if ctype == "float" else _lib.F64
+        if len(self.pmls) > _lib.MAX_FACES:
+            raise Exception("at most {} PML faces are supported".format(_lib.MAX_FACES))
+        desc.n_face = len(self.pmls)
+        coefs = []
+        for k, p in enumerate(self.pmls):
+            desc.face[k].axis = "xyz".find(p["axis"])
+            desc.face[k].polarity = 0 if p["polarity"] == "n" else 1
+            desc.face[k].thickness = int(p["thickness"])
+            coefs += face_coefficients(p, self.dt, self._ndtype)
+        desc.dt = float(self.dt)
+        desc.inv_dx = 1.0 / uniform_dx if uniform_dx is not None else 1.0
+        desc.nonuniform = 0 if uniform_dx is not None else 1
+        desc.poynting = {"off": _lib.POYNTING_OFF, "fused": _lib.POYNTING_FUSED,
+                         "two_pass": _lib.POYNTING_TWO_PASS}[self.poynting_mode]
+        mask = 0
+        re, im = [1.0, 1.0, 1.0], [0.0, 0.0, 0.0]
+        for b in self.bloch_boundaries:
+            r = "xyz".find(b["axis"])
+            mask |= 1 << r
+            re[r], im[r] = float(b["real"]), float(b["imag"])
+        desc.bloch_mask = mask
+        desc.bloch_mode = _lib.BLOCH_PHASE if bloch_mode == "phase" else _lib.BLOCH_DECOUPLED
+        desc.bloch_re[:] = re
+        desc.bloch_im[:] = im
+        self._desc = desc
+        plan = ctypes.c_void_p()
+        _lib.check(self._lib.fdtd_plan_create(ctypes.byref(desc), ctypes.byref(plan)), "fdtd_plan_create")
+        self._plan = plan
+        n_psi = int(self._lib.fdtd_psi_elems(plan))
+        self._coef = _vec([numpy.concatenate(coefs)] if coefs else [numpy.zeros(1)], self._tdtype, self.device)
+        zeros = lambda: torch.zeros(max(n_psi, 1), dtype=self._tdtype, device=self.device)
+        self._psi_e = zeros()
+        self._psi_h = [zeros()] + ([zeros()] if pingpong else [])
+        self._psi_f = self._psi_g = None
+        if self.bloch_boundaries:
+            self._psi_f = zeros()
+            self._psi_g = zeros()
+        self._inv_d = torch.cat(self.inv_dxes).contiguous() if self.inv_dxes is not None else None
+
+        names = {"float": "f32", "double": "f64"}[ctype]
+        pair = ", paired with F/G ({})".format(bloch_mode) if self.bloch_boundaries else ""
+        self.sources["E"] = "e_march<{}>{}: E-step, {} CPML faces".format(names, pair, len(self.pmls))
+        self.sources["H"] = "h_march<{}>{}: H-step, Poynting {}".format(names, pair, self.poynting_mode)
+        if self.do_poynting:
+            self.sources["S"] = ("fused into H" if self.poynting_mode == "fused" else "s_from_os<{}>".format(names))
+
+        self.update_E = self._update_E
+        self.update_H = self._update_H
+        self.update_S = self._update_S if self.do_poynting else None
+        self.update_F = self._noop if self.bloch_boundaries else None
+        self.update_G = self._noop if self.bloch_boundaries else None
+        self.update_J = self._update_J if do_fieldsrc else None
+
+    # ------------------------------------------------------------------ fields
+    def _create_field(self, initial_value):
+        if initial_value is None:
+            return torch.zeros(3 * self.N, dtype=self._tdtype, device=self.device)
+        if len(initial_value) != 3:
+            raise Exception("Initial field value must be a list of length 3")
+        if not all(tuple(f.shape) == tuple(self.shape) for f in initial_value):
+            raise Exception("Initial field list elements must have same shape as epsilon elements")
+        return _vec(initial_value, self._tdtype, self.device)
+
+    @property
+    def E(self) -> DeviceField:
+        return self._E.as_subclass(DeviceField)
+
+    @property
+    def H(self) -> DeviceField:
+        return self._H[self._h].as_subclass(DeviceField)
+
+    @property
+    def F(self) -> Optional[DeviceField]:
+        return None if self._F is None else self._F.as_subclass(DeviceField)
+
+    @property
+    def G(self) -> Optional[DeviceField]:
+        return None if self._G is None else self._G.as_subclass(DeviceField)
+
+    @property
+    def S(self) -> Optional[DeviceField]:
+        return None if self._S is None else self._S.as_subclass(DeviceField)
+
+    @property
+    def oS(self) -> Optional[DeviceField]:
+        return None if self._oS is None else self._oS.as_subclass(DeviceField)
+
+    # ------------------------------------------------------------------ plumbing
+    def _stream(self) -> torch.cuda.Stream:
+        return self.queue if self.queue is not None else torch.cuda.current_stream(self.device)
+
+    def _fields(self) -> _lib.Fields:
+        f = _lib.Fields()
+        ptr = lambda t: None if t is None else t.data_ptr()
+        f.E = ptr(self._E)
+        f.H = ptr(self._H[self._h])
+        f.H_out = ptr(self._H[1 - self._h]) if len(self._H) == 2 else f.H
+        f.eps = ptr(self.eps)
+        f.S = ptr(self._S)
+        f.oS = ptr(self._oS)
+        f.F = ptr(self._F)
+        f.G = ptr(self._G)
+        f.G_out = f.G
+        f.psi_e = ptr(self._psi_e)
+        f.psi_h = ptr(self._psi_h[self._h if len(self._psi_h) == 2 else 0])
+        f.psi_h_out = ptr(self._psi_h[1 - self._h]) if len(self._psi_h) == 2 else f.psi_h
+        f.psi_f = ptr(self._psi_f)
+        f.psi_g = ptr(self._psi_g)
+        f.psi_g_out = f.psi_g
+        f.coef = ptr(self._coef)
+        f.inv_d = ptr(self._inv_d)
+        return f
+
+    def _launch(self, fn, events, what, *extra):
+        stream = self._stream()
+        for e in events or ():
+            stream.wait_event(e.torch_event if isinstance(e, Event) else e)
+        f = self._fields()
+        with torch.cuda.device(self.device):
+            _lib.check(fn(self._plan, ctypes.byref(f), *extra, ctypes.c_void_p(stream.cuda_stream)), what)
+        return Event(stream)
+
+    def _update_E(self, events: Sequence = ()) -> Event:
+        return self._launch(self._lib.fdtd_update_e, events, "fdtd_update_e")
+
+    def _update_H(self, events: Sequence = ()) -> Event:
+        ev = self._launch(self._lib.fdtd_update_h, events, "fdtd_update_h")
+        if len(self._H) == 2:
+            self._h = 1 - self._h
+        return ev
+
+    def _update_S(self, events: Sequence = ()) -> Event:
+        if self.poynting_mode == "two_pass":
+            return self._launch(self._lib.fdtd_update_s, events, "fdtd_update_s")
+        return self._noop(events)
+
+    def _noop(self, events: Sequence = ()) -> Event:
+        stream = self._stream()
+        for e in events or ():
+            stream.wait_event(e.torch_event if isinstance(e, Event) else e)
+        return Event(stream)
+
+    def _update_J(self, events, c, s, xmin, xmax, ymin, ymax, zmin, zmax) -> Event:
+        box = (ctypes.c_int64 * 6)(int(xmin), int(xmax), int(ymin), int(ymax), int(zmin), int(zmax))
+        return self._launch(self._lib.fdtd_update_j, events, "fdtd_update_j",
+                            ctypes.c_void_p(self.Jr.data_ptr()), ctypes.c_void_p(self.Ji.data_ptr()),
+                            ctypes.c_double(c), ctypes.c_double(s), box)
+
+    # ------------------------------------------------------------------ fast path
+    def step_n(self, n: int, src_index: Optional[int] = None, src_wave=None, events: Sequence = ()) -> Event:
+        """``n`` iterations of the reference caller loop (fdtd.py:162-175)
+        enqueued without host synchronisation: E (+F), ``E[src_index] +=
+        src_wave[k]``, H (+G) with S.  ``src_wave`` is a device tensor (or
+        array) of ``n`` source values in the field dtype."""
+        stream = self._stream()
+        for e in events or ():
+            stream.wait_event(e.torch_event if isinstance(e, Event) else e)
+        wave_ptr = None
+        if src_wave is not None:
+            if src_index is None:
+                raise Exception("src_index is required with src_wave")
+            wave = src_wave if isinstance(src_wave, torch.Tensor) else torch.as_tensor(numpy.asarray(src_wave))
+            wave = wave.to(device=self.device, dtype=self._tdtype).contiguous()
+            if wave.numel() < n:
+                raise Exception("src_wave has {} values, need {}".format(wave.numel(), n))
+            self._wave_keepalive = wave
+            wave_ptr = wave.data_ptr()
+        f = self._fields()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.fdtd_step_n(self._plan, ctypes.byref(f), int(n),
+                                             int(src_index if src_index is not None else -1),
+                                             wave_ptr, ctypes.c_void_p(stream.cuda_stream)), "fdtd_step_n")
+        if len(self._H) == 2 and n % 2 == 1:
+            self._h = 1 - self._h
+        return Event(stream)
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            try:
+                self._lib.fdtd_plan_destroy(plan)
+            except Exception:
+                pass
+            self._plan = None

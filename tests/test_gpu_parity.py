@@ -1,0 +1,175 @@
+"""HIP kernels vs the CPU oracle on the same seeded inputs.
+
+Both sides round every operation separately in the same order, so the
+expected result is bit-identical fields; the north-star bar (relative L2
+<= 1e-5 per component) is asserted as well, to report the margin.
+"""
+import numpy
+import pytest
+
+from tests.helpers import build_pair, comp_rel_l2, fetch, random_fields, step_gpu, step_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _assert_parity(sim, orc, names=("E", "H", "S"), exact=True):
+    n = sim.N
+    for name in names:
+        g = fetch(sim, name)
+        o = getattr(orc, name)
+        if g is None and o is None:
+            continue
+        errs = comp_rel_l2(g, o, n)
+        print("%s rel-L2 per component: %s" % (name, errs), flush=True)
+        assert max(errs) <= TOL, (name, errs)
+        if exact:
+            bad = numpy.flatnonzero(g != o)
+            assert bad.size == 0, "%s differs at %d cells, first %s: gpu %r oracle %r" % (
+                name, bad.size, bad[:5], g[bad[:5]], o[bad[:5]])
+
+
+def _tiny(shape=(12, 10, 9), T=3, seed=0):
+    rng = numpy.random.default_rng(seed)
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    pmls = [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    return eps, pmls, init
+
+
+@pytest.mark.parametrize("dtype", [numpy.float32, numpy.float64])
+@pytest.mark.parametrize("mode", ["fused", "two_pass"])
+def test_tiny_all_faces(dtype, mode):
+    eps, pmls, init = _tiny()
+    print("case tiny", dtype.__name__, mode, flush=True)
+    sim, orc = build_pair(eps, pmls, dtype=dtype, init=init, poynting_mode=mode)
+    step_gpu(sim, 25)
+    step_oracle(orc, 25)
+    _assert_parity(sim, orc)
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 73), (5, 37, 3), (33, 4, 130), (3, 3, 3)])
+def test_odd_shapes(shape):
+    print("case odd", shape, flush=True)
+    eps, pmls, init = _tiny(shape=shape, T=2, seed=1)
+    sim, orc = build_pair(eps, pmls, init=init)
+    step_gpu(sim, 12)
+    step_oracle(orc, 12)
+    _assert_parity(sim, orc)
+
+
+def test_face_order_and_overlap():
+    """Faces listed in a scrambled order, one axis with thick slabs, one axis
+    without CPML."""
+    shape = (10, 9, 8)
+    rng = numpy.random.default_rng(3)
+    eps = [rng.uniform(1, 3, shape) for _ in range(3)]
+    pmls = [{"axis": "z", "polarity": "p", "thickness": 3}, {"axis": "x", "polarity": "n", "thickness": 4},
+            {"axis": "z", "polarity": "n", "thickness": 2}, {"axis": "x", "polarity": "p", "thickness": 4,
+                                                             "m": 3.0, "cfs_alpha": 0.05}]
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    sim, orc = build_pair(eps, pmls, init=init)
+    step_gpu(sim, 15)
+    step_oracle(orc, 15)
+    _assert_parity(sim, orc)
+
+
+def test_pec_cells():
+    shape = (10, 11, 12)
+    rng = numpy.random.default_rng(4)
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    mask = rng.random(shape) < 0.1
+    for e in eps:
+        e[mask] = numpy.inf
+    init = {"E": [numpy.where(mask, 0.0, f) for f in random_fields(rng, shape)],
+            "H": random_fields(rng, shape)}
+    pmls = [{"axis": a, "polarity": p, "thickness": 2} for a in "xyz" for p in "np"]
+    sim, orc = build_pair(eps, pmls, init=init)
+    step_gpu(sim, 10)
+    step_oracle(orc, 10)
+    _assert_parity(sim, orc)
+    E = fetch(sim, "E").reshape(3, *shape)
+    assert numpy.all(E[:, mask] == 0.0)
+
+
+def test_nonuniform_dxes():
+    shape = (9, 8, 10)
+    rng = numpy.random.default_rng(5)
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    dxes = [[rng.uniform(0.8, 1.3, s) for s in shape], [rng.uniform(0.8, 1.3, s) for s in shape]]
+    pmls = [{"axis": a, "polarity": p, "thickness": 2} for a in "xy" for p in "np"]
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    sim, orc = build_pair(eps, pmls, init=init, dxes=dxes)
+    step_gpu(sim, 10)
+    step_oracle(orc, 10)
+    _assert_parity(sim, orc)
+
+
+@pytest.mark.parametrize("bloch_mode", ["phase", "decoupled"])
+@pytest.mark.parametrize("mode", ["fused", "two_pass"])
+def test_bloch(bloch_mode, mode):
+    shape = (8, 10, 12)
+    rng = numpy.random.default_rng(6)
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    phx, phy = 0.7, -1.9
+    bloch = [{"axis": "x", "real": numpy.cos(phx), "imag": numpy.sin(phx)},
+             {"axis": "y", "real": numpy.cos(phy), "imag": numpy.sin(phy)}]
+    pmls = [{"axis": "z", "polarity": p, "thickness": 3} for p in "np"]
+    init = {k: random_fields(rng, shape) for k in "EHFG"}
+    sim, orc = build_pair(eps, pmls, bloch=bloch, init=init, bloch_mode=bloch_mode, poynting_mode=mode)
+    step_gpu(sim, 15)
+    step_oracle(orc, 15)
+    _assert_parity(sim, orc, names=("E", "H", "F", "G", "S"))
+
+
+def test_update_j():
+    shape = (8, 9, 10)
+    rng = numpy.random.default_rng(7)
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    sim, orc = build_pair(eps, [], do_fieldsrc=True)
+    jr = rng.standard_normal(3 * sim.N).astype(numpy.float32)
+    ji = rng.standard_normal(3 * sim.N).astype(numpy.float32)
+    import torch
+    sim.Jr.copy_(torch.from_numpy(jr))
+    sim.Ji.copy_(torch.from_numpy(ji))
+    orc.Jr[:] = jr
+    orc.Ji[:] = ji
+    for c, s, box in [(0.3, -1.2, (1, 5, 2, 7, 0, 10)), (1.0, 0.5, (0, 8, 0, 9, 3, 4)), (2.0, 2.0, (4, 4, 0, 9, 0, 10))]:
+        sim.update_J([], c, s, *box).wait()
+        orc.update_J(c, s, *box)
+    _assert_parity(sim, orc, names=("E",))
+
+
+def test_step_n_matches_loop():
+    """The sync-free multi-step entry with a device-side source equals the
+    caller loop with host-side source writes."""
+    from opencl_fdtd_amd.workloads import small
+    w = small(steps=60, n=24)
+    sim, orc = build_pair(w.epsilon, w.pmls, dt=w.dt)
+    ev = sim.step_n(60, w.src_index, w.src_wave.astype(numpy.float32))
+    ev.wait()
+    step_oracle(orc, 60, w.src_index, w.src_wave)
+    _assert_parity(sim, orc)
+
+
+def test_small_config_full_run():
+    """64^3, 20 boxes, CPML-8, Ez source, 1000 steps, Poynting on."""
+    from opencl_fdtd_amd.workloads import small
+    w = small()
+    print("case small_64 1000 steps", flush=True)
+    sim, orc = build_pair(w.epsilon, w.pmls, dt=w.dt)
+    step_gpu(sim, w.steps, w.src_index, w.src_wave)
+    step_oracle(orc, w.steps, w.src_index, w.src_wave)
+    _assert_parity(sim, orc)
+
+
+def test_phc_slab_shape():
+    """The README-scale 395x345x73 slab for 40 steps via step_n."""
+    from opencl_fdtd_amd.workloads import phc_slab
+    w = phc_slab(steps=40)
+    print("case phc_slab 40 steps", flush=True)
+    sim, orc = build_pair(w.epsilon, w.pmls, dt=w.dt)
+    sim.step_n(40, w.src_index, w.src_wave.astype(numpy.float32)).wait()
+    step_oracle(orc, 40, w.src_index, w.src_wave)
+    _assert_parity(sim, orc)
