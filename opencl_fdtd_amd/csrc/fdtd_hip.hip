@@ -4,23 +4,33 @@
 // (component-major, C order, z contiguous; kernels/common.cl:34-61), so the
 // (y, z) plane of one x index is contiguous in memory.  A thread owns one
 // plane cell p = y*sz + z and marches along x (the slowest axis) through a
-// chunk of planes, keeping the previous/next plane values it needs in
-// registers; 64 consecutive p of a wavefront give fully coalesced 256-byte
-// loads for every component whatever sz is.  In-plane neighbours (p - sz,
-// p - 1, ...) are re-read through the L1/L2 caches.  No MFMA: the stencil
-// does a few flops per 4-byte word.
+// chunk of planes:
+//   * the -x (E step) / +x (H step) neighbour values stay in registers from
+//     the previous plane (register queue);
+//   * the loads of plane x+1 are issued before plane x is computed
+//     (software pipelining), so every wave keeps ~a dozen loads in flight;
+//   * 64 consecutive p of a wavefront give fully coalesced 256-byte accesses
+//     for every component whatever sz is; in-plane neighbours (p - sz, p - 1,
+//     p + sz, p + 1) are re-read through L1/L2;
+//   * addresses are 32-bit byte offsets from per-component base pointers held
+//     in SGPRs (global_load ... v_off, s_base), keeping VGPR use low;
+//   * workgroup ids are remapped so that consecutive tiles run on the same
+//     XCD (the hardware deals workgroups round-robin over the 8 XCDs), which
+//     keeps the halo rows of neighbouring tiles in that XCD's L2.
+// No MFMA: the stencil does a few flops per 4-byte word.
 //
 // Arithmetic follows the reference templates operation by operation
 // (kernels/update_e.cl:39-107, kernels/update_h.cl:36-154,
 // kernels/update_s.cl:33-36); the file is compiled with -ffp-contract=off so
 // every product and sum is rounded separately, as in the CPU restatement.
 //
-// Poynting S is fused into the H kernel: S at a cell needs the oS products of
-// its -x, -y and -z neighbours; the -x ones come from the previous plane of
-// the march, the in-plane ones are recomputed from the unmodified H^{n-1/2}
-// buffer (H is ping-ponged in this mode, so no block ever reads a value
-// another block has already overwritten).  The reference's two-pass dataflow
-// (6N oS buffer + S pass) is kept as FDTD_POYNTING_TWO_PASS.
+// Poynting S is fused into the H kernel.  S at a cell needs the oS products
+// of its -x, -y and -z neighbours.  The -x ones come from the previous plane
+// of the march.  The in-plane ones are exchanged through LDS: a workgroup
+// covers a contiguous range of plane cells preceded by sz halo cells (one
+// row) whose H update it evaluates without storing it.  H and the H-side psi
+// are ping-ponged in this mode, so every read of H^{n-1/2} sees the
+// unmodified value whichever workgroup owns the cell.
 
 #include <hip/hip_runtime.h>
 
@@ -30,13 +40,14 @@
 #include <cstring>
 #include <cmath>
 #include <string>
+#include <utility>
 
 #include "../../include/fdtd.h"
 
 namespace {
 
-constexpr int kBlock = 256;          // 4 wave64 per workgroup
-constexpr int kTargetBlocks = 4096;  // enough workgroups to fill 256 CUs several times over
+constexpr int kBlock = 256;          // E kernel / H kernel without fused S
+constexpr int kTargetBlocks = 4096;  // workgroups per launch (256 CUs, several each)
 
 thread_local std::string g_last_error;
 
@@ -61,15 +72,15 @@ template <class T>
 struct FaceDev {
     int pol, thick;
     int lo, hi;          // slab coordinates [lo, hi) along the axis
-    int psi_off;         // offset of psi_u inside a psi block; psi_v follows
+    int psi_off;         // element offset of psi_u inside a psi block; psi_v follows
     int face_elems;      // elements of psi_u
-    int coef_off;        // offset of p0e inside the coefficient block
+    int coef_off;        // element offset of p0e inside the coefficient block
 };
 
 template <class T>
 struct Geom {
     int sx, sy, sz, plane;
-    int xchunk;
+    int xchunk, nchunk, pblocks, tile;   // launch decomposition (tile = plane cells per workgroup)
     int seam_mask;       // Bloch axes with the phase applied (0 when decoupled)
     T dt;
     T inv_u;
@@ -104,7 +115,7 @@ struct EArgs {
     Geom<T> g;
     const T* eps[3];
     ESys<T> s[2];
-    // fused point source: E_comp[src_i] += *src_amp after the update
+    // fused point source: E_comp[src] += *src_amp after the update
     const T* src_amp;
     int src_x, src_p, src_comp;
 };
@@ -121,6 +132,28 @@ struct HArgs {
 // Device helpers
 // ---------------------------------------------------------------------------
 
+// 32-bit byte offset from a uniform base pointer.
+template <class T>
+__device__ __forceinline__ T ld(const T* base, uint32_t boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, uint32_t boff, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Workgroup -> (plane tile, x chunk).  Hardware workgroup b runs on XCD
+// b % 8; logical tiles are handed out so each XCD gets a contiguous range.
+__device__ __forceinline__ void block_coords(int pblocks, int nchunk, int* pb, int* ck) {
+    const int nb = pblocks * nchunk;
+    const int b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, j = b >> 3;
+    const int lb = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
+    *pb = lb % pblocks;
+    *ck = lb / pblocks;
+}
+
 template <class T>
 __device__ __forceinline__ bool slab(const FaceDev<T>& f, int c, int s, int* ir) {
     if (c < f.lo || c >= f.hi) return false;
@@ -128,48 +161,48 @@ __device__ __forceinline__ bool slab(const FaceDev<T>& f, int c, int s, int* ir)
     return true;
 }
 
-// psi index: C order over the grid shape with the face axis replaced by its
-// thickness, coordinate ir along that axis.
+// psi element index: C order over the grid shape with the face axis replaced
+// by its thickness, coordinate ir along that axis.
 template <int R, class T>
-__device__ __forceinline__ size_t psi_index(const Geom<T>& g, const FaceDev<T>& f, int x, int y, int z, int p,
-                                            int ir) {
-    if (R == 0) return (size_t)ir * g.plane + p;
-    if (R == 1) return ((size_t)x * f.thick + ir) * g.sz + z;
-    return ((size_t)x * g.sy + y) * f.thick + ir;
+__device__ __forceinline__ uint32_t psi_index(const Geom<T>& g, const FaceDev<T>& f, int x, int y, int z, int p,
+                                              int ir) {
+    if (R == 0) return (uint32_t)ir * (uint32_t)g.plane + (uint32_t)p;
+    if (R == 1) return ((uint32_t)x * (uint32_t)f.thick + (uint32_t)ir) * (uint32_t)g.sz + (uint32_t)z;
+    return ((uint32_t)x * (uint32_t)g.sy + (uint32_t)y) * (uint32_t)f.thick + (uint32_t)ir;
 }
 
 // CPML of one face on the derivatives of one cell; update_e.cl:64-100 (ISE)
 // and update_h.cl:80-116.  d_ab = d(field_a)/d(b).
-template <int R, class T, bool ISE, bool WRITE>
-__device__ __forceinline__ void apply_face(const Geom<T>& g, const FaceDev<T>& f, int ir, size_t ip,
-                                           const T* psi_in, T* psi_out,
+template <int R, class T, bool ISE>
+__device__ __forceinline__ void apply_face(const Geom<T>& g, const FaceDev<T>& f, int ir, uint32_t ip,
+                                           const T* psi_in, T* psi_out, bool write,
                                            T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy,
                                            T& px, T& py, T& pz) {
     const T* c = g.coef + f.coef_off + (ISE ? 0 : 3 * f.thick);
     const T p0 = c[ir], p1 = c[f.thick + ir], p2 = c[2 * f.thick + ir];
-    const T* iu = psi_in + f.psi_off;
-    const T* iv = iu + f.face_elems;
+    const uint32_t bu = ((uint32_t)f.psi_off + ip) * (uint32_t)sizeof(T);
+    const uint32_t bv = bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T);
+    const T ou = ld(psi_in, bu), ov = ld(psi_in, bv);
     T su, sv;
     if (R == 0) {                      // r = x, (u, v) = (y, z)
         dzx *= p2; dyx *= p2;
-        su = p0 * iu[ip] + p1 * dzx;
-        sv = p0 * iv[ip] + p1 * dyx;
+        su = p0 * ou + p1 * dzx;
+        sv = p0 * ov + p1 * dyx;
         if (ISE) { py -= su; pz += sv; } else { py += su; pz -= sv; }
     } else if (R == 1) {               // r = y, (u, v) = (x, z)
         dzy *= p2; dxy *= p2;
-        su = p0 * iu[ip] + p1 * dzy;
-        sv = p0 * iv[ip] + p1 * dxy;
+        su = p0 * ou + p1 * dzy;
+        sv = p0 * ov + p1 * dxy;
         if (ISE) { px += su; pz -= sv; } else { px -= su; pz += sv; }
     } else {                           // r = z, (u, v) = (x, y)
         dyz *= p2; dxz *= p2;
-        su = p0 * iu[ip] + p1 * dyz;
-        sv = p0 * iv[ip] + p1 * dxz;
+        su = p0 * ou + p1 * dyz;
+        sv = p0 * ov + p1 * dxz;
         if (ISE) { px -= su; py += sv; } else { px += su; py -= sv; }
     }
-    if (WRITE) {
-        T* ou = psi_out + f.psi_off;
-        ou[ip] = su;
-        ou[ip + f.face_elems] = sv;
+    if (write) {
+        st(psi_out, bu, su);
+        st(psi_out, bv, sv);
     }
 }
 
@@ -183,125 +216,191 @@ __device__ __forceinline__ T seam_mix(T own, T partner, T a, T b) {
 // derivatives and add at most one term each to a psi-sum that starts at 0,
 // so this order gives bit-identical results to any interleaving of the list
 // unless two slabs of the same axis overlap (a grid thinner than both).
-template <class T, bool ISE, bool WRITE>
+template <class T, bool ISE>
 __device__ __forceinline__ void cpml_cell(const Geom<T>& g, int x, int y, int z, int p, const T* psi_in,
-                                          T* psi_out, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy,
-                                          T& px, T& py, T& pz) {
+                                          T* psi_out, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
+                                          T& dzy, T& px, T& py, T& pz) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const FaceDev<T>& f = g.face[0][k];
         int ir;
         if (f.thick && slab(f, x, g.sx, &ir))
-            apply_face<0, T, ISE, WRITE>(g, f, ir, psi_index<0>(g, f, x, y, z, p, ir), psi_in, psi_out,
-                                         dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+            apply_face<0, T, ISE>(g, f, ir, psi_index<0>(g, f, x, y, z, p, ir), psi_in, psi_out, write,
+                                  dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const FaceDev<T>& f = g.face[1][k];
         int ir;
         if (f.thick && slab(f, y, g.sy, &ir))
-            apply_face<1, T, ISE, WRITE>(g, f, ir, psi_index<1>(g, f, x, y, z, p, ir), psi_in, psi_out,
-                                         dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+            apply_face<1, T, ISE>(g, f, ir, psi_index<1>(g, f, x, y, z, p, ir), psi_in, psi_out, write,
+                                  dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const FaceDev<T>& f = g.face[2][k];
         int ir;
         if (f.thick && slab(f, z, g.sz, &ir))
-            apply_face<2, T, ISE, WRITE>(g, f, ir, psi_index<2>(g, f, x, y, z, p, ir), psi_in, psi_out,
-                                         dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+            apply_face<2, T, ISE>(g, f, ir, psi_index<2>(g, f, x, y, z, p, ir), psi_in, psi_out, write,
+                                  dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     }
 }
 
+// Per-thread constants of a plane cell.
+struct Cell {
+    int p, y, z;
+    uint32_t op;          // byte offset of p
+    uint32_t om_y, om_z;  // byte offsets of p - e_y, p - e_z (periodic)
+    uint32_t op_y, op_z;  // byte offsets of p + e_y, p + e_z (periodic)
+};
+
+template <class T>
+__device__ __forceinline__ Cell make_cell(const Geom<T>& g, int p) {
+    Cell c;
+    c.p = p;
+    c.y = p / g.sz;
+    c.z = p - c.y * g.sz;
+    const int pym = (c.y == 0) ? p + (g.sy - 1) * g.sz : p - g.sz;
+    const int pzm = (c.z == 0) ? p + g.sz - 1 : p - 1;
+    const int pyp = (c.y == g.sy - 1) ? p - (g.sy - 1) * g.sz : p + g.sz;
+    const int pzp = (c.z == g.sz - 1) ? p - (g.sz - 1) : p + 1;
+    constexpr uint32_t B = sizeof(T);
+    c.op = (uint32_t)p * B;
+    c.om_y = (uint32_t)pym * B;
+    c.om_z = (uint32_t)pzm * B;
+    c.op_y = (uint32_t)pyp * B;
+    c.op_z = (uint32_t)pzp * B;
+    return c;
+}
+
 // ---------------------------------------------------------------------------
-// E half-step, update_e.cl:20-107.  One thread per plane cell, marching x.
+// E half-step, update_e.cl:20-107.
 // ---------------------------------------------------------------------------
 
+template <class T>
+struct EIn {
+    T hx, hy, hz, hx_ym, hz_ym, hx_zm, hy_zm, ex, ey, ez;
+};
+
 template <class T, bool SEAM>
-__device__ __forceinline__ void e_cell(const EArgs<T>& a, const ESys<T>& s, int x, int y, int z, int p,
-                                       int pym, int pzm, size_t b, T hy_xm, T hz_xm, T& hy, T& hz,
-                                       const T* eps0, const T* eps1, const T* eps2, bool src_here) {
-    const Geom<T>& g = a.g;
-    const T hx = s.H[0][b + p];
-    hy = s.H[1][b + p];
-    hz = s.H[2][b + p];
-    T hx_ym = s.H[0][b + pym], hz_ym = s.H[2][b + pym];
-    T hx_zm = s.H[0][b + pzm], hy_zm = s.H[1][b + pzm];
+__device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, const Cell& c, uint32_t ob,
+                                        EIn<T>& v) {
+    v.hx = ld(s.H[0], ob + c.op);
+    v.hy = ld(s.H[1], ob + c.op);
+    v.hz = ld(s.H[2], ob + c.op);
+    v.hx_ym = ld(s.H[0], ob + c.om_y);
+    v.hz_ym = ld(s.H[2], ob + c.om_y);
+    v.hx_zm = ld(s.H[0], ob + c.om_z);
+    v.hy_zm = ld(s.H[1], ob + c.om_z);
+    v.ex = ld(s.E[0], ob + c.op);
+    v.ey = ld(s.E[1], ob + c.op);
+    v.ez = ld(s.E[2], ob + c.op);
     if (SEAM) {
-        if ((g.seam_mask & 2) && y == 0) {
-            hx_ym = seam_mix(hx_ym, s.P[0][b + pym], s.sa[1], s.sb[1]);
-            hz_ym = seam_mix(hz_ym, s.P[2][b + pym], s.sa[1], s.sb[1]);
+        if ((g.seam_mask & 2) && c.y == 0) {
+            v.hx_ym = seam_mix(v.hx_ym, ld(s.P[0], ob + c.om_y), s.sa[1], s.sb[1]);
+            v.hz_ym = seam_mix(v.hz_ym, ld(s.P[2], ob + c.om_y), s.sa[1], s.sb[1]);
         }
-        if ((g.seam_mask & 4) && z == 0) {
-            hx_zm = seam_mix(hx_zm, s.P[0][b + pzm], s.sa[2], s.sb[2]);
-            hy_zm = seam_mix(hy_zm, s.P[1][b + pzm], s.sa[2], s.sb[2]);
+        if ((g.seam_mask & 4) && c.z == 0) {
+            v.hx_zm = seam_mix(v.hx_zm, ld(s.P[0], ob + c.om_z), s.sa[2], s.sb[2]);
+            v.hy_zm = seam_mix(v.hy_zm, ld(s.P[1], ob + c.om_z), s.sa[2], s.sb[2]);
         }
     }
-    T ix = g.inv_u, iy = g.inv_u, iz = g.inv_u;
-    if (g.inv_h[0]) { ix = g.inv_h[0][x]; iy = g.inv_h[1][y]; iz = g.inv_h[2][z]; }
-    T dxy = (hx - hx_ym) * iy;
-    T dxz = (hx - hx_zm) * iz;
-    T dyx = (hy - hy_xm) * ix;
-    T dyz = (hy - hy_zm) * iz;
-    T dzx = (hz - hz_xm) * ix;
-    T dzy = (hz - hz_ym) * iy;
+}
+
+template <class T>
+__device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, const Cell& c, int x, uint32_t oi,
+                                         const EIn<T>& v, T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy,
+                                         T iz, int src_comp, T amp) {
+    T dxy = (v.hx - v.hx_ym) * iy;
+    T dxz = (v.hx - v.hx_zm) * iz;
+    T dyx = (v.hy - hy_xm) * ix;
+    T dyz = (v.hy - v.hy_zm) * iz;
+    T dzx = (v.hz - hz_xm) * ix;
+    T dzy = (v.hz - v.hz_ym) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
-    cpml_cell<T, true, true>(g, x, y, z, p, s.psi, s.psi, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    const size_t i = b + p;
-    T ex = s.E[0][i] + a.g.dt / eps0[i] * (dzy - dyz + px);
-    T ey = s.E[1][i] + a.g.dt / eps1[i] * (dxz - dzx + py);
-    T ez = s.E[2][i] + a.g.dt / eps2[i] * (dyx - dxy + pz);
-    if (src_here) {
-        const T amp = *a.src_amp;
-        if (a.src_comp == 0) ex = ex + amp;
-        else if (a.src_comp == 1) ey = ey + amp;
-        else ez = ez + amp;
-    }
-    s.E[0][i] = ex;
-    s.E[1][i] = ey;
-    s.E[2][i] = ez;
+    cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    T ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
+    T ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
+    T ez = v.ez + g.dt / eps2 * (dyx - dxy + pz);
+    if (src_comp == 0) ex = ex + amp;
+    else if (src_comp == 1) ey = ey + amp;
+    else if (src_comp == 2) ez = ez + amp;
+    st(s.E[0], oi, ex);
+    st(s.E[1], oi, ey);
+    st(s.E[2], oi, ez);
 }
 
 template <class T, bool PAIR, bool SEAM>
 __global__ __launch_bounds__(kBlock) void e_march(EArgs<T> a) {
     const Geom<T>& g = a.g;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    int pb, ck;
+    block_coords(g.pblocks, g.nchunk, &pb, &ck);
+    const int p = pb * kBlock + threadIdx.x;
     if (p >= g.plane) return;
-    const int y = p / g.sz;
-    const int z = p - y * g.sz;
-    const int pym = (y == 0) ? p + (g.sy - 1) * g.sz : p - g.sz;
-    const int pzm = (z == 0) ? p + g.sz - 1 : p - 1;
-    const int x0 = blockIdx.y * g.xchunk;
+    const Cell c = make_cell<T>(g, p);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const int x0 = ck * g.xchunk;
     const int x1 = min(x0 + g.xchunk, g.sx);
-    const int xm = (x0 == 0) ? g.sx - 1 : x0 - 1;
-    const size_t bm = (size_t)xm * g.plane + p;
     constexpr int NS = PAIR ? 2 : 1;
+
+    // H at the -x neighbour of the first plane (wrap; Bloch phase on x)
+    const int xm = (x0 == 0) ? g.sx - 1 : x0 - 1;
+    const uint32_t obm = (uint32_t)xm * pbytes;
     T hy_m[NS], hz_m[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        hy_m[k] = a.s[k].H[1][bm];
-        hz_m[k] = a.s[k].H[2][bm];
+        const ESys<T>& s = a.s[k];
+        hy_m[k] = ld(s.H[1], obm + c.op);
+        hz_m[k] = ld(s.H[2], obm + c.op);
         if (SEAM && (g.seam_mask & 1) && x0 == 0) {
-            hy_m[k] = seam_mix(hy_m[k], a.s[k].P[1][bm], a.s[k].sa[0], a.s[k].sb[0]);
-            hz_m[k] = seam_mix(hz_m[k], a.s[k].P[2][bm], a.s[k].sa[0], a.s[k].sb[0]);
+            hy_m[k] = seam_mix(hy_m[k], ld(s.P[1], obm + c.op), s.sa[0], s.sb[0]);
+            hz_m[k] = seam_mix(hz_m[k], ld(s.P[2], obm + c.op), s.sa[0], s.sb[0]);
         }
     }
+    const T iy = g.inv_h[1] ? g.inv_h[1][c.y] : g.inv_u;
+    const T iz = g.inv_h[2] ? g.inv_h[2][c.z] : g.inv_u;
+
+    EIn<T> cur[NS], nxt[NS];
+    T e0, e1, e2, n0 = (T)0, n1 = (T)0, n2 = (T)0;
+    {
+        const uint32_t ob = (uint32_t)x0 * pbytes;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, ob, cur[k]);
+        e0 = ld(a.eps[0], ob + c.op);
+        e1 = ld(a.eps[1], ob + c.op);
+        e2 = ld(a.eps[2], ob + c.op);
+    }
     for (int x = x0; x < x1; ++x) {
-        const size_t b = (size_t)x * g.plane;
-        const bool src_here = a.src_amp != nullptr && x == a.src_x && p == a.src_p;
+        const uint32_t ob = (uint32_t)x * pbytes;
+        if (x + 1 < x1) {
+            const uint32_t on = ob + pbytes;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, on, nxt[k]);
+            n0 = ld(a.eps[0], on + c.op);
+            n1 = ld(a.eps[1], on + c.op);
+            n2 = ld(a.eps[2], on + c.op);
+        }
+        const T ix = g.inv_h[0] ? g.inv_h[0][x] : g.inv_u;
+        int src_comp = -1;
+        T amp = (T)0;
+        if (a.src_amp != nullptr && x == a.src_x && p == a.src_p) {
+            src_comp = a.src_comp;
+            amp = *a.src_amp;
+        }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            T hy, hz;
-            e_cell<T, SEAM>(a, a.s[k], x, y, z, p, pym, pzm, b, hy_m[k], hz_m[k], hy, hz,
-                            a.eps[0], a.eps[1], a.eps[2], k == 0 && src_here);
-            hy_m[k] = hy;
-            hz_m[k] = hz;
+            e_update<T>(g, a.s[k], c, x, ob + c.op, cur[k], hy_m[k], hz_m[k], e0, e1, e2, ix, iy, iz,
+                        k == 0 ? src_comp : -1, amp);
+            hy_m[k] = cur[k].hy;
+            hz_m[k] = cur[k].hz;
+            cur[k] = nxt[k];
         }
+        e0 = n0; e1 = n1; e2 = n2;
     }
 }
 
 // ---------------------------------------------------------------------------
-// H half-step, update_h.cl:22-154, with S fused (update_s.cl:33-36).
+// H half-step, update_h.cl:22-154, with S (update_s.cl:33-36).
 // ---------------------------------------------------------------------------
 
 template <class T>
@@ -310,168 +409,219 @@ struct HCell {
     T aExy, aExz, aEyx, aEyz, aEzx, aEzy;
 };
 
-// The H update of cell (x, y, z).  ex, ey, ez: E at the cell; ey_x1, ez_x1:
-// E at the +x neighbour (already mixed across a Bloch x seam).
-template <class T, bool SEAM, bool WRITE>
-__device__ __forceinline__ HCell<T> h_cell(const Geom<T>& g, const HSys<T>& s, int x, int y, int z, int p,
-                                           size_t b, T ex, T ey, T ez, T ey_x1, T ez_x1) {
-    const int pyp = (y == g.sy - 1) ? p - (g.sy - 1) * g.sz : p + g.sz;
-    const int pzp = (z == g.sz - 1) ? p - (g.sz - 1) : p + 1;
-    T ex_y = s.E[0][b + pyp], ez_y = s.E[2][b + pyp];
-    T ex_z = s.E[0][b + pzp], ey_z = s.E[1][b + pzp];
+// Loads of one cell's H update except E at the cell itself (register queue):
+// E at x+1 (raw), E at p + e_y and p + e_z (seam-mixed), H at the cell.
+template <class T>
+struct HIn {
+    T exn, eyn, ezn, ex_y, ez_y, ex_z, ey_z, hx, hy, hz;
+};
+
+template <class T, bool SEAM>
+__device__ __forceinline__ void h_fetch(const Geom<T>& g, const HSys<T>& s, const Cell& c, int x, uint32_t pbytes,
+                                        HIn<T>& v) {
+    const uint32_t ob = (uint32_t)x * pbytes;
+    const uint32_t on = (x == g.sx - 1) ? 0u : ob + pbytes;
+    v.exn = ld(s.E[0], on + c.op);
+    v.eyn = ld(s.E[1], on + c.op);
+    v.ezn = ld(s.E[2], on + c.op);
+    v.ex_y = ld(s.E[0], ob + c.op_y);
+    v.ez_y = ld(s.E[2], ob + c.op_y);
+    v.ex_z = ld(s.E[0], ob + c.op_z);
+    v.ey_z = ld(s.E[1], ob + c.op_z);
+    v.hx = ld(s.Hin[0], ob + c.op);
+    v.hy = ld(s.Hin[1], ob + c.op);
+    v.hz = ld(s.Hin[2], ob + c.op);
     if (SEAM) {
-        if ((g.seam_mask & 2) && y == g.sy - 1) {
-            ex_y = seam_mix(ex_y, s.P[0][b + pyp], s.sa[1], s.sb[1]);
-            ez_y = seam_mix(ez_y, s.P[2][b + pyp], s.sa[1], s.sb[1]);
+        if ((g.seam_mask & 2) && c.y == g.sy - 1) {
+            v.ex_y = seam_mix(v.ex_y, ld(s.P[0], ob + c.op_y), s.sa[1], s.sb[1]);
+            v.ez_y = seam_mix(v.ez_y, ld(s.P[2], ob + c.op_y), s.sa[1], s.sb[1]);
         }
-        if ((g.seam_mask & 4) && z == g.sz - 1) {
-            ex_z = seam_mix(ex_z, s.P[0][b + pzp], s.sa[2], s.sb[2]);
-            ey_z = seam_mix(ey_z, s.P[1][b + pzp], s.sa[2], s.sb[2]);
+        if ((g.seam_mask & 4) && c.z == g.sz - 1) {
+            v.ex_z = seam_mix(v.ex_z, ld(s.P[0], ob + c.op_z), s.sa[2], s.sb[2]);
+            v.ey_z = seam_mix(v.ey_z, ld(s.P[1], ob + c.op_z), s.sa[2], s.sb[2]);
         }
     }
-    T ix = g.inv_u, iy = g.inv_u, iz = g.inv_u;
-    if (g.inv_e[0]) { ix = g.inv_e[0][x]; iy = g.inv_e[1][y]; iz = g.inv_e[2][z]; }
-    T dxy = (ex_y - ex) * iy;
-    T dxz = (ex_z - ex) * iz;
-    T dyx = (ey_x1 - ey) * ix;
-    T dyz = (ey_z - ey) * iz;
-    T dzx = (ez_x1 - ez) * ix;
-    T dzy = (ez_y - ez) * iy;
+}
+
+// The H update of cell (x, c).  ex, ey, ez: E at the cell.  Stores H_out and
+// psi_out when write is set.
+template <class T, bool SEAM>
+__device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s, const Cell& c, int x,
+                                             uint32_t pbytes, T ex, T ey, T ez, const HIn<T>& v, T ix, T iy,
+                                             T iz, bool write) {
+    T ey1 = v.eyn, ez1 = v.ezn;
+    if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
+        ey1 = seam_mix(ey1, ld(s.P[1], c.op), s.sa[0], s.sb[0]);
+        ez1 = seam_mix(ez1, ld(s.P[2], c.op), s.sa[0], s.sb[0]);
+    }
+    T dxy = (v.ex_y - ex) * iy;
+    T dxz = (v.ex_z - ex) * iz;
+    T dyx = (ey1 - ey) * ix;
+    T dyz = (v.ey_z - ey) * iz;
+    T dzx = (ez1 - ez) * ix;
+    T dzy = (v.ez_y - ez) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
-    cpml_cell<T, false, WRITE>(g, x, y, z, p, s.psi_in, s.psi_out, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    cpml_cell<T, false>(g, x, c.y, c.z, c.p, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py,
+                        pz);
     HCell<T> r;
-    const size_t i = b + p;
-    r.hx0 = s.Hin[0][i];
-    r.hy0 = s.Hin[1][i];
-    r.hz0 = s.Hin[2][i];
-    r.hx1 = r.hx0 - g.dt * (dzy - dyz - px);
-    r.hy1 = r.hy0 - g.dt * (dxz - dzx - py);
-    r.hz1 = r.hz0 - g.dt * (dyx - dxy - pz);
-    if (WRITE) {
-        s.Hout[0][i] = r.hx1;
-        s.Hout[1][i] = r.hy1;
-        s.Hout[2][i] = r.hz1;
+    r.hx0 = v.hx;
+    r.hy0 = v.hy;
+    r.hz0 = v.hz;
+    r.hx1 = v.hx - g.dt * (dzy - dyz - px);
+    r.hy1 = v.hy - g.dt * (dxz - dzx - py);
+    r.hz1 = v.hz - g.dt * (dyx - dxy - pz);
+    if (write) {
+        const uint32_t oi = (uint32_t)x * pbytes + c.op;
+        st(s.Hout[0], oi, r.hx1);
+        st(s.Hout[1], oi, r.hy1);
+        st(s.Hout[2], oi, r.hz1);
     }
-    r.aExy = ex_y + ex; r.aExz = ex_z + ex;
-    r.aEyx = ey_x1 + ey; r.aEyz = ey_z + ey;
-    r.aEzx = ez_x1 + ez; r.aEzy = ez_y + ez;
+    r.aExy = v.ex_y + ex; r.aExz = v.ex_z + ex;
+    r.aEyx = ey1 + ey; r.aEyz = v.ey_z + ey;
+    r.aEzx = ez1 + ez; r.aEzy = v.ez_y + ez;
     return r;
 }
 
-// E values of cell (x, p) and of its +x neighbour, with the Bloch x seam.
+// The H update of an arbitrary cell, evaluated without storing anything.
 template <class T, bool SEAM>
-__device__ __forceinline__ void load_e_pair(const Geom<T>& g, const HSys<T>& s, int x, int p, T& ex, T& ey,
-                                            T& ez, T& ey1, T& ez1) {
-    const size_t b = (size_t)x * g.plane + p;
-    const int xn = (x == g.sx - 1) ? 0 : x + 1;
-    const size_t bn = (size_t)xn * g.plane + p;
-    ex = s.E[0][b];
-    ey = s.E[1][b];
-    ez = s.E[2][b];
-    ey1 = s.E[1][bn];
-    ez1 = s.E[2][bn];
-    if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
-        ey1 = seam_mix(ey1, s.P[1][bn], s.sa[0], s.sb[0]);
-        ez1 = seam_mix(ez1, s.P[2][bn], s.sa[0], s.sb[0]);
-    }
+__device__ __forceinline__ HCell<T> h_eval(const Geom<T>& g, const HSys<T>& s, int x, int p, uint32_t pbytes) {
+    const Cell c = make_cell<T>(g, p);
+    HIn<T> v;
+    h_fetch<T, SEAM>(g, s, c, x, pbytes, v);
+    const uint32_t oi = (uint32_t)x * pbytes + c.op;
+    const T ex = ld(s.E[0], oi), ey = ld(s.E[1], oi), ez = ld(s.E[2], oi);
+    const T ix = g.inv_e[0] ? g.inv_e[0][x] : g.inv_u;
+    const T iy = g.inv_e[1] ? g.inv_e[1][c.y] : g.inv_u;
+    const T iz = g.inv_e[2] ? g.inv_e[2][c.z] : g.inv_u;
+    return h_update<T, SEAM>(g, s, c, x, pbytes, ex, ey, ez, v, ix, iy, iz, false);
 }
 
-// MODE: 0 = no Poynting, 1 = write oS (two-pass), 2 = fused S
-template <class T, bool PAIR, bool SEAM, int MODE>
-__global__ __launch_bounds__(kBlock) void h_march(HArgs<T> a) {
+// MODE: 0 = no Poynting, 1 = write oS (two-pass), 2 = fused S with an LDS
+// exchange, 3 = fused S with the in-plane neighbours re-evaluated per cell
+// (any sz).
+//
+// MODE 2: a workgroup of HB threads owns g.tile plane cells, a whole number
+// of rows (tile = k*sz), plus one halo row in front: the row before its first
+// row, taken cyclically, so workgroup 0's halo is row sy-1.  Every -y and -z
+// neighbour of an owned cell (including the periodic wraps y=0 -> sy-1 and
+// z=0 -> sz-1) is then evaluated by a thread of the same workgroup.
+//
+// Fused modes start the march one plane early: that first iteration only
+// evaluates the oS products of plane x0-1 (the -x neighbours of plane x0).
+template <class T, bool PAIR, bool SEAM, int MODE, int HB>
+__global__ __launch_bounds__(HB) void h_march(HArgs<T> a) {
+    constexpr bool LDSX = (MODE == 2);
+    constexpr bool FUSED = (MODE == 2 || MODE == 3);
+    __shared__ T lds[LDSX ? 2 : 1][4][LDSX ? HB : 1];
     const Geom<T>& g = a.g;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= g.plane) return;
-    const int y = p / g.sz;
-    const int z = p - y * g.sz;
-    const int ym = (y == 0) ? g.sy - 1 : y - 1;
-    const int zm = (z == 0) ? g.sz - 1 : z - 1;
-    const int pym = ym * g.sz + z;
-    const int pzm = y * g.sz + zm;
-    const int x0 = blockIdx.y * g.xchunk;
+    int pb, ck;
+    block_coords(g.pblocks, g.nchunk, &pb, &ck);
+    const int t = threadIdx.x;
+    const int halo = LDSX ? g.sz : 0;
+    const int praw = pb * g.tile - halo + t;
+    const bool valid = (t < g.tile + halo) && praw < g.plane;
+    if (!LDSX && !valid) return;   // no workgroup barrier outside the LDS mode
+    const int p = praw < 0 ? praw + g.plane : praw;
+    const bool own = valid && t >= halo;
+    const Cell c = make_cell<T>(g, valid ? p : 0);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const int x0 = ck * g.xchunk;
     const int x1 = min(x0 + g.xchunk, g.sx);
     const T sf = g.dt * (T)0.125;
+    const T iy = g.inv_e[1] ? g.inv_e[1][c.y] : g.inv_u;
+    const T iz = g.inv_e[2] ? g.inv_e[2][c.z] : g.inv_u;
+    constexpr int NS = PAIR ? 2 : 1;
+    const int xs = FUSED ? (x0 == 0 ? g.sx - 1 : x0 - 1) : x0;
+    const int n_it = (x1 - x0) + (FUSED ? 1 : 0);
 
-    // register queue: E of the current plane
-    T ex = a.s[0].E[0][(size_t)x0 * g.plane + p];
-    T ey = a.s[0].E[1][(size_t)x0 * g.plane + p];
-    T ez = a.s[0].E[2][(size_t)x0 * g.plane + p];
-    T fx = (T)0, fy = (T)0, fz = (T)0;
-    if (PAIR) {
-        fx = a.s[1].E[0][(size_t)x0 * g.plane + p];
-        fy = a.s[1].E[1][(size_t)x0 * g.plane + p];
-        fz = a.s[1].E[2][(size_t)x0 * g.plane + p];
+    // register queue: E (and F) at the current plane
+    T qx[NS], qy[NS], qz[NS];
+    HIn<T> cur[NS], nxt[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const uint32_t ob = (uint32_t)xs * pbytes;
+        qx[k] = ld(a.s[k].E[0], ob + c.op);
+        qy[k] = ld(a.s[k].E[1], ob + c.op);
+        qz[k] = ld(a.s[k].E[2], ob + c.op);
+        h_fetch<T, SEAM>(g, a.s[k], c, xs, pbytes, cur[k]);
     }
-    // oS of the -x neighbour (fused S)
     T oSyx_m = (T)0, oSzx_m = (T)0;
-    if (MODE == 2) {
-        const int xp = (x0 == 0) ? g.sx - 1 : x0 - 1;
-        T qx, qy, qz, qy1, qz1;
-        load_e_pair<T, SEAM>(g, a.s[0], xp, p, qx, qy, qz, qy1, qz1);
-        const HCell<T> c = h_cell<T, SEAM, false>(g, a.s[0], xp, y, z, p, (size_t)xp * g.plane, qx, qy, qz,
-                                                  qy1, qz1);
-        const T aHy = c.hy1 + c.hy0, aHz = c.hz1 + c.hz0;
-        oSyx_m = -c.aExy * aHz;
-        oSzx_m = c.aExz * aHy;
-    }
-    for (int x = x0; x < x1; ++x) {
-        const size_t b = (size_t)x * g.plane;
+    int x = xs;
+    for (int it = 0; it < n_it; ++it) {
         const int xn = (x == g.sx - 1) ? 0 : x + 1;
-        const size_t bn = (size_t)xn * g.plane + p;
+        if (it + 1 < n_it) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, xn, pbytes, nxt[k]);
+        }
+        const bool dry = FUSED && it == 0;   // uniform
+        const T ix = g.inv_e[0] ? g.inv_e[0][x] : g.inv_u;
+        const uint32_t oi = (uint32_t)x * pbytes + c.op;
         // ---- system 0 (E -> H), with Poynting
-        {
-            const HSys<T>& s = a.s[0];
-            const T exn = s.E[0][bn], eyn = s.E[1][bn], ezn = s.E[2][bn];
-            T ey1 = eyn, ez1 = ezn;
-            if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
-                ey1 = seam_mix(ey1, s.P[1][bn], s.sa[0], s.sb[0]);
-                ez1 = seam_mix(ez1, s.P[2][bn], s.sa[0], s.sb[0]);
-            }
-            const HCell<T> c = h_cell<T, SEAM, true>(g, s, x, y, z, p, b, ex, ey, ez, ey1, ez1);
+        T oSxy = (T)0, oSxz = (T)0, oSyz = (T)0, oSyx = (T)0, oSzx = (T)0, oSzy = (T)0;
+        if (valid) {
+            const HCell<T> h = h_update<T, SEAM>(g, a.s[0], c, x, pbytes, qx[0], qy[0], qz[0], cur[0], ix, iy,
+                                                 iz, own && !dry);
             if (MODE != 0) {
-                const T aHx = c.hx1 + c.hx0, aHy = c.hy1 + c.hy0, aHz = c.hz1 + c.hz0;
-                const T oSxy = c.aEyx * aHz;
-                const T oSxz = -c.aEzx * aHy;
-                const T oSyz = c.aEzy * aHx;
-                const T oSyx = -c.aExy * aHz;
-                const T oSzx = c.aExz * aHy;
-                const T oSzy = -c.aEyz * aHx;
-                const size_t i = b + p;
-                if (MODE == 1) {
-                    a.oS[0][i] = oSxy; a.oS[1][i] = oSyz; a.oS[2][i] = oSzx;
-                    a.oS[3][i] = oSxz; a.oS[4][i] = oSyx; a.oS[5][i] = oSzy;
-                } else {
-                    // -y neighbour: oSxy, oSzy; -z neighbour: oSxz, oSyz
-                    T qx, qy, qz, qy1, qz1;
-                    load_e_pair<T, SEAM>(g, s, x, pym, qx, qy, qz, qy1, qz1);
-                    const HCell<T> cy = h_cell<T, SEAM, false>(g, s, x, ym, z, pym, b, qx, qy, qz, qy1, qz1);
-                    load_e_pair<T, SEAM>(g, s, x, pzm, qx, qy, qz, qy1, qz1);
-                    const HCell<T> cz = h_cell<T, SEAM, false>(g, s, x, y, zm, pzm, b, qx, qy, qz, qy1, qz1);
-                    const T oSxy_ym = cy.aEyx * (cy.hz1 + cy.hz0);
-                    const T oSzy_ym = -cy.aEyz * (cy.hx1 + cy.hx0);
-                    const T oSxz_zm = -cz.aEzx * (cz.hy1 + cz.hy0);
-                    const T oSyz_zm = cz.aEzy * (cz.hx1 + cz.hx0);
-                    a.S[0][i] = (oSxy + oSxz + oSxy_ym + oSxz_zm) * sf;
-                    a.S[1][i] = (oSyz + oSyx + oSyz_zm + oSyx_m) * sf;
-                    a.S[2][i] = (oSzx + oSzy + oSzx_m + oSzy_ym) * sf;
-                    oSyx_m = oSyx;
-                    oSzx_m = oSzx;
+                const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+                oSxy = h.aEyx * aHz;
+                oSxz = -h.aEzx * aHy;
+                oSyz = h.aEzy * aHx;
+                oSyx = -h.aExy * aHz;
+                oSzx = h.aExz * aHy;
+                oSzy = -h.aEyz * aHx;
+            }
+            if (MODE == 1) {
+                st(a.oS[0], oi, oSxy); st(a.oS[1], oi, oSyz); st(a.oS[2], oi, oSzx);
+                st(a.oS[3], oi, oSxz); st(a.oS[4], oi, oSyx); st(a.oS[5], oi, oSzy);
+            }
+        }
+        if (FUSED && !dry) {
+            // oSxy, oSzy of the -y neighbour; oSxz, oSyz of the -z neighbour
+            T nxy = (T)0, nzy = (T)0, nxz = (T)0, nyz = (T)0;
+            if (LDSX) {
+                const int b = it & 1;
+                lds[b][0][t] = oSxy;
+                lds[b][1][t] = oSzy;
+                lds[b][2][t] = oSxz;
+                lds[b][3][t] = oSyz;
+                __syncthreads();
+                if (own) {
+                    nxy = lds[b][0][t - g.sz];
+                    nzy = lds[b][1][t - g.sz];
+                    const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
+                    nxz = lds[b][2][tz];
+                    nyz = lds[b][3][tz];
                 }
+            } else if (own) {
+                const int pym = (c.y == 0) ? p + (g.sy - 1) * g.sz : p - g.sz;
+                const HCell<T> hy = h_eval<T, SEAM>(g, a.s[0], x, pym, pbytes);
+                nxy = hy.aEyx * (hy.hz1 + hy.hz0);
+                nzy = -hy.aEyz * (hy.hx1 + hy.hx0);
+                const int pzm = (c.z == 0) ? p + g.sz - 1 : p - 1;
+                const HCell<T> hz = h_eval<T, SEAM>(g, a.s[0], x, pzm, pbytes);
+                nxz = -hz.aEzx * (hz.hy1 + hz.hy0);
+                nyz = hz.aEzy * (hz.hx1 + hz.hx0);
             }
-            ex = exn; ey = eyn; ez = ezn;
+            if (own) {
+                st(a.S[0], oi, (oSxy + oSxz + nxy + nxz) * sf);
+                st(a.S[1], oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+                st(a.S[2], oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            }
         }
+        oSyx_m = oSyx;
+        oSzx_m = oSzx;
         // ---- system 1 (F -> G), no Poynting
-        if (PAIR) {
-            const HSys<T>& s = a.s[1];
-            const T fxn = s.E[0][bn], fyn = s.E[1][bn], fzn = s.E[2][bn];
-            T fy1 = fyn, fz1 = fzn;
-            if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
-                fy1 = seam_mix(fy1, s.P[1][bn], s.sa[0], s.sb[0]);
-                fz1 = seam_mix(fz1, s.P[2][bn], s.sa[0], s.sb[0]);
-            }
-            (void)h_cell<T, SEAM, true>(g, s, x, y, z, p, b, fx, fy, fz, fy1, fz1);
-            fx = fxn; fy = fyn; fz = fzn;
+        if (PAIR && own && !dry) {
+            (void)h_update<T, SEAM>(g, a.s[1], c, x, pbytes, qx[1], qy[1], qz[1], cur[1], ix, iy, iz, true);
         }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            qx[k] = cur[k].exn;
+            qy[k] = cur[k].eyn;
+            qz[k] = cur[k].ezn;
+            cur[k] = nxt[k];
+        }
+        x = xn;
     }
 }
 
@@ -484,16 +634,17 @@ __global__ __launch_bounds__(kBlock) void s_from_os(HArgs<T> a) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= g.plane) return;
     const int x = blockIdx.y;
-    const int y = p / g.sz;
-    const int z = p - y * g.sz;
-    const size_t i = (size_t)x * g.plane + p;
-    const size_t imy = (size_t)x * g.plane + ((y == 0) ? p + (g.sy - 1) * g.sz : p - g.sz);
-    const size_t imz = (size_t)x * g.plane + ((z == 0) ? p + g.sz - 1 : p - 1);
-    const size_t imx = (size_t)((x == 0) ? g.sx - 1 : x - 1) * g.plane + p;
+    const Cell c = make_cell<T>(g, p);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const uint32_t ob = (uint32_t)x * pbytes;
+    const uint32_t i = ob + c.op;
+    const uint32_t imy = ob + c.om_y;
+    const uint32_t imz = ob + c.om_z;
+    const uint32_t imx = (uint32_t)((x == 0) ? g.sx - 1 : x - 1) * pbytes + c.op;
     const T sf = g.dt * (T)0.125;
-    a.S[0][i] = (a.oS[0][i] + a.oS[3][i] + a.oS[0][imy] + a.oS[3][imz]) * sf;
-    a.S[1][i] = (a.oS[1][i] + a.oS[4][i] + a.oS[1][imz] + a.oS[4][imx]) * sf;
-    a.S[2][i] = (a.oS[2][i] + a.oS[5][i] + a.oS[2][imx] + a.oS[5][imy]) * sf;
+    st(a.S[0], i, (ld(a.oS[0], i) + ld(a.oS[3], i) + ld(a.oS[0], imy) + ld(a.oS[3], imz)) * sf);
+    st(a.S[1], i, (ld(a.oS[1], i) + ld(a.oS[4], i) + ld(a.oS[1], imz) + ld(a.oS[4], imx)) * sf);
+    st(a.S[2], i, (ld(a.oS[2], i) + ld(a.oS[5], i) + ld(a.oS[2], imx) + ld(a.oS[5], imy)) * sf);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,11 +674,15 @@ __global__ void add_point(T* f, const T* amp) {
 // Host side
 // ---------------------------------------------------------------------------
 
+struct Launch {
+    int block, tile, pblocks, nchunk, xchunk, mode;
+};
+
 struct Plan {
     fdtd_desc d;
     int sx, sy, sz, plane;
     int64_t N;
-    int xchunk, nchunk, pblocks;
+    Launch le, lh;
     int64_t psi_elems, coef_elems;
     struct F { int lo, hi, psi_off, face_elems, coef_off; } face[FDTD_MAX_FACES];
 };
@@ -538,10 +693,27 @@ int64_t plane_of(const Plan& P, int axis) {
     return (int64_t)P.sx * P.sy;
 }
 
+Launch make_launch(int plane, int sx, int block, int tile, int mode, int target) {
+    Launch L;
+    L.block = block;
+    L.tile = tile;
+    L.mode = mode;
+    L.pblocks = (plane + tile - 1) / tile;
+    int nchunk = (target + L.pblocks - 1) / L.pblocks;
+    if (nchunk > sx) nchunk = sx;
+    if (nchunk < 1) nchunk = 1;
+    L.xchunk = (sx + nchunk - 1) / nchunk;
+    L.nchunk = (sx + L.xchunk - 1) / L.xchunk;
+    return L;
+}
+
 template <class T>
-void fill_geom(const Plan& P, const fdtd_fields* f, Geom<T>& g) {
+void fill_geom(const Plan& P, const Launch& L, const fdtd_fields* f, Geom<T>& g) {
     g.sx = P.sx; g.sy = P.sy; g.sz = P.sz; g.plane = P.plane;
-    g.xchunk = P.xchunk;
+    g.xchunk = L.xchunk;
+    g.nchunk = L.nchunk;
+    g.pblocks = L.pblocks;
+    g.tile = L.tile;
     g.seam_mask = (P.d.bloch_mode == FDTD_BLOCH_PHASE) ? P.d.bloch_mask : 0;
     g.dt = (T)P.d.dt;
     g.inv_u = (T)P.d.inv_dx;
@@ -566,15 +738,19 @@ void fill_geom(const Plan& P, const fdtd_fields* f, Geom<T>& g) {
         fd.thick = P.d.face[k].thickness;
         fd.lo = P.face[k].lo;
         fd.hi = P.face[k].hi;
-        fd.psi_off = (int)P.face[k].psi_off;
-        fd.face_elems = (int)P.face[k].face_elems;
+        fd.psi_off = P.face[k].psi_off;
+        fd.face_elems = P.face[k].face_elems;
         fd.coef_off = P.face[k].coef_off;
     }
 }
 
 template <class T>
-void comp3(void* base, int64_t N, T* out[3]) {
+void comp3(const void* base, int64_t N, T* out[3]) {
     for (int k = 0; k < 3; ++k) out[k] = base ? (T*)base + k * N : nullptr;
+}
+template <class T>
+void comp3c(const void* base, int64_t N, const T* out[3]) {
+    for (int k = 0; k < 3; ++k) out[k] = base ? (const T*)base + k * N : nullptr;
 }
 
 int check_launch(const char* what) {
@@ -587,25 +763,23 @@ template <class T>
 int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index) {
     EArgs<T> a;
     memset(&a, 0, sizeof(a));
-    fill_geom<T>(P, f, a.g);
+    fill_geom<T>(P, P.le, f, a.g);
     const int64_t N = P.N;
-    T* tmp[3];
-    comp3<T>(f->eps, N, tmp);
-    for (int k = 0; k < 3; ++k) a.eps[k] = tmp[k];
+    comp3c<T>(f->eps, N, a.eps);
     const bool pair = P.d.bloch_mask != 0;
     const double* re = P.d.bloch_re;
     const double* im = P.d.bloch_im;
     // system 0: E from H, partner G; seam H(-1) = (H + iG)(c - is) -> real: Hc + Gs
     comp3<T>(f->E, N, a.s[0].E);
-    comp3<T>(f->H, N, tmp); for (int k = 0; k < 3; ++k) a.s[0].H[k] = tmp[k];
-    comp3<T>(pair ? f->G : nullptr, N, tmp); for (int k = 0; k < 3; ++k) a.s[0].P[k] = tmp[k];
+    comp3c<T>(f->H, N, a.s[0].H);
+    comp3c<T>(pair ? f->G : nullptr, N, a.s[0].P);
     a.s[0].psi = (T*)f->psi_e;
     for (int r = 0; r < 3; ++r) { a.s[0].sa[r] = (T)re[r]; a.s[0].sb[r] = (T)im[r]; }
     if (pair) {
         // system 1: F from G, partner H; imaginary part: Gc - Hs
         comp3<T>(f->F, N, a.s[1].E);
-        comp3<T>(f->G, N, tmp); for (int k = 0; k < 3; ++k) a.s[1].H[k] = tmp[k];
-        comp3<T>(f->H, N, tmp); for (int k = 0; k < 3; ++k) a.s[1].P[k] = tmp[k];
+        comp3c<T>(f->G, N, a.s[1].H);
+        comp3c<T>(f->H, N, a.s[1].P);
         a.s[1].psi = (T*)f->psi_f;
         for (int r = 0; r < 3; ++r) { a.s[1].sa[r] = (T)re[r]; a.s[1].sb[r] = (T)(-im[r]); }
     }
@@ -617,7 +791,7 @@ int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_a
         a.src_x = (int)(i / P.plane);
         a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
     }
-    dim3 grid(P.pblocks, P.nchunk);
+    const dim3 grid(P.le.pblocks * P.le.nchunk);
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
     if (!pair) hipLaunchKernelGGL((e_march<T, false, false>), grid, dim3(kBlock), 0, st, a);
     else if (!seam) hipLaunchKernelGGL((e_march<T, true, false>), grid, dim3(kBlock), 0, st, a);
@@ -626,55 +800,58 @@ int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_a
 }
 
 template <class T, bool PAIR, bool SEAM>
-void launch_h_mode(int mode, dim3 grid, hipStream_t st, const HArgs<T>& a) {
-    if (mode == 0) hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 0>), grid, dim3(kBlock), 0, st, a);
-    else if (mode == 1) hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 1>), grid, dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2>), grid, dim3(kBlock), 0, st, a);
+void launch_h_mode(const Launch& L, hipStream_t st, const HArgs<T>& a) {
+    const dim3 grid(L.pblocks * L.nchunk);
+    switch (L.mode) {
+        case 0: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 0, kBlock>), grid, dim3(kBlock), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 1, kBlock>), grid, dim3(kBlock), 0, st, a); break;
+        case 2:
+            if (L.block == 512) hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 512>), grid, dim3(512), 0, st, a);
+            else hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 1024>), grid, dim3(1024), 0, st, a);
+            break;
+        default: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 3, kBlock>), grid, dim3(kBlock), 0, st, a); break;
+    }
 }
 
 template <class T>
-int fill_h(const Plan& P, const fdtd_fields* f, HArgs<T>& a) {
+void fill_h(const Plan& P, const Launch& L, const fdtd_fields* f, HArgs<T>& a) {
     memset(&a, 0, sizeof(a));
-    fill_geom<T>(P, f, a.g);
+    fill_geom<T>(P, L, f, a.g);
     const int64_t N = P.N;
     const bool pair = P.d.bloch_mask != 0;
     const double* re = P.d.bloch_re;
     const double* im = P.d.bloch_im;
-    T* tmp[3];
     // system 0: H from E, partner F; seam E(s) = (E + iF)(c + is) -> real: Ec - Fs
-    comp3<T>(f->E, N, tmp); for (int k = 0; k < 3; ++k) a.s[0].E[k] = tmp[k];
-    comp3<T>(f->H, N, tmp); for (int k = 0; k < 3; ++k) a.s[0].Hin[k] = tmp[k];
+    comp3c<T>(f->E, N, a.s[0].E);
+    comp3c<T>(f->H, N, a.s[0].Hin);
     comp3<T>(f->H_out ? f->H_out : f->H, N, a.s[0].Hout);
-    comp3<T>(pair ? f->F : nullptr, N, tmp); for (int k = 0; k < 3; ++k) a.s[0].P[k] = tmp[k];
+    comp3c<T>(pair ? f->F : nullptr, N, a.s[0].P);
     a.s[0].psi_in = (const T*)f->psi_h;
     a.s[0].psi_out = (T*)(f->psi_h_out ? f->psi_h_out : f->psi_h);
     for (int r = 0; r < 3; ++r) { a.s[0].sa[r] = (T)re[r]; a.s[0].sb[r] = (T)(-im[r]); }
     if (pair) {
         // system 1: G from F, partner E; imaginary part: Fc + Es
-        comp3<T>(f->F, N, tmp); for (int k = 0; k < 3; ++k) a.s[1].E[k] = tmp[k];
-        comp3<T>(f->G, N, tmp); for (int k = 0; k < 3; ++k) a.s[1].Hin[k] = tmp[k];
+        comp3c<T>(f->F, N, a.s[1].E);
+        comp3c<T>(f->G, N, a.s[1].Hin);
         comp3<T>(f->G_out ? f->G_out : f->G, N, a.s[1].Hout);
-        comp3<T>(f->E, N, tmp); for (int k = 0; k < 3; ++k) a.s[1].P[k] = tmp[k];
+        comp3c<T>(f->E, N, a.s[1].P);
         a.s[1].psi_in = (const T*)f->psi_g;
         a.s[1].psi_out = (T*)(f->psi_g_out ? f->psi_g_out : f->psi_g);
         for (int r = 0; r < 3; ++r) { a.s[1].sa[r] = (T)re[r]; a.s[1].sb[r] = (T)im[r]; }
     }
     comp3<T>(f->S, N, a.S);
     for (int k = 0; k < 6; ++k) a.oS[k] = f->oS ? (T*)f->oS + k * N : nullptr;
-    return 0;
 }
 
 template <class T>
 int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     HArgs<T> a;
-    fill_h<T>(P, f, a);
-    dim3 grid(P.pblocks, P.nchunk);
-    const int mode = P.d.poynting == FDTD_POYNTING_FUSED ? 2 : (P.d.poynting == FDTD_POYNTING_TWO_PASS ? 1 : 0);
+    fill_h<T>(P, P.lh, f, a);
     const bool pair = P.d.bloch_mask != 0;
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
-    if (!pair) launch_h_mode<T, false, false>(mode, grid, st, a);
-    else if (!seam) launch_h_mode<T, true, false>(mode, grid, st, a);
-    else launch_h_mode<T, true, true>(mode, grid, st, a);
+    if (!pair) launch_h_mode<T, false, false>(P.lh, st, a);
+    else if (!seam) launch_h_mode<T, true, false>(P.lh, st, a);
+    else launch_h_mode<T, true, true>(P.lh, st, a);
     return check_launch("fdtd_update_h");
 }
 
@@ -719,10 +896,12 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     const fdtd_desc& d = *desc;
     for (int r = 0; r < 3; ++r)
         if (d.shape[r] < 1) return fail(1, "shape[%d] = %lld must be >= 1", r, (long long)d.shape[r]);
-    const int64_t N = d.shape[0] * d.shape[1] * d.shape[2];
-    if (N > (int64_t)INT32_MAX || d.shape[1] * d.shape[2] > (int64_t)INT32_MAX)
-        return fail(1, "grid of %lld cells exceeds the 2^31-1 cells per component supported", (long long)N);
     if (d.dtype != FDTD_F32 && d.dtype != FDTD_F64) return fail(1, "dtype %d unsupported (0=f32, 1=f64)", d.dtype);
+    const int64_t N = d.shape[0] * d.shape[1] * d.shape[2];
+    const int64_t esz = d.dtype == FDTD_F32 ? 4 : 8;
+    // every per-component byte offset must fit the 32-bit offsets of the kernels
+    if (N * esz > (int64_t)UINT32_MAX)
+        return fail(1, "grid of %lld cells exceeds 4 GiB per field component", (long long)N);
     if (d.n_face < 0 || d.n_face > FDTD_MAX_FACES) return fail(1, "n_face %d out of [0, %d]", d.n_face, FDTD_MAX_FACES);
     if (!(d.dt > 0)) return fail(1, "Invalid dt: %g", d.dt);
     if (d.poynting < 0 || d.poynting > 2) return fail(1, "poynting mode %d unknown", d.poynting);
@@ -750,21 +929,30 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         if (f.polarity == 0) { pf.lo = 0; pf.hi = f.thickness < s ? f.thickness : s; }
         else if (f.thickness <= s) { pf.lo = s - f.thickness; pf.hi = s; }
         else { pf.lo = 0; pf.hi = 0; }   // the reference's size_t test is false everywhere
-        pf.face_elems = plane_of(*P, f.axis) * f.thickness;
-        pf.psi_off = psi_off;
-        psi_off += 2 * pf.face_elems;
+        pf.face_elems = (int)(plane_of(*P, f.axis) * f.thickness);
+        pf.psi_off = (int)psi_off;
+        psi_off += 2 * (int64_t)pf.face_elems;
         pf.coef_off = (int)coef_off;
         coef_off += 6 * f.thickness;
+        if (psi_off * esz > (int64_t)UINT32_MAX) { delete P; return fail(1, "psi block exceeds 4 GiB"); }
     }
-    if (psi_off > (int64_t)INT32_MAX) { delete P; return fail(1, "psi block too large"); }
     P->psi_elems = psi_off;
     P->coef_elems = coef_off;
-    P->pblocks = (P->plane + kBlock - 1) / kBlock;
-    int nchunk = (kTargetBlocks + P->pblocks - 1) / P->pblocks;
-    if (nchunk > P->sx) nchunk = P->sx;
-    if (nchunk < 1) nchunk = 1;
-    P->xchunk = (P->sx + nchunk - 1) / nchunk;
-    P->nchunk = (P->sx + P->xchunk - 1) / P->xchunk;
+    P->le = make_launch(P->plane, P->sx, kBlock, kBlock, 0, kTargetBlocks);
+    if (d.poynting == FDTD_POYNTING_FUSED) {
+        // LDS exchange: 512 threads for rows up to 128 cells, 1024 up to
+        // 341, else per-cell re-evaluation of the in-plane neighbours.
+        // whole rows per workgroup plus one halo row
+        if (P->sz <= 128)
+            P->lh = make_launch(P->plane, P->sx, 512, ((512 - P->sz) / P->sz) * P->sz, 2, kTargetBlocks / 2);
+        else if (P->sz <= 341)
+            P->lh = make_launch(P->plane, P->sx, 1024, ((1024 - P->sz) / P->sz) * P->sz, 2, kTargetBlocks / 4);
+        else
+            P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, 3, kTargetBlocks);
+    } else {
+        P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, d.poynting == FDTD_POYNTING_TWO_PASS ? 1 : 0,
+                            kTargetBlocks);
+    }
     *plan_out = P;
     return 0;
 }
@@ -801,14 +989,14 @@ int fdtd_update_s(const void* plan, const fdtd_fields* f, void* stream) {
     if (P.d.poynting != FDTD_POYNTING_TWO_PASS) return 0;
     if (!f || !f->S || !f->oS) return fail(2, "S and oS are required");
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid(P.pblocks, P.sx);
+    const dim3 grid((P.plane + kBlock - 1) / kBlock, P.sx);
     if (P.d.dtype == FDTD_F32) {
         HArgs<float> a;
-        fill_h<float>(P, f, a);
+        fill_h<float>(P, P.lh, f, a);
         hipLaunchKernelGGL(s_from_os<float>, grid, dim3(kBlock), 0, st, a);
     } else {
         HArgs<double> a;
-        fill_h<double>(P, f, a);
+        fill_h<double>(P, P.lh, f, a);
         hipLaunchKernelGGL(s_from_os<double>, grid, dim3(kBlock), 0, st, a);
     }
     return check_launch("fdtd_update_s");

@@ -1,0 +1,160 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports what
+include/fdtd.h declares, plan validation, coefficient agreement with the
+oracle, workload builders, and the loud failure without a GPU."""
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "fdtd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdtd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+    declared = _header_functions()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.EXPORTS)
+    assert lib.fdtd_abi_version() == _lib.ABI_VERSION
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of fdtd_desc / fdtd_fields have the C sizes."""
+    from opencl_fdtd_amd import _lib
+    # fdtd_face: 4 int32; fdtd_desc: 3 int64 + 2 int32 + 12 faces + 2 double + 4 int32 + 6 double
+    assert ctypes.sizeof(_lib.Face) == 16
+    assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
+    assert ctypes.sizeof(_lib.Fields) == 17 * 8
+
+
+def _desc(shape=(10, 11, 12), faces=(), **kw):
+    from opencl_fdtd_amd import _lib
+    d = _lib.Desc()
+    d.shape[:] = list(shape)
+    d.dtype = kw.get("dtype", 0)
+    d.n_face = len(faces)
+    for k, (a, p, t) in enumerate(faces):
+        d.face[k].axis, d.face[k].polarity, d.face[k].thickness = a, p, t
+    d.dt = kw.get("dt", 0.5)
+    d.inv_dx = 1.0
+    d.poynting = kw.get("poynting", 1)
+    d.bloch_mask = kw.get("bloch_mask", 0)
+    d.bloch_mode = kw.get("bloch_mode", 0)
+    return d
+
+
+def _create(d):
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    rc = lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan))
+    msg = lib.fdtd_last_error().decode()
+    if rc == 0:
+        psi = lib.fdtd_psi_elems(plan)
+        coef = lib.fdtd_coef_elems(plan)
+        lib.fdtd_plan_destroy(plan)
+        return rc, msg, psi, coef
+    return rc, msg, None, None
+
+
+def test_plan_sizes():
+    faces = [(a, p, 8) for a in range(3) for p in range(2)]
+    rc, _, psi, coef = _create(_desc((20, 30, 40), faces))
+    assert rc == 0
+    # psi: two arrays per face of T * (area of the face)
+    assert psi == 2 * 8 * 2 * (30 * 40 + 20 * 40 + 20 * 30)
+    assert coef == 6 * 8 * 6
+
+
+@pytest.mark.parametrize("bad, needle", [
+    (dict(dt=0.0), "dt"),
+    (dict(dt=-1.0), "dt"),
+    (dict(dtype=2), "dtype"),
+    (dict(poynting=7), "poynting"),
+    (dict(bloch_mask=9), "bloch_mask"),
+    (dict(bloch_mode=5), "bloch_mode"),
+])
+def test_plan_rejects_bad_descriptions(bad, needle):
+    rc, msg, _, _ = _create(_desc(**bad))
+    assert rc != 0 and needle in msg
+
+
+def test_plan_rejects_bad_faces():
+    assert _create(_desc(faces=[(3, 0, 4)]))[0] != 0
+    assert _create(_desc(faces=[(0, 2, 4)]))[0] != 0
+    assert _create(_desc(faces=[(0, 0, 0)]))[0] != 0
+    rc, msg, _, _ = _create(_desc(faces=[(0, 0, 2), (0, 1, 2), (0, 0, 3)]))
+    assert rc != 0 and "more than two" in msg
+    d = _desc((0, 4, 4))
+    assert _create(d)[0] != 0
+
+
+def test_plan_rejects_grids_beyond_32bit_offsets():
+    rc, msg, _, _ = _create(_desc((1100, 1000, 1000)))
+    assert rc != 0 and "4 GiB" in msg
+
+
+def test_coefficients_match_oracle_bitwise():
+    from opencl_fdtd_amd.pml import face_coefficients, with_defaults
+    from oracle.oracle import pml_coefficients, pml_defaults
+    for dtype in (numpy.float32, numpy.float64):
+        for pml in ({"axis": "x", "polarity": "n"}, {"axis": "z", "polarity": "p", "thickness": 16},
+                    {"axis": "y", "polarity": "n", "cfs_alpha": 0.05, "m": 3, "ma": 2, "epsilon_eff": 2.0}):
+            a = face_coefficients(with_defaults(dict(pml)), 0.5715, dtype)
+            b = pml_coefficients(pml_defaults(dict(pml)), 0.5715, dtype)
+            for x, y in zip(a, b):
+                assert x.dtype == dtype
+                assert numpy.array_equal(x, y)
+
+
+def test_simulation_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from opencl_fdtd_amd import Simulation, _lib
+    eps = [numpy.ones((4, 4, 4))] * 3
+    with pytest.raises(_lib.FdtdError):
+        Simulation(eps, [])
+
+
+def test_type_to_c():
+    from opencl_fdtd_amd import type_to_C
+    assert type_to_C(numpy.float32) == "float"
+    assert type_to_C(numpy.float64) == "double"
+    assert type_to_C(numpy.float16) == "half"
+    with pytest.raises(Exception):
+        type_to_C(numpy.int32)
+
+
+def test_workloads():
+    from opencl_fdtd_amd import workloads as W
+    w = W.small(steps=10)
+    assert w.shape == (64, 64, 64) and len(w.pmls) == 6
+    e = w.epsilon[0]
+    assert e.min() == 1.0 and e.max() <= 12 and (e > 1).mean() > 0.05
+    assert 0 <= w.src_index < 3 * w.cells
+    s = W.phc_slab(steps=10)
+    assert s.shape == (395, 345, 73)
+    exx = s.epsilon[0]
+    slab = exx[:, :, 31:42]
+    assert numpy.all(exx[:, :, :31] == 1) and numpy.all(exx[:, :, 42:] == 1)
+    fill = (slab == 12.25).mean()
+    assert 0.5 < fill < 0.9              # holes of r = 7 on a pitch-20 lattice
+    # L3 defect: the centre cell of the slab is dielectric
+    assert exx[197, 172, 36] == 12.25 and s.epsilon[2][197, 172, 36] == 11.9
+    b = W.bloch(steps=10)
+    assert numpy.isinf(b.epsilon[0][:, :, 200:205]).all()
+    assert abs(numpy.isinf(b.epsilon[0]).mean() - 5 / 256) < 1e-9
+    assert len(b.bloch_boundaries) == 2 and [p["axis"] for p in b.pmls] == ["z", "z"]
+    bytes_c2 = W.algorithmic_bytes_per_cell(s.shape, s.pmls)
+    assert abs(bytes_c2 - 105.8) < 0.1
