@@ -42,6 +42,21 @@ def kernel_bytes(shape, pmls, kind, dtype_bytes=4, poynting=True):
     return dtype_bytes * (words * n + 4 * slab)
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/traffic_<workload>.md): FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the factor the same counter reads for a copy
+    kernel of known size.  None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().strip("|").split("|")
+        if len(parts) >= 2 and parts[0].strip() == "hbm_bytes_per_launch " + kernel:
+            return float(parts[1])
+    return None
+
+
 def cpu_baseline(w, seconds: float):
     from oracle.oracle import OracleSimulation
     cores = len(os.sched_getaffinity(0))
@@ -109,7 +124,15 @@ def main():
     gcell = w.cells * args.steps / (ms / 1e3) / 1e9
     print("bench: timed %.1f ms (wall %.3f s) -> %.2f Gcell/s" % (ms, wall, gcell), file=sys.stderr, flush=True)
 
-    # kernel-by-kernel timing (HIP events on the launch stream)
+    # kernel-by-kernel timing (HIP events on the launch stream).  An event
+    # pair around a launch also times the marker packets themselves: that
+    # overhead is measured with empty pairs and subtracted.
+    empty = []
+    for _ in range(50):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        b.record(stream)
+        empty.append((a, b))
     dur = {"E": [], "H": []}
     evs = []
     for _ in range(args.kernel_steps):
@@ -121,15 +144,17 @@ def main():
         c.record(stream)
         evs.append((a, b, c))
     torch.cuda.synchronize()
+    marker_ms = float(numpy.median([a.elapsed_time(b) for a, b in empty]))
     for a, b, c in evs:
-        dur["E"].append(a.elapsed_time(b))
-        dur["H"].append(b.elapsed_time(c))
+        dur["E"].append(a.elapsed_time(b) - marker_ms)
+        dur["H"].append(b.elapsed_time(c) - marker_ms)
     dbytes = 4 if dtype == numpy.float32 else 8
     kern = {}
     for k in ("E", "H"):
         avg_ms = float(numpy.mean(dur[k]))
         byts = kernel_bytes(w.shape, w.pmls, k, dbytes, w.do_poynting)
-        kern[k] = {"avg_us": avg_ms * 1e3, "bytes": byts, "GBs": byts / (avg_ms / 1e3) / 1e9}
+        kern[k] = {"avg_us": avg_ms * 1e3, "bytes": byts, "GBs": byts / (avg_ms / 1e3) / 1e9,
+                   "marker_us_subtracted": marker_ms * 1e3}
     dom = max(kern, key=lambda k: kern[k]["avg_us"])
     print("bench: kernels %s" % json.dumps(kern), file=sys.stderr, flush=True)
 
@@ -153,7 +178,8 @@ def main():
                    "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "kernel": "h_march (H+S)" if dom == "H" else "e_march (E)",
                      "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(w.name, "h_march" if dom == "H" else "e_march"),
                      "avg_us": kern[dom]["avg_us"], "bytes_per_launch": kern[dom]["bytes"],
                      "other_kernel": {k: v for k, v in kern.items() if k != dom}},
     }

@@ -7,8 +7,8 @@
 // chunk of planes:
 //   * the -x (E step) / +x (H step) neighbour values stay in registers from
 //     the previous plane (register queue);
-//   * the loads of plane x+1 are issued before plane x is computed
-//     (software pipelining), so every wave keeps ~a dozen loads in flight;
+//   * latency is hidden by occupancy (>= 6 waves per SIMD) rather than by
+//     software pipelining, which costs more VGPRs than it gains here;
 //   * 64 consecutive p of a wavefront give fully coalesced 256-byte accesses
 //     for every component whatever sz is; in-plane neighbours (p - sz, p - 1,
 //     p + sz, p + 1) are re-read through L1/L2;
@@ -46,8 +46,36 @@
 
 namespace {
 
+// Launch-shape tuning knobs (compile-time; the defaults are the measured best
+// on MI355X for the README-scale 395x345x73 grid).
+#ifndef FDTD_TARGET_BLOCKS
+#define FDTD_TARGET_BLOCKS 8192      // workgroups per launch (256 CUs, several each)
+#endif
+#ifndef FDTD_PREFETCH
+#define FDTD_PREFETCH 0              // 1: issue the loads of plane x+1 before computing plane x
+                                     // (costs VGPRs/occupancy; slower on MI355X)
+#endif
+#ifndef FDTD_NT_E
+#define FDTD_NT_E 0                  // non-temporal hint on the streamed accesses of the E kernel
+#endif
+#ifndef FDTD_NT_H
+#define FDTD_NT_H 0                  // same for the H kernel (1 re-reads the E plane from HBM)
+#endif
+#ifndef FDTD_XCD
+#define FDTD_XCD 1                   // XCD-contiguous workgroup order
+#endif
+#ifndef FDTD_HB_SMALL
+#define FDTD_HB_SMALL 512            // fused-S H workgroup for rows of <= 128 cells
+#endif
+#ifndef FDTD_E_WAVES
+#define FDTD_E_WAVES 1               // __launch_bounds__ min waves per SIMD, E kernel
+#endif
+#ifndef FDTD_H_WAVES
+#define FDTD_H_WAVES 6               // same, H kernel
+#endif
+
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
-constexpr int kTargetBlocks = 4096;  // workgroups per launch (256 CUs, several each)
+constexpr int kTargetBlocks = FDTD_TARGET_BLOCKS;
 
 thread_local std::string g_last_error;
 
@@ -141,15 +169,35 @@ template <class T>
 __device__ __forceinline__ void st(T* base, uint32_t boff, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+// streamed accesses (each value touched once per launch); NT selects the
+// non-temporal cache policy
+template <bool NT, class T>
+__device__ __forceinline__ T lds_(const T* base, uint32_t boff) {
+    if (NT)
+        return __builtin_nontemporal_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff));
+    return ld(base, boff);
+}
+template <bool NT, class T>
+__device__ __forceinline__ void sts_(T* base, uint32_t boff, T v) {
+    if (NT)
+        __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff));
+    else
+        st(base, boff, v);
+}
 
 // Workgroup -> (plane tile, x chunk).  Hardware workgroup b runs on XCD
 // b % 8; logical tiles are handed out so each XCD gets a contiguous range.
 __device__ __forceinline__ void block_coords(int pblocks, int nchunk, int* pb, int* ck) {
     const int nb = pblocks * nchunk;
     const int b = blockIdx.x;
+#if FDTD_XCD
     const int q = nb >> 3, r = nb & 7;
     const int xcd = b & 7, j = b >> 3;
     const int lb = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
+#else
+    const int lb = b;
+    (void)nb;
+#endif
     *pb = lb % pblocks;
     *ck = lb / pblocks;
 }
@@ -285,16 +333,16 @@ struct EIn {
 template <class T, bool SEAM>
 __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, const Cell& c, uint32_t ob,
                                         EIn<T>& v) {
-    v.hx = ld(s.H[0], ob + c.op);
-    v.hy = ld(s.H[1], ob + c.op);
-    v.hz = ld(s.H[2], ob + c.op);
+    v.hx = lds_<FDTD_NT_E>(s.H[0], ob + c.op);
+    v.hy = lds_<FDTD_NT_E>(s.H[1], ob + c.op);
+    v.hz = lds_<FDTD_NT_E>(s.H[2], ob + c.op);
     v.hx_ym = ld(s.H[0], ob + c.om_y);
     v.hz_ym = ld(s.H[2], ob + c.om_y);
     v.hx_zm = ld(s.H[0], ob + c.om_z);
     v.hy_zm = ld(s.H[1], ob + c.om_z);
-    v.ex = ld(s.E[0], ob + c.op);
-    v.ey = ld(s.E[1], ob + c.op);
-    v.ez = ld(s.E[2], ob + c.op);
+    v.ex = lds_<FDTD_NT_E>(s.E[0], ob + c.op);
+    v.ey = lds_<FDTD_NT_E>(s.E[1], ob + c.op);
+    v.ez = lds_<FDTD_NT_E>(s.E[2], ob + c.op);
     if (SEAM) {
         if ((g.seam_mask & 2) && c.y == 0) {
             v.hx_ym = seam_mix(v.hx_ym, ld(s.P[0], ob + c.om_y), s.sa[1], s.sb[1]);
@@ -325,13 +373,13 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, con
     if (src_comp == 0) ex = ex + amp;
     else if (src_comp == 1) ey = ey + amp;
     else if (src_comp == 2) ez = ez + amp;
-    st(s.E[0], oi, ex);
-    st(s.E[1], oi, ey);
-    st(s.E[2], oi, ez);
+    sts_<FDTD_NT_E>(s.E[0], oi, ex);
+    sts_<FDTD_NT_E>(s.E[1], oi, ey);
+    sts_<FDTD_NT_E>(s.E[2], oi, ez);
 }
 
 template <class T, bool PAIR, bool SEAM>
-__global__ __launch_bounds__(kBlock) void e_march(EArgs<T> a) {
+__global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     const Geom<T>& g = a.g;
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
@@ -366,19 +414,26 @@ __global__ __launch_bounds__(kBlock) void e_march(EArgs<T> a) {
         const uint32_t ob = (uint32_t)x0 * pbytes;
 #pragma unroll
         for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, ob, cur[k]);
-        e0 = ld(a.eps[0], ob + c.op);
-        e1 = ld(a.eps[1], ob + c.op);
-        e2 = ld(a.eps[2], ob + c.op);
+        e0 = lds_<FDTD_NT_E>(a.eps[0], ob + c.op);
+        e1 = lds_<FDTD_NT_E>(a.eps[1], ob + c.op);
+        e2 = lds_<FDTD_NT_E>(a.eps[2], ob + c.op);
     }
     for (int x = x0; x < x1; ++x) {
         const uint32_t ob = (uint32_t)x * pbytes;
-        if (x + 1 < x1) {
+        if (!FDTD_PREFETCH && x > x0) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, ob, cur[k]);
+            e0 = lds_<FDTD_NT_E>(a.eps[0], ob + c.op);
+            e1 = lds_<FDTD_NT_E>(a.eps[1], ob + c.op);
+            e2 = lds_<FDTD_NT_E>(a.eps[2], ob + c.op);
+        }
+        if (FDTD_PREFETCH && x + 1 < x1) {
             const uint32_t on = ob + pbytes;
 #pragma unroll
             for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, on, nxt[k]);
-            n0 = ld(a.eps[0], on + c.op);
-            n1 = ld(a.eps[1], on + c.op);
-            n2 = ld(a.eps[2], on + c.op);
+            n0 = lds_<FDTD_NT_E>(a.eps[0], on + c.op);
+            n1 = lds_<FDTD_NT_E>(a.eps[1], on + c.op);
+            n2 = lds_<FDTD_NT_E>(a.eps[2], on + c.op);
         }
         const T ix = g.inv_h[0] ? g.inv_h[0][x] : g.inv_u;
         int src_comp = -1;
@@ -393,9 +448,9 @@ __global__ __launch_bounds__(kBlock) void e_march(EArgs<T> a) {
                         k == 0 ? src_comp : -1, amp);
             hy_m[k] = cur[k].hy;
             hz_m[k] = cur[k].hz;
-            cur[k] = nxt[k];
+            if (FDTD_PREFETCH) cur[k] = nxt[k];
         }
-        e0 = n0; e1 = n1; e2 = n2;
+        if (FDTD_PREFETCH) { e0 = n0; e1 = n1; e2 = n2; }
     }
 }
 
@@ -421,16 +476,16 @@ __device__ __forceinline__ void h_fetch(const Geom<T>& g, const HSys<T>& s, cons
                                         HIn<T>& v) {
     const uint32_t ob = (uint32_t)x * pbytes;
     const uint32_t on = (x == g.sx - 1) ? 0u : ob + pbytes;
-    v.exn = ld(s.E[0], on + c.op);
-    v.eyn = ld(s.E[1], on + c.op);
-    v.ezn = ld(s.E[2], on + c.op);
+    v.exn = lds_<FDTD_NT_H>(s.E[0], on + c.op);
+    v.eyn = lds_<FDTD_NT_H>(s.E[1], on + c.op);
+    v.ezn = lds_<FDTD_NT_H>(s.E[2], on + c.op);
     v.ex_y = ld(s.E[0], ob + c.op_y);
     v.ez_y = ld(s.E[2], ob + c.op_y);
     v.ex_z = ld(s.E[0], ob + c.op_z);
     v.ey_z = ld(s.E[1], ob + c.op_z);
-    v.hx = ld(s.Hin[0], ob + c.op);
-    v.hy = ld(s.Hin[1], ob + c.op);
-    v.hz = ld(s.Hin[2], ob + c.op);
+    v.hx = lds_<FDTD_NT_H>(s.Hin[0], ob + c.op);
+    v.hy = lds_<FDTD_NT_H>(s.Hin[1], ob + c.op);
+    v.hz = lds_<FDTD_NT_H>(s.Hin[2], ob + c.op);
     if (SEAM) {
         if ((g.seam_mask & 2) && c.y == g.sy - 1) {
             v.ex_y = seam_mix(v.ex_y, ld(s.P[0], ob + c.op_y), s.sa[1], s.sb[1]);
@@ -472,9 +527,9 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s,
     r.hz1 = v.hz - g.dt * (dyx - dxy - pz);
     if (write) {
         const uint32_t oi = (uint32_t)x * pbytes + c.op;
-        st(s.Hout[0], oi, r.hx1);
-        st(s.Hout[1], oi, r.hy1);
-        st(s.Hout[2], oi, r.hz1);
+        sts_<FDTD_NT_H>(s.Hout[0], oi, r.hx1);
+        sts_<FDTD_NT_H>(s.Hout[1], oi, r.hy1);
+        sts_<FDTD_NT_H>(s.Hout[2], oi, r.hz1);
     }
     r.aExy = v.ex_y + ex; r.aExz = v.ex_z + ex;
     r.aEyx = ey1 + ey; r.aEyz = v.ey_z + ey;
@@ -509,7 +564,7 @@ __device__ __forceinline__ HCell<T> h_eval(const Geom<T>& g, const HSys<T>& s, i
 // Fused modes start the march one plane early: that first iteration only
 // evaluates the oS products of plane x0-1 (the -x neighbours of plane x0).
 template <class T, bool PAIR, bool SEAM, int MODE, int HB>
-__global__ __launch_bounds__(HB) void h_march(HArgs<T> a) {
+__global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
     constexpr bool LDSX = (MODE == 2);
     constexpr bool FUSED = (MODE == 2 || MODE == 3);
     __shared__ T lds[LDSX ? 2 : 1][4][LDSX ? HB : 1];
@@ -549,7 +604,11 @@ __global__ __launch_bounds__(HB) void h_march(HArgs<T> a) {
     int x = xs;
     for (int it = 0; it < n_it; ++it) {
         const int xn = (x == g.sx - 1) ? 0 : x + 1;
-        if (it + 1 < n_it) {
+        if (!FDTD_PREFETCH && it > 0) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, x, pbytes, cur[k]);
+        }
+        if (FDTD_PREFETCH && it + 1 < n_it) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, xn, pbytes, nxt[k]);
         }
@@ -603,9 +662,9 @@ __global__ __launch_bounds__(HB) void h_march(HArgs<T> a) {
                 nyz = hz.aEzy * (hz.hx1 + hz.hx0);
             }
             if (own) {
-                st(a.S[0], oi, (oSxy + oSxz + nxy + nxz) * sf);
-                st(a.S[1], oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
-                st(a.S[2], oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+                sts_<FDTD_NT_H>(a.S[0], oi, (oSxy + oSxz + nxy + nxz) * sf);
+                sts_<FDTD_NT_H>(a.S[1], oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+                sts_<FDTD_NT_H>(a.S[2], oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
             }
         }
         oSyx_m = oSyx;
@@ -619,7 +678,7 @@ __global__ __launch_bounds__(HB) void h_march(HArgs<T> a) {
             qx[k] = cur[k].exn;
             qy[k] = cur[k].eyn;
             qz[k] = cur[k].ezn;
-            cur[k] = nxt[k];
+            if (FDTD_PREFETCH) cur[k] = nxt[k];
         }
         x = xn;
     }
@@ -806,7 +865,8 @@ void launch_h_mode(const Launch& L, hipStream_t st, const HArgs<T>& a) {
         case 0: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 0, kBlock>), grid, dim3(kBlock), 0, st, a); break;
         case 1: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 1, kBlock>), grid, dim3(kBlock), 0, st, a); break;
         case 2:
-            if (L.block == 512) hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 512>), grid, dim3(512), 0, st, a);
+            if (L.block == FDTD_HB_SMALL && FDTD_HB_SMALL != 1024)
+                hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, FDTD_HB_SMALL>), grid, dim3(FDTD_HB_SMALL), 0, st, a);
             else hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 1024>), grid, dim3(1024), 0, st, a);
             break;
         default: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 3, kBlock>), grid, dim3(kBlock), 0, st, a); break;
@@ -944,7 +1004,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         // 341, else per-cell re-evaluation of the in-plane neighbours.
         // whole rows per workgroup plus one halo row
         if (P->sz <= 128)
-            P->lh = make_launch(P->plane, P->sx, 512, ((512 - P->sz) / P->sz) * P->sz, 2, kTargetBlocks / 2);
+            P->lh = make_launch(P->plane, P->sx, FDTD_HB_SMALL, ((FDTD_HB_SMALL - P->sz) / P->sz) * P->sz, 2,
+                                kTargetBlocks * kBlock / FDTD_HB_SMALL);
         else if (P->sz <= 341)
             P->lh = make_launch(P->plane, P->sx, 1024, ((1024 - P->sz) / P->sz) * P->sz, 2, kTargetBlocks / 4);
         else
