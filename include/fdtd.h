@@ -25,6 +25,9 @@
  *                         (fdtd.py:168-169), with f(t) read from device memory
  *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
  *                         (E, F, source, H, G, S) without host round trips
+ *   fdtd_adjoint_step, fdtd_plane_sum, fdtd_energy, fdtd_reduce_partials
+ *                      <- no reference counterpart (gradients and FP64
+ *                         reductions over the same step; see below)
  *
  * Field layout (kernels/common.cl:34-61, fdfd_tools.vec at simulation.py:347):
  * a vector field is 3 contiguous components of N = sx*sy*sz values each, and
@@ -139,6 +142,52 @@ int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp
  * roles every step: after an odd n the newest H is in f->H_out. */
 int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
                 const void* src_wave, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Discrete adjoint of the step and FP64 reductions.  No counterpart in the
+ * reference: they serve gradient-based design (d objective / d eps) on top
+ * of the same step.  Supported for uniform dx, no Bloch boundaries and
+ * non-overlapping CPML slabs.
+ *
+ * Objective: J = sum over steps of sum over the plane {coordinate of axis
+ * obj_axis == obj_index} of S_{obj_axis}.  One adjoint step reverses one
+ * forward step (E step, source, H step with S), given the forward fields
+ * before and after it.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    const void* E_prev;   /* E before the step's E update */
+    const void* E_next;   /* E after the E update and the source */
+    const void* H_prev;   /* H before the step's H update */
+    const void* H_next;   /* H after the H update */
+    const void* eps;
+    void* Eb;             /* adjoint of E: in = after the step, out = before (in place) */
+    void* Hb;             /* adjoint of H, same convention */
+    const void* psib_e;   /* adjoint of psi_E after the step */
+    void* psib_e_out;     /* adjoint of psi_E before the step (distinct buffer) */
+    const void* psib_h;
+    void* psib_h_out;
+    const void* coef;     /* the forward coefficient block */
+    double* eps_grad;     /* 3N doubles, accumulated d J / d eps (component-wise) */
+    int64_t src_index;    /* forward point source index, or -1 */
+    double src_amp;       /* value added at src_index after the E update */
+    int32_t obj_axis;     /* 0..2, or -1 for no objective term this step */
+    int32_t obj_index;
+} fdtd_adjoint_bufs;
+
+/* Elements of one adjoint psi block (same as fdtd_psi_elems). */
+int fdtd_adjoint_step(const void* plan, const fdtd_adjoint_bufs* b, void* stream);
+
+/* acc[0] += sum of field component comp over the plane {axis coordinate ==
+ * index}, accumulated in FP64 in a fixed order (deterministic). */
+int fdtd_plane_sum(const void* plan, const void* field, int32_t comp, int32_t axis, int64_t index,
+                   double* acc, void* stream);
+
+/* Electromagnetic energy sum(eps E.E) + sum(H.H_prev) in FP64 (the quantity
+ * the leapfrog step conserves without sources and losses); partials must hold
+ * fdtd_reduce_partials() doubles; the result is written to out[0]. */
+int64_t fdtd_reduce_partials(const void* plan);
+int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
+                double* partials, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ EXPORTS = (
     "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy",
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
+    "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
 )
 
 
@@ -47,6 +48,14 @@ class Fields(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "E", "H", "H_out", "eps", "S", "oS", "F", "G", "G_out",
         "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d")]
+
+
+class AdjointBufs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "E_prev", "E_next", "H_prev", "H_next", "eps", "Eb", "Hb",
+        "psib_e", "psib_e_out", "psib_h", "psib_h_out", "coef", "eps_grad")] + [
+        ("src_index", ctypes.c_int64), ("src_amp", ctypes.c_double),
+        ("obj_axis", ctypes.c_int32), ("obj_index", ctypes.c_int32)]
 
 
 class FdtdError(RuntimeError):
@@ -84,8 +93,13 @@ def load() -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_int64), P]
     lib.fdtd_add_point.argtypes = [P, P, ctypes.c_int64, P, P]
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_adjoint_step.argtypes = [P, ctypes.POINTER(AdjointBufs), P]
+    lib.fdtd_plane_sum.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_reduce_partials.argtypes = [P]
+    lib.fdtd_reduce_partials.restype = ctypes.c_int64
+    lib.fdtd_energy.argtypes = [P, P, P, P, P, P, P, P]
     for name in EXPORTS:
-        if name not in ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems"):
+        if name not in ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials"):
             getattr(lib, name).restype = ctypes.c_int
     v = lib.fdtd_abi_version()
     if v != ABI_VERSION:

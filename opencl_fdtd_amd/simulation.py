@@ -387,6 +387,40 @@ class Simulation:
             self._h = 1 - self._h
         return Event(stream)
 
+    # ------------------------------------------------------------------ state
+    def checkpoint(self) -> Dict[str, torch.Tensor]:
+        """Copies of everything the step reads: E, H, psi (and F, G with
+        Bloch).  ``restore`` puts them back, so stepping resumes bit for bit
+        (``initial_fields`` restores only E/H/F/G, as in the reference)."""
+        st = {"E": self._E.clone(), "H": self._H[self._h].clone(), "psi_e": self._psi_e.clone(),
+              "psi_h": self._psi_h[self._h if len(self._psi_h) == 2 else 0].clone()}
+        if self._F is not None:
+            st.update(F=self._F.clone(), G=self._G.clone(), psi_f=self._psi_f.clone(), psi_g=self._psi_g.clone())
+        return st
+
+    def restore(self, st: Dict[str, torch.Tensor]) -> None:
+        self._E.copy_(st["E"])
+        self._H[self._h].copy_(st["H"])
+        self._psi_e.copy_(st["psi_e"])
+        self._psi_h[self._h if len(self._psi_h) == 2 else 0].copy_(st["psi_h"])
+        if self._F is not None:
+            self._F.copy_(st["F"])
+            self._G.copy_(st["G"])
+            self._psi_f.copy_(st["psi_f"])
+            self._psi_g.copy_(st["psi_g"])
+
+    @property
+    def plan(self) -> ctypes.c_void_p:
+        return self._plan
+
+    @property
+    def coef(self) -> torch.Tensor:
+        return self._coef
+
+    @property
+    def psi_elems(self) -> int:
+        return int(self._lib.fdtd_psi_elems(self._plan))
+
     def __del__(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:
