@@ -156,6 +156,48 @@ def test_fp64_reductions():
                                 ctypes.c_void_p(hp.data_ptr()), ctypes.c_void_p(sim.eps.data_ptr()),
                                 ctypes.c_void_p(parts.data_ptr()), ctypes.c_void_p(out.data_ptr()), st) == 0
     E = sim.E.get().astype(numpy.float64)
-    want = float(numpy.sum(numpy.concatenate([e.ravel() for e in eps]) * E * E)
+    epsv = sim.eps.cpu().numpy().astype(numpy.float64)     # eps as the FP32 simulation holds it
+    want = float(numpy.sum(epsv * E * E)
                  + numpy.sum(sim.H.get().astype(numpy.float64) * hp.cpu().numpy().astype(numpy.float64)))
     assert abs(float(out.item()) - want) <= 1e-12 * abs(want)
+
+
+def test_adjoint_config_128():
+    """128^3, CPML-8, 2000 steps, checkpoints every 100, flux through z = 100,
+    eps ~ U[1,4] in a 32^3 design region: the FP32 adjoint gradient on 16
+    seeded design cells against FP32 central differences (relative step
+    1e-3) and FP64 central differences (relative step 1e-6)."""
+    torch = _torch()
+    from opencl_fdtd_amd import workloads
+    from opencl_fdtd_amd.adjoint import FluxAdjoint, FluxProblem, flux
+    w = workloads.adjoint()
+    eps = torch.as_tensor(numpy.array(w.epsilon), device="cuda")
+    c = w.shape[0] // 2
+    design = (slice(c - 16, c + 16), slice(c - 16, c + 16), slice(40, 72))
+    mk = lambda ft: FluxProblem(shape=w.shape, pmls=w.pmls, steps=w.steps, src_index=w.src_index,
+                                src_wave=w.src_wave, axis=2, index=100, dt=w.dt, checkpoint_every=100,
+                                float_type=ft, design=design)
+    p32, p64 = mk(numpy.float32), mk(numpy.float64)
+    print("case adjoint_128 forward+adjoint", flush=True)
+    J, grad = FluxAdjoint(p32, eps).gradient()
+    g = grad.sum(dim=0).cpu().numpy()
+    rng = numpy.random.default_rng(4)
+    cells = [tuple(int(rng.integers(s.start, s.stop)) for s in design) for _ in range(16)]
+    rows = []
+    for cell in cells:
+        fd = {}
+        for name, p, rel in (("f32", p32, 1e-3), ("f64", p64, 1e-6)):
+            h = rel * float(eps[(0,) + cell])
+            ep, em = eps.clone(), eps.clone()
+            for a in range(3):
+                ep[(a,) + cell] += h
+                em[(a,) + cell] -= h
+            fd[name] = (flux(p, ep) - flux(p, em)) / (2 * h)
+        rows.append((cell, g[cell], fd["f32"], fd["f64"]))
+        print("cell", cell, "adjoint f32 %.6e  FD f32 %.6e  FD f64 %.6e" % rows[-1][1:], flush=True)
+    gmax = max(abs(r[3]) for r in rows)
+    err_adj = max(abs(r[1] - r[3]) for r in rows) / gmax
+    err_fd32 = max(abs(r[2] - r[3]) for r in rows) / gmax
+    print("J %.9e  max|adjoint - FD64|/max|g| %.3e  max|FD32 - FD64|/max|g| %.3e" % (J, err_adj, err_fd32),
+          flush=True)
+    assert err_adj < 1e-3
