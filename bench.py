@@ -79,6 +79,104 @@ def cpu_baseline(w, seconds: float):
                       "%.1f s on %s" % (steps, "x".join(map(str, w.shape)), el, platform.processor() or "host")}
 
 
+def _run_fields(w, dtype, steps, backend):
+    """Runs the workload for ``steps`` steps in the reference caller-loop
+    order; backend 'gpu' (this library) or 'oracle' (CPU, test
+    infrastructure).  Returns the final fields as float64 numpy arrays."""
+    pmls = [dict(p) for p in w.pmls]
+    if backend == "gpu":
+        import torch
+        from opencl_fdtd_amd import Simulation
+        sim = Simulation(w.epsilon, pmls, dt=w.dt, float_type=dtype, bloch_boundaries=w.bloch_boundaries,
+                         do_poynting=w.do_poynting)
+    else:
+        from oracle.oracle import OracleSimulation
+        sim = OracleSimulation(w.epsilon, pmls, dt=w.dt, float_type=dtype, bloch_boundaries=w.bloch_boundaries,
+                               do_poynting=w.do_poynting, nthreads=min(len(os.sched_getaffinity(0)), 16))
+    ps = w.plane_source
+    if ps is not None:
+        z, comp = ps["z"], ps["comp"]
+        pr = numpy.real(ps["phase"]).astype(dtype)
+        pi = numpy.imag(ps["phase"]).astype(dtype)
+        if backend == "gpu":
+            pr_t, pi_t = torch.as_tensor(pr, device="cuda"), torch.as_tensor(pi, device="cuda")
+    for t in range(steps):
+        if backend == "gpu":
+            sim.update_E([])
+        else:
+            sim.update_E()
+            if sim.bloch:
+                sim.update_F()
+        if ps is not None:
+            a = dtype(ps["wave"][t])
+            if backend == "gpu":
+                Ev = sim.E.view(3, *w.shape)
+                Fv = sim.F.view(3, *w.shape)
+                Ev[comp, :, :, z] += pr_t * float(a)
+                Fv[comp, :, :, z] += pi_t * float(a)
+            else:
+                Ev = sim.E.reshape(3, *w.shape)
+                Fv = sim.F.reshape(3, *w.shape)
+                Ev[comp, :, :, z] += pr * a
+                Fv[comp, :, :, z] += pi * a
+        elif w.src_wave is not None:
+            if backend == "gpu":
+                sim.E[w.src_index] += float(dtype(w.src_wave[t]))
+            else:
+                sim.E[w.src_index] += dtype(w.src_wave[t])
+        if backend == "gpu":
+            sim.update_H([])
+        else:
+            sim.update_H()
+            if sim.bloch:
+                sim.update_G()
+            if sim.do_poynting:
+                sim.update_S()
+    out = {}
+    for k in ("E", "H", "S", "F", "G"):
+        v = getattr(sim, k, None)
+        if v is None:
+            continue
+        out[k] = (v.get() if backend == "gpu" else v).astype(numpy.float64)
+    return out
+
+
+def _rel(a, b, n):
+    res = []
+    for k in range(3):
+        x, y = a[k * n:(k + 1) * n], b[k * n:(k + 1) * n]
+        d = numpy.linalg.norm(y)
+        res.append(float(numpy.linalg.norm(x - y) / d) if d > 0 else float(numpy.linalg.norm(x - y)))
+    return res
+
+
+def precision_report():
+    """Per configuration: GPU FP32 vs GPU FP64 after the full step count
+    (error of the FP32 path against an FP64 run), and GPU FP32 vs the FP32
+    oracle (bit-exactness) after ``oracle_steps``."""
+    from opencl_fdtd_amd import workloads
+    cases = [("small", workloads.small(), 1000), ("phc_slab", workloads.phc_slab(), 300),
+             ("bloch", workloads.bloch(), 4000), ("adjoint", workloads.adjoint(), 2000),
+             ("learned_pad16", workloads.learned(16, steps=500), 100),
+             ("learned_pad8", workloads.learned(8, steps=500), 100)]
+    rows = []
+    for name, w, oracle_steps in cases:
+        t0 = time.perf_counter()
+        f32 = _run_fields(w, numpy.float32, w.steps, "gpu")
+        f64 = _run_fields(w, numpy.float64, w.steps, "gpu")
+        vs64 = {k: _rel(f32[k], f64[k], w.cells) for k in f32}
+        g32 = _run_fields(w, numpy.float32, oracle_steps, "gpu")
+        o32 = _run_fields(w, numpy.float32, oracle_steps, "oracle")
+        vso = {k: _rel(g32[k], o32[k], w.cells) for k in g32}
+        exact = all(numpy.array_equal(g32[k], o32[k]) for k in g32)
+        row = {"config": name, "shape": list(w.shape), "steps": w.steps, "fp32_vs_fp64_relL2": vs64,
+               "oracle_steps": oracle_steps, "fp32_vs_oracle_fp32_relL2": vso, "bit_exact_vs_oracle": exact,
+               "seconds": round(time.perf_counter() - t0, 1)}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +187,17 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=200, help="steps timed kernel by kernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", action="store_true",
+                    help="per-config FP32 vs FP64 and FP32 vs oracle report instead of the benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.precision:
+        precision_report()
+        return
 
     import torch
     from opencl_fdtd_amd import Simulation
