@@ -167,11 +167,15 @@ typedef struct {
     const void* psib_h;
     void* psib_h_out;
     const void* coef;     /* the forward coefficient block */
-    double* eps_grad;     /* 3N doubles, accumulated d J / d eps (component-wise) */
+    double* eps_grad;     /* 3N doubles, accumulated d J / d eps (component-wise), or NULL */
     int64_t src_index;    /* forward point source index, or -1 */
     double src_amp;       /* value added at src_index after the E update */
     int32_t obj_axis;     /* 0..2, or -1 for no objective term this step */
     int32_t obj_index;
+    int32_t parts;        /* 0 or 3: the whole step; 1: objective + H step only;
+                             2: E step only (lets a caller insert its own
+                             adjoint between the two, e.g. the shell net) */
+    int32_t reserved;
 } fdtd_adjoint_bufs;
 
 /* Elements of one adjoint psi block (same as fdtd_psi_elems). */
@@ -188,6 +192,32 @@ int fdtd_plane_sum(const void* plan, const void* field, int32_t comp, int32_t ax
 int64_t fdtd_reduce_partials(const void* plan);
 int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
                 double* partials, double* out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned absorbing shell.  No counterpart in the reference: a thin shell of
+ * T cells on every face runs the CPML-T update, and after each E update (and
+ * source) a linear 2-layer 3x3x3 convolution (6 -> 16 -> 3 channels: E, H in,
+ * a correction to E out) is added to E on the shell cells.  Six weight sets,
+ * one per face box; BF16 weights and activations, FP32 accumulation.
+ * Weights: fdtd_shell_weight_elems() BF16 values, W1[box][16][6][27] for the
+ * six boxes, then W2[box][3][16][27]; taps t = 9(dx+1) + 3(dy+1) + (dz+1).
+ * ------------------------------------------------------------------------ */
+int64_t fdtd_shell_weight_elems(void);
+/* FP32 scratch for the hidden layer (and, in the backward, its adjoint). */
+int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness);
+/* FP32 scratch for the weight-gradient partials. */
+int64_t fdtd_shell_partials_elems(void);
+/* E += correction(E, H) on the shell cells (E read before it is corrected). */
+int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
+                     float* hidden, void* stream);
+/* Adjoint of E_out = E + correction(E, H): given Ebar (adjoint of E_out),
+ * Ebar_out += Ebar-contribution through the correction's E input (callers
+ * initialise Ebar_out with Ebar for the identity part), Hbar_out += the H
+ * part, and wgrad (FP32, weight layout) += the weight gradient.  E and H are
+ * the correction's inputs. */
+int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
+                        float* hidden, float* hbar, const void* Ebar, void* Ebar_out, void* Hbar_out,
+                        float* partials, float* wgrad, void* stream);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,8 @@ EXPORTS = (
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
+    "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems",
+    "fdtd_shell_apply", "fdtd_shell_backward",
 )
 
 
@@ -55,7 +57,8 @@ class AdjointBufs(ctypes.Structure):
         "E_prev", "E_next", "H_prev", "H_next", "eps", "Eb", "Hb",
         "psib_e", "psib_e_out", "psib_h", "psib_h_out", "coef", "eps_grad")] + [
         ("src_index", ctypes.c_int64), ("src_amp", ctypes.c_double),
-        ("obj_axis", ctypes.c_int32), ("obj_index", ctypes.c_int32)]
+        ("obj_axis", ctypes.c_int32), ("obj_index", ctypes.c_int32),
+        ("parts", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class FdtdError(RuntimeError):
@@ -98,8 +101,17 @@ def load() -> ctypes.CDLL:
     lib.fdtd_reduce_partials.argtypes = [P]
     lib.fdtd_reduce_partials.restype = ctypes.c_int64
     lib.fdtd_energy.argtypes = [P, P, P, P, P, P, P, P]
+    lib.fdtd_shell_weight_elems.argtypes = []
+    lib.fdtd_shell_hidden_elems.argtypes = [P, ctypes.c_int32]
+    lib.fdtd_shell_partials_elems.argtypes = []
+    for name in ("fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems"):
+        getattr(lib, name).restype = ctypes.c_int64
+    lib.fdtd_shell_apply.argtypes = [P, ctypes.c_int32, P, P, P, P, P]
+    lib.fdtd_shell_backward.argtypes = [P, ctypes.c_int32] + [P] * 10 + [P]
+    sized = ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials",
+             "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems")
     for name in EXPORTS:
-        if name not in ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials"):
+        if name not in sized:
             getattr(lib, name).restype = ctypes.c_int
     v = lib.fdtd_abi_version()
     if v != ABI_VERSION:

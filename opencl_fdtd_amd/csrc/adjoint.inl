@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void adj_e(AdjArgs<T> q) {
         if (a == q.src_comp && x == q.src_x && p == q.src_p) en -= (double)q.src_amp;
         const double de = en - (double)q.Ep[a][i];
         const double ep = (double)q.eps[a][i];
-        if (isfinite(ep)) q.grad[a][i] += -(double)q.Eb[a][i] * de / ep;
+        if (q.grad[a] && isfinite(ep)) q.grad[a][i] += -(double)q.Eb[a][i] * de / ep;
     }
 }
 
@@ -287,7 +287,7 @@ int launch_adjoint(const Plan& P, const fdtd_adjoint_bufs* b, hipStream_t st) {
     comp3c<T>(b->eps, N, q.eps);
     comp3<T>(b->Eb, N, q.Eb);
     comp3<T>(b->Hb, N, q.Hb);
-    for (int k = 0; k < 3; ++k) q.grad[k] = b->eps_grad + k * N;
+    for (int k = 0; k < 3; ++k) q.grad[k] = b->eps_grad ? b->eps_grad + k * N : nullptr;
     q.src_comp = -1;
     if (b->src_index >= 0) {
         const int64_t comp = b->src_index / N;
@@ -306,6 +306,8 @@ int launch_adjoint(const Plan& P, const fdtd_adjoint_bufs* b, hipStream_t st) {
         obj_cells = (int)(P.N / s[b->obj_axis]);
     }
     const dim3 ogrid((obj_cells + kBlock - 1) / kBlock);
+    const int parts = (b->parts == 0) ? 3 : b->parts;
+    if (parts & 1) {
     // 1. objective: Ebar and the H_next part of Hbar
     if (obj_cells) {
         q.obj_mode = 1;
@@ -320,10 +322,13 @@ int launch_adjoint(const Plan& P, const fdtd_adjoint_bufs* b, hipStream_t st) {
         q.obj_mode = 2;
         hipLaunchKernelGGL(adj_obj<T>, ogrid, dim3(kBlock), 0, st, q);
     }
-    // 4. E step reversed (the source adds a constant: no adjoint term)
-    q.pin = (const T*)b->psib_e;
-    q.pout = (T*)b->psib_e_out;
-    hipLaunchKernelGGL(adj_e<T>, grid, dim3(kBlock), 0, st, q);
+    }
+    if (parts & 2) {
+        // 4. E step reversed (the source adds a constant: no adjoint term)
+        q.pin = (const T*)b->psib_e;
+        q.pout = (T*)b->psib_e_out;
+        hipLaunchKernelGGL(adj_e<T>, grid, dim3(kBlock), 0, st, q);
+    }
     return check_launch("fdtd_adjoint_step");
 }
 
@@ -334,9 +339,9 @@ extern "C" {
 int fdtd_adjoint_step(const void* plan, const fdtd_adjoint_bufs* b, void* stream) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
-    if (!b || !b->E_prev || !b->E_next || !b->H_prev || !b->H_next || !b->eps || !b->Eb || !b->Hb ||
-        !b->eps_grad)
-        return fail(2, "adjoint: E_prev, E_next, H_prev, H_next, eps, Eb, Hb and eps_grad are required");
+    if (!b || !b->E_prev || !b->E_next || !b->H_prev || !b->H_next || !b->eps || !b->Eb || !b->Hb)
+        return fail(2, "adjoint: E_prev, E_next, H_prev, H_next, eps, Eb and Hb are required");
+    if (b->parts < 0 || b->parts > 3) return fail(3, "adjoint: parts %d not in 0..3", b->parts);
     if (P.d.nonuniform) return fail(4, "adjoint: non-uniform grids are not supported");
     if (P.d.bloch_mask) return fail(4, "adjoint: Bloch boundaries are not supported");
     if (P.d.n_face > 0) {

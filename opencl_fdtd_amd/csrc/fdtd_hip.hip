@@ -1142,3 +1142,5 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
 }  // extern "C"
 
 #include "adjoint.inl"
+
+#include "shell_net.inl"

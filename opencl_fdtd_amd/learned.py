@@ -1,0 +1,414 @@
+"""Thin absorbing boundaries: scenarios, reflection measurement, and the
+learned 4-cell absorber.
+
+A *scenario* is an interior region (eps ~ U[1, 4] in 8^3 blocks within 16
+cells of the faces, 1 elsewhere) driven by a soft dipole with a seeded
+position, polarisation and wavelength U[12, 40] cells.  The same scenario
+can be embedded in domains padded by different absorbers: CPML of thickness
+T on every face, or the learned 4-cell shell.  Reflection of a padded run
+is measured against the same scenario padded by a 48-cell CPML, which
+absorbs far below the levels compared here:
+
+    R = 20 log10( ||E_pad - E_wide|| / ||E_wide|| )
+
+with the norms over the interior cells and over every ``sample_every``-th
+step of the run.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+import numpy
+import torch
+
+from .simulation import Simulation
+
+WIDE_PAD = 48
+
+
+@dataclasses.dataclass
+class Scenario:
+    interior: Tuple[int, int, int]
+    eps: numpy.ndarray             # interior permittivity (sx, sy, sz), isotropic
+    src: Tuple[int, int, int, int]  # (component, x, y, z) in interior coordinates
+    wavelength: float
+    steps: int
+    dt: float = 0.99 / math.sqrt(3)
+    width_wl: float = 1.5           # pulse envelope width in wavelengths (of steps)
+
+    def wave(self) -> numpy.ndarray:
+        """sin(2 pi/wl n dt) exp(-((n - n0)/w)^2 / 2), w = width_wl * wl, n0 = 4 w."""
+        n = numpy.arange(self.steps, dtype=numpy.float64)
+        width = self.width_wl * self.wavelength
+        n0 = 4 * width
+        return numpy.sin(2 * numpy.pi / self.wavelength * n * self.dt) * numpy.exp(-0.5 * ((n - n0) / width) ** 2)
+
+
+def make_scenario(seed: int, interior=(256, 256, 128), steps: int = 600, near_face: int = 16,
+                  src_margin: int = 8, width_wl: float = 1.5) -> Scenario:
+    """eps ~ U[1, 4] per 8^3 block within ``near_face`` cells of a face (1
+    elsewhere); dipole of random polarisation at a random position at least
+    ``src_margin`` cells inside, wavelength U[12, 40]."""
+    rng = numpy.random.default_rng(seed)
+    ix, iy, iz = interior
+    blocks = rng.uniform(1, 4, size=(-(-ix // 8), -(-iy // 8), -(-iz // 8)))
+    eps = numpy.kron(blocks, numpy.ones((8, 8, 8)))[:ix, :iy, :iz]
+    core = numpy.zeros(interior, bool)
+    core[near_face:ix - near_face, near_face:iy - near_face, near_face:iz - near_face] = True
+    eps = numpy.where(core, 1.0, eps)
+    pos = tuple(int(rng.integers(src_margin, s - src_margin)) for s in interior)
+    comp = int(rng.integers(0, 3))
+    wl = float(rng.uniform(12, 40))
+    return Scenario(interior, eps, (comp,) + pos, wl, steps, width_wl=width_wl)
+
+
+def embed(sc: Scenario, pad: int):
+    """(shape, eps list, source flat index, interior slices) of the scenario
+    padded by ``pad`` cells of eps = 1 on every face."""
+    ix, iy, iz = sc.interior
+    shape = (ix + 2 * pad, iy + 2 * pad, iz + 2 * pad)
+    eps = numpy.ones(shape)
+    sl = (slice(pad, pad + ix), slice(pad, pad + iy), slice(pad, pad + iz))
+    eps[sl] = sc.eps
+    c, x, y, z = sc.src
+    idx = int(numpy.ravel_multi_index((x + pad, y + pad, z + pad), shape)) + c * int(numpy.prod(shape))
+    return shape, [eps, eps, eps], idx, sl
+
+
+def cpml_faces(T: int) -> List[Dict]:
+    return [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
+
+
+def run_interior(sc: Scenario, pad: int, pmls: List[Dict], sample_every: int = 10,
+                 correction=None, float_type=numpy.float32) -> torch.Tensor:
+    """Runs the padded scenario; returns the interior E every
+    ``sample_every`` steps as a (samples, 3, ix, iy, iz) tensor.
+    ``correction(sim, n)``, if given, is applied after each E update and
+    source (the learned shell)."""
+    shape, eps, idx, sl = embed(sc, pad)
+    sim = Simulation(eps, pmls, dt=sc.dt, do_poynting=False, float_type=float_type)
+    wave = torch.as_tensor(sc.wave(), dtype=sim._tdtype, device=sim.device)
+    out = []
+    for n in range(sc.steps):
+        if correction is None:
+            sim.step_n(1, idx, wave[n:n + 1])
+        else:
+            sim.update_E([])
+            sim.E[idx] += wave[n]
+            correction(sim, n)
+            sim.update_H([])
+        if (n + 1) % sample_every == 0:
+            out.append(sim.E.view(3, *shape)[(slice(None),) + sl].clone())
+    return torch.stack(out)
+
+
+def reflection_db(test: torch.Tensor, wide: torch.Tensor) -> float:
+    num = torch.linalg.vector_norm((test - wide).double())
+    den = torch.linalg.vector_norm(wide.double())
+    return float(20 * torch.log10(num / den))
+
+
+# ---------------------------------------------------------------------------
+# The learned 4-cell shell
+# ---------------------------------------------------------------------------
+
+from . import _lib  # noqa: E402
+
+N_HID, N_IN, N_OUT, N_TAP = 16, 6, 3, 27
+W1_SIZE = N_HID * N_IN * N_TAP
+W2_SIZE = N_OUT * N_HID * N_TAP
+N_WEIGHTS = 6 * (W1_SIZE + W2_SIZE)
+
+
+BASE_PML = {"cfs_alpha": 0.05}   # the CPML the shell runs under the correction
+
+
+class ShellNet:
+    """Weights of the shell correction: an FP32 master copy (for training)
+    and the BF16 copy the kernels read.  ``base_pml`` are the CPML parameters
+    of the T-cell shell (defaults of simulation.py:161-168 otherwise)."""
+
+    def __init__(self, thickness: int = 4, weights: Optional[torch.Tensor] = None, seed: int = 0,
+                 w1_scale: float = 0.05, device="cuda", base_pml: Optional[Dict] = None):
+        self.T = thickness
+        self.base_pml = dict(BASE_PML if base_pml is None else base_pml)
+        if weights is None:
+            g = torch.Generator().manual_seed(seed)
+            w = torch.zeros(N_WEIGHTS, dtype=torch.float32)
+            w[:6 * W1_SIZE] = torch.randn(6 * W1_SIZE, generator=g) * w1_scale   # W2 = 0: starts as CPML-T
+            weights = w
+        self.master = weights.to(device=device, dtype=torch.float32).clone()
+        self.sync()
+
+    def sync(self) -> None:
+        self.bf16 = self.master.to(torch.bfloat16).contiguous()
+
+    def zero_(self) -> "ShellNet":
+        self.master.zero_()
+        self.sync()
+        return self
+
+    def faces(self) -> List[Dict]:
+        return [dict(f, **self.base_pml) for f in cpml_faces(self.T)]
+
+    def save(self, path: str) -> None:
+        numpy.savez_compressed(path, thickness=self.T, weights=self.master.detach().cpu().numpy(),
+                               base_pml_keys=numpy.array(list(self.base_pml.keys())),
+                               base_pml_values=numpy.array(list(self.base_pml.values()), dtype=numpy.float64))
+
+    @classmethod
+    def load(cls, path: str, device="cuda") -> "ShellNet":
+        with numpy.load(path, allow_pickle=False) as z:
+            base = {str(k): float(v) for k, v in zip(z["base_pml_keys"], z["base_pml_values"])}
+            return cls(int(z["thickness"]), weights=torch.as_tensor(z["weights"]), device=device, base_pml=base)
+
+
+class ShellRunner:
+    """Applies a ShellNet to one Simulation (the thin domain with CPML-T)."""
+
+    def __init__(self, sim: Simulation, net: ShellNet):
+        self.sim, self.net = sim, net
+        self.lib = sim._lib
+        nh = int(self.lib.fdtd_shell_hidden_elems(sim.plan, net.T))
+        if nh <= 0:
+            raise _lib.FdtdError("shell thickness %d does not fit the grid %s" % (net.T, sim.shape))
+        self.hidden = torch.zeros(nh, dtype=torch.float32, device=sim.device)
+        self.hbar = None
+        self.partials = None
+
+    def _st(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.sim.device).cuda_stream)
+
+    def apply(self, E: torch.Tensor, H: torch.Tensor) -> None:
+        """E += correction(E, H) on the shell cells (in place)."""
+        _lib.check(self.lib.fdtd_shell_apply(self.sim.plan, self.net.T, ctypes.c_void_p(self.net.bf16.data_ptr()),
+                                             ctypes.c_void_p(E.data_ptr()), ctypes.c_void_p(H.data_ptr()),
+                                             ctypes.c_void_p(self.hidden.data_ptr()), self._st()),
+                   "fdtd_shell_apply")
+
+    def backward(self, E, H, Ebar, Ebar_out, Hbar_out, wgrad) -> None:
+        if self.hbar is None:
+            self.hbar = torch.zeros_like(self.hidden)
+            self.partials = torch.zeros(int(self.lib.fdtd_shell_partials_elems()), dtype=torch.float32,
+                                        device=self.sim.device)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        _lib.check(self.lib.fdtd_shell_backward(self.sim.plan, self.net.T, p(self.net.bf16), p(E), p(H),
+                                                p(self.hidden), p(self.hbar), p(Ebar), p(Ebar_out), p(Hbar_out),
+                                                p(self.partials), p(wgrad), self._st()), "fdtd_shell_backward")
+
+
+import ctypes  # noqa: E402
+
+
+def run_learned(sc: Scenario, net: ShellNet, sample_every: int = 10, float_type=numpy.float32) -> torch.Tensor:
+    """The scenario in the thin domain (CPML-T shell + the net's correction);
+    interior E every ``sample_every`` steps."""
+    pad = net.T
+    shape, eps, idx, sl = embed(sc, pad)
+    sim = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=False, float_type=float_type)
+    runner = ShellRunner(sim, net)
+    wave = torch.as_tensor(sc.wave(), dtype=sim._tdtype, device=sim.device)
+    out = []
+    for n in range(sc.steps):
+        sim.update_E([])
+        sim._E[idx] += wave[n]
+        runner.apply(sim._E, sim._H[0])
+        sim.update_H([])
+        if (n + 1) % sample_every == 0:
+            out.append(sim._E.view(3, *shape)[(slice(None),) + sl].clone())
+    return torch.stack(out)
+
+
+class ShellTrainer:
+    """End-to-end training of a ShellNet through the solver.
+
+    Loss of one scenario: sum over steps of ||E_thin - E_wide||^2 on the
+    interior, divided by sum of ||E_wide||^2 (10 log10 of it is the
+    reflection in dB over every step).  Its gradient with respect to the
+    weights is exact: the thin run is reversed step by step with the adjoint
+    of the solver (fdtd_adjoint_step, split around the shell) and of the
+    shell correction (fdtd_shell_backward), with segments of
+    ``checkpoint_every`` steps recomputed from checkpoints."""
+
+    def __init__(self, net: ShellNet, scenarios: Sequence[Scenario], checkpoint_every: int = 60,
+                 lr: float = 1e-3, betas=(0.9, 0.999)):
+        self.net = net
+        self.K = checkpoint_every
+        self.lr, self.betas = lr, betas
+        self.m = torch.zeros_like(net.master)
+        self.v = torch.zeros_like(net.master)
+        self.t = 0
+        self.cases = []
+        for sc in scenarios:
+            wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=1)
+            self.cases.append((sc, wide, float(torch.sum(wide.double() ** 2))))
+
+    def _thin(self, sc):
+        shape, eps, idx, sl = embed(sc, self.net.T)
+        sim = Simulation(eps, self.net.faces(), dt=sc.dt, do_poynting=False)
+        return sim, idx, sl
+
+    def loss_and_grad(self, case) -> Tuple[float, torch.Tensor]:
+        sc, wide, Z = case
+        sim, idx, sl = self._thin(sc)
+        runner = ShellRunner(sim, self.net)
+        shape = sim.shape
+        wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device=sim.device)
+        wave_host = wave.double().cpu().numpy()
+        E, H = sim._E, sim._H[0]
+        Ev = lambda t: t.view(3, *shape)[(slice(None),) + sl]
+
+        def step(n, keep=None):
+            sim.update_E([])
+            E[idx] += wave[n]
+            if keep is not None:
+                keep.append(E.clone())           # E after the E step and source, before the correction
+            runner.apply(E, H)
+            sim.update_H([])
+
+        cps, loss = [], torch.zeros((), dtype=torch.float64, device=sim.device)
+        for n in range(sc.steps):
+            if n % self.K == 0:
+                cps.append(sim.checkpoint())
+            step(n)
+            loss += torch.sum((Ev(E) - wide[n]).double() ** 2)
+        # reverse
+        N3 = 3 * sim.N
+        Eb = torch.zeros(N3, dtype=torch.float32, device=sim.device)
+        Hb = torch.zeros_like(Eb)
+        Eb_mid = torch.zeros_like(Eb)
+        npsi = max(sim.psi_elems, 1)
+        pe = [torch.zeros(npsi, dtype=torch.float32, device=sim.device) for _ in range(2)]
+        ph = [torch.zeros(npsi, dtype=torch.float32, device=sim.device) for _ in range(2)]
+        wgrad = torch.zeros(N_WEIGHTS, dtype=torch.float32, device=sim.device)
+        b = _lib.AdjointBufs()
+        b.eps, b.coef = sim.eps.data_ptr(), sim.coef.data_ptr()
+        b.eps_grad = None
+        b.obj_axis = -1
+        b.src_index = -1
+        st = ctypes.c_void_p(torch.cuda.current_stream(sim.device).cuda_stream)
+        cur = 0
+        for s in reversed(range(len(cps))):
+            n0, n1 = s * self.K, min(s * self.K + self.K, sc.steps)
+            sim.restore(cps[s])
+            Es, Hs, Em = [E.clone()], [H.clone()], []
+            for n in range(n0, n1):
+                step(n, Em)
+                Es.append(E.clone())
+                Hs.append(H.clone())
+            for n in reversed(range(n0, n1)):
+                k = n - n0
+                # loss adjoint of E after step n
+                Ev(Eb).add_((2.0 / Z) * (Ev(Es[k + 1]) - wide[n]))
+                b.E_prev, b.E_next = Es[k].data_ptr(), Es[k + 1].data_ptr()
+                b.H_prev, b.H_next = Hs[k].data_ptr(), Hs[k + 1].data_ptr()
+                b.Eb, b.Hb = Eb.data_ptr(), Hb.data_ptr()
+                b.psib_e, b.psib_e_out = pe[cur].data_ptr(), pe[1 - cur].data_ptr()
+                b.psib_h, b.psib_h_out = ph[cur].data_ptr(), ph[1 - cur].data_ptr()
+                b.parts = 1                      # the H step
+                _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint H")
+                Eb_mid.copy_(Eb)                 # the correction: E_out = E_mid + y(E_mid, H_k)
+                runner.backward(Em[k], Hs[k], Eb, Eb_mid, Hb, wgrad)
+                Eb, Eb_mid = Eb_mid, Eb
+                b.Eb = Eb.data_ptr()
+                b.E_next = Em[k].data_ptr()
+                b.parts = 2                      # the E step
+                _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint E")
+                cur = 1 - cur
+            del Es, Hs, Em
+        return float(loss) / Z, wgrad
+
+    def adam(self, g: torch.Tensor) -> None:
+        b1, b2 = self.betas
+        self.t += 1
+        self.m.mul_(b1).add_(g, alpha=1 - b1)
+        self.v.mul_(b2).addcmul_(g, g, value=1 - b2)
+        mh = self.m / (1 - b1 ** self.t)
+        vh = self.v / (1 - b2 ** self.t)
+        self.net.master.sub_(self.lr * mh / (vh.sqrt() + 1e-12))
+        self.net.sync()
+
+    def train(self, iterations: int, log: Optional[Callable[[int, float], None]] = None,
+              decay_every: int = 0, decay: float = 0.5) -> List[float]:
+        hist = []
+        lr0 = self.lr
+        for it in range(iterations):
+            if decay_every:
+                self.lr = lr0 * decay ** (it // decay_every)
+            case = self.cases[it % len(self.cases)]
+            L, g = self.loss_and_grad(case)
+            self.adam(g)
+            hist.append(L)
+            if log:
+                log(it, L)
+        return hist
+
+
+# ---------------------------------------------------------------------------
+# Training and evaluation entry point
+# ---------------------------------------------------------------------------
+
+import json  # noqa: E402
+import os  # noqa: E402
+import time  # noqa: E402
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "shell4.npz")
+
+
+def train_scenarios(n: int, seed0: int = 100, interior=(48, 48, 48), steps: int = 300) -> List[Scenario]:
+    """Training trajectories: small interiors with eps ~ U[1, 4] near the
+    faces and random dipoles (position, polarisation, wavelength U[12, 40])."""
+    return [make_scenario(seed0 + s, interior=interior, steps=steps, near_face=12, width_wl=1.0)
+            for s in range(n)]
+
+
+def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print) -> ShellNet:
+    net = ShellNet(4, seed=1)
+    tr = ShellTrainer(net, train_scenarios(n_scen), checkpoint_every=60, lr=lr)
+    t0 = time.time()
+    tr.train(iterations, decay_every=max(iterations // 3, 1),
+             log=lambda it, L: log("it %d loss %.2f dB lr %.1e (%.0f s)" % (it, 10 * numpy.log10(L), tr.lr,
+                                                                            time.time() - t0)))
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    net.save(out)
+    return net
+
+
+def evaluate_main(net: ShellNet, seeds: Sequence[int], interior=(256, 256, 128), steps: int = 600,
+                  log=print) -> List[Dict]:
+    """Reflection (dB, vs the 48-cell padded run) of CPML-4 with the shell's
+    base parameters (the net zeroed), the learned 4-cell shell, CPML-8 and
+    CPML-16 on unseen scenarios of the given interior."""
+    rows = []
+    base = ShellNet(net.T, weights=torch.zeros_like(net.master), base_pml=net.base_pml)
+    for sd in seeds:
+        sc = make_scenario(sd, interior=interior, steps=steps)
+        wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=5)
+        row = {"seed": sd, "wavelength": round(sc.wavelength, 1), "src": list(sc.src)}
+        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=5), wide)
+        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=5), wide)
+        for T in (4, 8, 16):
+            row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=5), wide)
+        rows.append(row)
+        log(json.dumps(row))
+    return rows
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train / evaluate the learned 4-cell absorbing shell")
+    ap.add_argument("--train", type=int, default=0, help="training iterations")
+    ap.add_argument("--lr", type=float, default=5e-5)
+    ap.add_argument("--scenarios", type=int, default=16)
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
+    a = ap.parse_args(argv)
+    net = train_main(a.train, a.lr, a.scenarios, a.weights) if a.train else ShellNet.load(a.weights)
+    if a.evaluate:
+        evaluate_main(net, range(1000, 1000 + a.evaluate))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+"""CPU oracle of the learned absorbing shell (TEST INFRASTRUCTURE ONLY).
+
+Restates the shell correction (no counterpart in the reference): on a shell
+of T cells per face, split into six boxes (x faces first, then y, then z),
+  h_k(j) = sum_{c,t} W1[b][k][c][t] x_c(j + t)      x = (E, H), 16 hidden
+  y_o(i) = sum_{k,t} W2[b][o][k][t] h_k(i + t)      3 outputs, i in box b
+  E_out = E + y on the shell cells,
+with t over the 27 offsets of a periodic 3x3x3 stencil and weights in BF16.
+
+``forward(..., exact_f32=True)`` reproduces the GPU arithmetic: inputs and
+hidden values rounded to BF16, products and sums in float32, in the GPU's
+summation order.  ``forward(..., exact_f32=False)`` and ``backward`` work in
+float64 without rounding (the backward treats the BF16 roundings as identity,
+as the GPU backward does).
+"""
+from __future__ import annotations
+
+import numpy
+
+N_HID, N_IN, N_OUT, N_TAP = 16, 6, 3, 27
+W1_SIZE = N_HID * N_IN * N_TAP
+W2_SIZE = N_OUT * N_HID * N_TAP
+
+
+def offsets():
+    return [(t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) for t in range(N_TAP)]
+
+
+def boxes(shape, T):
+    """Six output boxes [o0, o1) per axis partitioning the shell."""
+    out = []
+    for r in range(3):
+        for side in range(2):
+            lo, hi = [], []
+            for q in range(3):
+                if q < r:
+                    lo.append(T); hi.append(shape[q] - T)
+                elif q > r:
+                    lo.append(0); hi.append(shape[q])
+                else:
+                    lo.append(0 if side == 0 else shape[q] - T)
+                    hi.append(T if side == 0 else shape[q])
+            out.append((tuple(lo), tuple(hi)))
+    return out
+
+
+def box_mask(shape, box):
+    m = numpy.zeros(shape, bool)
+    (a0, b0, c0), (a1, b1, c1) = box
+    m[a0:a1, b0:b1, c0:c1] = True
+    return m
+
+
+def bf16_round(a):
+    a32 = numpy.asarray(a, dtype=numpy.float32)
+    u = a32.view(numpy.uint32).astype(numpy.uint64)
+    special = (u & 0x7F800000) == 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(numpy.uint32).view(numpy.float32)
+    return numpy.where(special, a32, r)
+
+
+def split_weights(w, nbox=6):
+    """Flat weight vector (float) -> W1 [box][16][6][27], W2 [box][3][16][27]."""
+    w = numpy.asarray(w)
+    W1 = w[:nbox * W1_SIZE].reshape(nbox, N_HID, N_IN, N_TAP)
+    W2 = w[nbox * W1_SIZE:].reshape(nbox, N_OUT, N_HID, N_TAP)
+    return W1, W2
+
+
+def _shift(a, d):
+    """a[j + d] with periodic wrap, over the last three axes."""
+    return numpy.roll(a, (-d[0], -d[1], -d[2]), axis=(-3, -2, -1))
+
+
+def forward(E, H, w, T, exact_f32=True):
+    """E, H: (3, sx, sy, sz); w: flat weights already rounded to BF16 values
+    (float).  Returns (E_out, y) with y the correction on the shell."""
+    shape = E.shape[1:]
+    W1, W2 = split_weights(w)
+    dt = numpy.float32 if exact_f32 else numpy.float64
+    x = numpy.concatenate([E, H]).astype(dt)
+    if exact_f32:
+        x = bf16_round(x)
+    xs = [_shift(x, d) for d in offsets()]
+    y = numpy.zeros((3,) + shape, dt)
+    for b, box in enumerate(boxes(shape, T)):
+        h = numpy.zeros((N_HID,) + shape, dt)
+        for k in range(N_HID):
+            acc = numpy.zeros(shape, dt)
+            for t in range(N_TAP):
+                for c in range(N_IN):
+                    acc = acc + dt(W1[b, k, c, t]) * xs[t][c]
+            h[k] = acc
+        if exact_f32:
+            h = bf16_round(h)
+        hs = [_shift(h, d) for d in offsets()]
+        yb = numpy.zeros((3,) + shape, dt)
+        for t in range(N_TAP):
+            for k in range(N_HID):
+                for o in range(N_OUT):
+                    yb[o] = yb[o] + dt(W2[b, o, k, t]) * hs[t][k]
+        m = box_mask(shape, box)
+        y[:, m] = yb[:, m]
+    Eout = (E.astype(dt) + y).astype(dt)
+    return Eout, y
+
+
+def backward(E, H, w, T, Ebar):
+    """FP64 adjoint of E_out = E + y(E, H): returns (xbar_E, xbar_H, wbar)
+    with wbar in the flat weight layout; BF16 roundings as identity."""
+    shape = E.shape[1:]
+    W1, W2 = split_weights(numpy.asarray(w, dtype=numpy.float64))
+    x = numpy.concatenate([E, H]).astype(numpy.float64)
+    xs = [_shift(x, d) for d in offsets()]
+    xbar = numpy.zeros_like(x)
+    W1b = numpy.zeros_like(W1)
+    W2b = numpy.zeros_like(W2)
+    n = int(numpy.prod(shape))
+    X = numpy.array(xs).reshape(N_TAP, N_IN, n)                  # x_c(j + t)
+    for b, box in enumerate(boxes(shape, T)):
+        m = box_mask(shape, box)
+        h = numpy.einsum("kct,tcn->kn", W1[b], X).reshape((N_HID,) + shape)
+        Hs = numpy.array([_shift(h, d) for d in offsets()]).reshape(N_TAP, N_HID, n)
+        yb = numpy.where(m, Ebar, 0.0)
+        # W2bar[o][k][t] = sum_i ybar_o(i) h_k(i + t)
+        W2b[b] = numpy.einsum("on,tkn->okt", yb.reshape(N_OUT, n), Hs)
+        # hbar_k(j) = sum_{o,t} W2[o][k][t] ybar_o(j - t)
+        hbar = numpy.zeros_like(h)
+        for t, d in enumerate(offsets()):
+            hbar += numpy.einsum("ok,on->kn", W2[b][:, :, t],
+                                 _shift(yb, (-d[0], -d[1], -d[2])).reshape(N_OUT, n)).reshape(h.shape)
+        # W1bar[k][c][t] = sum_j hbar_k(j) x_c(j + t)
+        W1b[b] = numpy.einsum("kn,tcn->kct", hbar.reshape(N_HID, n), X)
+        for t, d in enumerate(offsets()):
+            xbar += numpy.einsum("kc,kn->cn", W1[b][:, :, t],
+                                 _shift(hbar, (-d[0], -d[1], -d[2])).reshape(N_HID, n)).reshape(x.shape)
+    wbar = numpy.concatenate([W1b.ravel(), W2b.ravel()])
+    return xbar[:3], xbar[3:], wbar

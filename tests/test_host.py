@@ -35,7 +35,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.Face) == 16
     assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
     assert ctypes.sizeof(_lib.Fields) == 17 * 8
-    assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 + 4
+    assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4
 
 
 def _desc(shape=(10, 11, 12), faces=(), **kw):

@@ -1,0 +1,136 @@
+"""The learned shell's convolution: the CPU oracle pinned by its own
+transpose and finite-difference identities (CPU), and the HIP forward and
+backward kernels against the oracle (GPU)."""
+import ctypes
+
+import numpy
+import pytest
+
+from oracle import shell_net as SN
+
+
+def _rand_weights(rng, scale=0.05):
+    return SN.bf16_round(rng.standard_normal(6 * (SN.W1_SIZE + SN.W2_SIZE)) * scale).astype(numpy.float64)
+
+
+def _fields(rng, shape):
+    return rng.standard_normal((3,) + shape), rng.standard_normal((3,) + shape)
+
+
+def test_oracle_boxes_partition_the_shell():
+    shape, T = (12, 11, 13), 3
+    cover = numpy.zeros(shape, int)
+    for box in SN.boxes(shape, T):
+        cover += SN.box_mask(shape, box)
+    inner = numpy.zeros(shape, bool)
+    inner[T:-T, T:-T, T:-T] = True
+    assert numpy.all(cover[inner] == 0) and numpy.all(cover[~inner] == 1)
+
+
+def test_oracle_backward_is_the_transpose():
+    rng = numpy.random.default_rng(401)
+    shape, T = (10, 9, 11), 2
+    w = _rand_weights(rng)
+    E, H = _fields(rng, shape)
+    Ebar = rng.standard_normal((3,) + shape)
+    _, y = SN.forward(E, H, w, T, exact_f32=False)
+    xbE, xbH, _ = SN.backward(E, H, w, T, Ebar)
+    lhs = float(numpy.sum(Ebar * y))
+    rhs = float(numpy.sum(xbE * E) + numpy.sum(xbH * H))
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), 1)
+
+
+def test_oracle_weight_gradient_finite_differences():
+    """<Ebar, y(W)> is bilinear in (W1, W2): its derivative along a weight
+    direction equals the central difference exactly (up to rounding)."""
+    rng = numpy.random.default_rng(402)
+    shape, T = (9, 8, 10), 2
+    w = _rand_weights(rng)
+    E, H = _fields(rng, shape)
+    Ebar = rng.standard_normal((3,) + shape)
+    _, _, wbar = SN.backward(E, H, w, T, Ebar)
+    f = lambda ww: float(numpy.sum(Ebar * SN.forward(E, H, ww, T, exact_f32=False)[1]))
+    for _ in range(6):
+        d = rng.standard_normal(w.size)
+        h = 1e-3
+        fd = (f(w + h * d) - f(w - h * d)) / (2 * h)
+        assert abs(fd - float(wbar @ d)) <= 1e-8 * max(abs(fd), 1)
+    # a few single weights in W1 and W2
+    for idx in (5, SN.W1_SIZE * 3 + 77, 6 * SN.W1_SIZE + 10, 6 * SN.W1_SIZE + 5 * SN.W2_SIZE + 400):
+        d = numpy.zeros(w.size)
+        d[idx] = 1.0
+        fd = (f(w + 1e-3 * d) - f(w - 1e-3 * d)) / 2e-3
+        assert abs(fd - wbar[idx]) <= 1e-8 * max(abs(fd), 1)
+
+
+def test_oracle_zero_weights_identity():
+    rng = numpy.random.default_rng(403)
+    shape = (8, 8, 8)
+    E, H = _fields(rng, shape)
+    Eo, y = SN.forward(E, H, numpy.zeros(6 * (SN.W1_SIZE + SN.W2_SIZE)), 2)
+    assert numpy.array_equal(Eo, E.astype(numpy.float32)) and not y.any()
+
+
+def _gpu_run(shape, T, w, E, H, Ebar=None):
+    import torch
+    from opencl_fdtd_amd import Simulation
+    sim = Simulation([numpy.ones(shape)] * 3, [], do_poynting=False)
+    lib = sim._lib
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    wb = torch.as_tensor(w, dtype=torch.float32).to(torch.bfloat16).cuda()
+    assert numpy.array_equal(wb.float().cpu().numpy(), w.astype(numpy.float32))
+    nh = int(lib.fdtd_shell_hidden_elems(sim.plan, T))
+    hid = torch.zeros(nh, dtype=torch.float32, device="cuda")
+    Et = torch.as_tensor(E.ravel(), dtype=torch.float32, device="cuda").contiguous()
+    Ht = torch.as_tensor(H.ravel(), dtype=torch.float32, device="cuda").contiguous()
+    if Ebar is None:
+        assert lib.fdtd_shell_apply(sim.plan, T, ctypes.c_void_p(wb.data_ptr()), ctypes.c_void_p(Et.data_ptr()),
+                                    ctypes.c_void_p(Ht.data_ptr()), ctypes.c_void_p(hid.data_ptr()), st) == 0
+        torch.cuda.synchronize()
+        return Et.cpu().numpy().reshape((3,) + shape)
+    hbar = torch.zeros(nh, dtype=torch.float32, device="cuda")
+    Eb = torch.as_tensor(Ebar.ravel(), dtype=torch.float32, device="cuda").contiguous()
+    Ebo = torch.zeros_like(Eb)
+    Hbo = torch.zeros_like(Eb)
+    parts = torch.zeros(int(lib.fdtd_shell_partials_elems()), dtype=torch.float32, device="cuda")
+    wg = torch.zeros(w.size, dtype=torch.float32, device="cuda")
+    rc = lib.fdtd_shell_backward(sim.plan, T, ctypes.c_void_p(wb.data_ptr()), ctypes.c_void_p(Et.data_ptr()),
+                                 ctypes.c_void_p(Ht.data_ptr()), ctypes.c_void_p(hid.data_ptr()),
+                                 ctypes.c_void_p(hbar.data_ptr()), ctypes.c_void_p(Eb.data_ptr()),
+                                 ctypes.c_void_p(Ebo.data_ptr()), ctypes.c_void_p(Hbo.data_ptr()),
+                                 ctypes.c_void_p(parts.data_ptr()), ctypes.c_void_p(wg.data_ptr()), st)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return (Ebo.cpu().numpy().reshape((3,) + shape), Hbo.cpu().numpy().reshape((3,) + shape), wg.cpu().numpy())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4), ((13, 12, 17), 3)])
+def test_hip_forward_bit_exact(shape, T):
+    rng = numpy.random.default_rng(404)
+    w = _rand_weights(rng)
+    E, H = _fields(rng, shape)
+    E, H = E.astype(numpy.float32), H.astype(numpy.float32)
+    got = _gpu_run(shape, T, w, E, H)
+    want, _ = SN.forward(E, H, w, T, exact_f32=True)
+    bad = numpy.flatnonzero(got != want)
+    assert bad.size == 0, (bad.size, bad[:5])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4)])
+def test_hip_backward_matches_oracle(shape, T):
+    rng = numpy.random.default_rng(405)
+    w = _rand_weights(rng)
+    E, H = _fields(rng, shape)
+    E = SN.bf16_round(E).astype(numpy.float64)   # the GPU rounds its inputs to BF16
+    H = SN.bf16_round(H).astype(numpy.float64)
+    Ebar = rng.standard_normal((3,) + shape).astype(numpy.float32).astype(numpy.float64)
+    gE, gH, gw = _gpu_run(shape, T, w, E.astype(numpy.float32), H.astype(numpy.float32), Ebar)
+    oE, oH, ow = SN.backward(E, H, w, T, Ebar)
+    for g, o in ((gE, oE), (gH, oH)):
+        assert numpy.abs(g - o).max() <= 2e-5 * numpy.abs(o).max()
+    # the W2 gradient uses BF16-rounded hidden values on the GPU
+    n1 = 6 * SN.W1_SIZE
+    assert numpy.abs(gw[:n1] - ow[:n1]).max() <= 2e-5 * numpy.abs(ow[:n1]).max()
+    assert numpy.abs(gw[n1:] - ow[n1:]).max() <= 1e-2 * numpy.abs(ow[n1:]).max()
