@@ -177,6 +177,58 @@ def precision_report():
     return rows
 
 
+def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001)):
+    """Interior 256x256x128: interior Gcell-updates/s of CPML-16, CPML-8, the
+    learned 4-cell shell and its CPML-4 base (Poynting off), and reflection
+    (dB vs a 48-cell padded run) of each on unseen scenarios."""
+    import torch
+    from opencl_fdtd_amd import Simulation, learned as L
+    net = L.ShellNet.load(L.WEIGHTS)
+    interior = (256, 256, 128)
+    n_int = int(numpy.prod(interior))
+    sc = L.make_scenario(seeds[0], interior=interior, steps=steps + warmup)
+    rows = {}
+
+    def timed(name, pad, faces, runner_net=None):
+        shape, eps, idx, sl = L.embed(sc, pad)
+        sim = Simulation(eps, faces, dt=sc.dt, do_poynting=False)
+        wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device="cuda")
+        runner = L.ShellRunner(sim, runner_net) if runner_net is not None else None
+        st = torch.cuda.current_stream()
+
+        def run(n0, n):
+            if runner is None:
+                sim.step_n(n, idx, wave[n0:n0 + n])
+                return
+            for k in range(n0, n0 + n):
+                sim.update_E([])
+                sim._E[idx] += wave[k]
+                runner.apply(sim._E, sim._H[0])
+                sim.update_H([])
+        run(0, warmup)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        run(warmup, steps)
+        b.record(st)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        rows[name] = {"shape": list(shape), "cells": int(numpy.prod(shape)), "ms_per_step": ms / steps,
+                      "interior_gcell_s": n_int * steps / ms / 1e6,
+                      "total_gcell_s": int(numpy.prod(shape)) * steps / ms / 1e6}
+        print(json.dumps({name: rows[name]}), file=sys.stderr, flush=True)
+
+    timed("cpml16", 16, L.cpml_faces(16))
+    timed("cpml8", 8, L.cpml_faces(8))
+    timed("learned4", 4, net.faces(), net)
+    timed("cpml4_base", 4, net.faces())
+    refl = L.evaluate_main(net, seeds, interior=interior, steps=600, log=lambda s: print(s, file=sys.stderr))
+    out = {"speed": rows, "reflection_db": refl,
+           "learned4_vs_cpml16_interior_speedup": rows["learned4"]["interior_gcell_s"] / rows["cpml16"]["interior_gcell_s"]}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", action="store_true",
                     help="per-config FP32 vs FP64 and FP32 vs oracle report instead of the benchmark")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned 4-cell shell vs CPML-8/16: interior speed and reflection")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
@@ -197,6 +251,9 @@ def main():
 
     if args.precision:
         precision_report()
+        return
+    if args.learned:
+        learned_report()
         return
 
     import torch

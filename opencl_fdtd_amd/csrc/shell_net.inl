@@ -384,10 +384,180 @@ __global__ __launch_bounds__(256) void shell_wgrad_sum(const float* part, int gr
     out[wi] += (float)s;
 }
 
+// ---- inference with MFMA (v_mfma_f32_16x16x32_bf16) ----------------------
+//
+// One workgroup (4 waves) per tile of a box: T x 8 x 16 output cells (the
+// thin axis first).  The input fields of the tile grown by 2 are staged in
+// LDS as BF16 [cell][8] (6 channels + 2 zero pads), the hidden layer of the
+// tile grown by 1 as BF16 [cell][16].  Both layers are GEMMs over (tap,
+// channel):
+//   layer 1: M = hidden cells, K = 28 taps x 8 channels (224; tap 27 and
+//            channels 6, 7 are zero), N = 16 hidden channels;
+//   layer 2: M = output cells, K = 28 taps x 16 hidden channels, N = 16
+//            (3 outputs used).
+// MFMA operands (16x16x32): lane l holds A row l%16 and B column l%16 at
+// k = 8*(l/16) + [0..7]; D column l%16 at rows 4*(l/16) + [0..3].  With the
+// K order above, a lane's 8 A values are one 16-byte LDS read.  The
+// correction is written to ybuf[out cell][3] and added to E by shell_add,
+// since neighbouring tiles read E around their own cells.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileA = 8, kTileB = 16;
+constexpr int kMaxT = 4;
+constexpr int kInMax = (kMaxT + 4) * (kTileA + 4) * (kTileB + 4);
+constexpr int kHidMax = (kMaxT + 2) * (kTileA + 2) * (kTileB + 2);
+
+struct ShellTiles {
+    int te[6][3];      // tile extents per axis
+    int nt[6][3];      // tiles per axis
+    int tstart[7];     // prefix of tiles per box
+    int ostart[6];     // prefix of output cells per box
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    return (uint16_t)(__float_as_uint(bf16_round(f)) >> 16);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void shell_mfma(ShellArgs<T> a, ShellTiles tl, float* ybuf) {
+    __shared__ uint4 s_in[kInMax];
+    __shared__ uint4 s_hid[kHidMax * 2];
+    int b = 0;
+    while (b < a.nbox - 1 && (int)blockIdx.x >= tl.tstart[b + 1]) ++b;
+    const ShellBox& B = a.box[b];
+    int tix = blockIdx.x - tl.tstart[b];
+    int tcoord[3];
+    tcoord[2] = tix % tl.nt[b][2];
+    tcoord[1] = (tix / tl.nt[b][2]) % tl.nt[b][1];
+    tcoord[0] = tix / (tl.nt[b][2] * tl.nt[b][1]);
+    int o[3], te[3], ie[3], he[3];
+    for (int q = 0; q < 3; ++q) {
+        te[q] = tl.te[b][q];
+        o[q] = B.o0[q] + tcoord[q] * te[q];
+        ie[q] = te[q] + 4;
+        he[q] = te[q] + 2;
+    }
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const int nin = ie[0] * ie[1] * ie[2];
+    const int nhid = he[0] * he[1] * he[2];
+    const int nout = te[0] * te[1] * te[2];
+    // 1. input tile -> LDS (BF16, 8 per cell)
+    for (int c = threadIdx.x; c < nin; c += 256) {
+        const int lz = c % ie[2], ly = (c / ie[2]) % ie[1], lx = c / (ie[2] * ie[1]);
+        const size_t gi = gidx(a, o[0] - 2 + lx, o[1] - 2 + ly, o[2] - 2 + lz);
+        uint32_t w[4];
+        const uint16_t v0 = f2bf((float)a.E[gi]), v1 = f2bf((float)a.E[N + gi]), v2 = f2bf((float)a.E[2 * N + gi]);
+        const uint16_t v3 = f2bf((float)a.H[gi]), v4 = f2bf((float)a.H[N + gi]), v5 = f2bf((float)a.H[2 * N + gi]);
+        w[0] = (uint32_t)v0 | ((uint32_t)v1 << 16);
+        w[1] = (uint32_t)v2 | ((uint32_t)v3 << 16);
+        w[2] = (uint32_t)v4 | ((uint32_t)v5 << 16);
+        w[3] = 0;
+        s_in[c] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    // weight fragments of this lane
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, n = lane & 15;
+    const uint16_t* w1 = a.w + (size_t)b * kW1;
+    const uint16_t* w2 = a.w + (size_t)a.nbox * kW1 + (size_t)b * kW2;
+    bf16x8 B1[7], B2[14];
+    for (int j = 0; j < 7; ++j) {
+        const int tap = 4 * j + g;
+        for (int i = 0; i < 8; ++i) {
+            const uint16_t v = (tap < kTaps && i < kIn) ? w1[(n * kIn + i) * kTaps + tap] : (uint16_t)0;
+            B1[j][i] = __builtin_bit_cast(__bf16, v);
+        }
+    }
+    for (int j = 0; j < 14; ++j) {
+        const int tap = 2 * j + (g >> 1);
+        for (int i = 0; i < 8; ++i) {
+            const int k = 8 * (g & 1) + i;
+            const uint16_t v = (tap < kTaps && n < kOut) ? w2[(n * kHid + k) * kTaps + tap] : (uint16_t)0;
+            B2[j][i] = __builtin_bit_cast(__bf16, v);
+        }
+    }
+    __syncthreads();
+    // 2. layer 1: hidden = conv1(input)
+    const int mt1 = (nhid + 15) / 16;
+    uint16_t* hid16 = reinterpret_cast<uint16_t*>(s_hid);
+    for (int mt = wave; mt < mt1; mt += 4) {
+        const int row = min(mt * 16 + n, nhid - 1);
+        const int hz = row % he[2], hy = (row / he[2]) % he[1], hx = row / (he[2] * he[1]);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int tap = min(4 * j + g, kTaps - 1);     // tap 27 has zero weights
+            const int ix = hx + 1 + tap / 9 - 1, iy = hy + 1 + (tap / 3) % 3 - 1, iz = hz + 1 + tap % 3 - 1;
+            const uint4 v = s_in[(ix * ie[1] + iy) * ie[2] + iz];
+            const bf16x8 av = __builtin_bit_cast(bf16x8, v);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, B1[j], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = mt * 16 + 4 * g + i;
+            if (r < nhid) hid16[r * kHid + n] = f2bf(acc[i]);
+        }
+    }
+    __syncthreads();
+    // 3. layer 2: y = conv2(hidden) on the tile's output cells
+    const int mt2 = (nout + 15) / 16;
+    for (int mt = wave; mt < mt2; mt += 4) {
+        const int row = min(mt * 16 + n, nout - 1);
+        const int oz = row % te[2], oy = (row / te[2]) % te[1], ox = row / (te[2] * te[1]);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 14; ++j) {
+            const int tap = min(2 * j + (g >> 1), kTaps - 1);
+            const int hx = ox + 1 + tap / 9 - 1, hy = oy + 1 + (tap / 3) % 3 - 1, hz = oz + 1 + tap % 3 - 1;
+            const uint4 v = s_hid[((hx * he[1] + hy) * he[2] + hz) * 2 + (g & 1)];
+            const bf16x8 av = __builtin_bit_cast(bf16x8, v);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, B2[j], acc, 0, 0, 0);
+        }
+        if (n < kOut) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = mt * 16 + 4 * g + i;
+                if (r >= nout) continue;
+                const int rz = r % te[2], ry = (r / te[2]) % te[1], rx = r / (te[2] * te[1]);
+                const int gx = o[0] + rx, gy = o[1] + ry, gz = o[2] + rz;
+                if (gx >= B.o1[0] || gy >= B.o1[1] || gz >= B.o1[2]) continue;
+                const int bx = gx - B.o0[0], by = gy - B.o0[1], bz = gz - B.o0[2];
+                const int oi = tl.ostart[b] + (bx * (B.o1[1] - B.o0[1]) + by) * (B.o1[2] - B.o0[2]) + bz;
+                ybuf[(size_t)oi * kOut + n] = acc[i];
+            }
+        }
+    }
+}
+
+// E += ybuf on the shell cells
+template <class T>
+__global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, const float* ybuf, int total_out) {
+    int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= total_out) return;
+    const int oi = cell;
+    int b = 0;
+    for (; b < a.nbox; ++b) {
+        const ShellBox& B = a.box[b];
+        const int n = (B.o1[0] - B.o0[0]) * (B.o1[1] - B.o0[1]) * (B.o1[2] - B.o0[2]);
+        if (cell < n) break;
+        cell -= n;
+    }
+    if (b >= a.nbox) return;
+    const ShellBox& B = a.box[b];
+    const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
+    const int lz = cell % nz, ly = (cell / nz) % ny, lx = cell / (nz * ny);
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const size_t gi = gidx(a, B.o0[0] + lx, B.o0[1] + ly, B.o0[2] + lz);
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) a.Eout[o * N + gi] += (T)ybuf[(size_t)oi * kOut + o];
+}
+
 struct ShellPlan {
     int nbox;
     ShellBox box[6];
     int hcells, out_cells;
+    ShellTiles tiles;
+    int ntiles;
 };
 
 ShellPlan make_shell(const Plan& P, int T) {
@@ -420,6 +590,24 @@ ShellPlan make_shell(const Plan& P, int T) {
     }
     S.hcells = off;
     S.out_cells = outc;
+    // MFMA tiles: the box's thin axis takes T, the other two 8 and 16
+    int ts = 0, os = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int r = b / 2;
+        int k = 0;
+        for (int q = 0; q < 3; ++q) {
+            const int te = (q == r) ? T : (k++ == 0 ? kTileA : kTileB);
+            const int ext = S.box[b].o1[q] - S.box[b].o0[q];
+            S.tiles.te[b][q] = te;
+            S.tiles.nt[b][q] = (ext + te - 1) / te;
+        }
+        S.tiles.tstart[b] = ts;
+        S.tiles.ostart[b] = os;
+        ts += S.tiles.nt[b][0] * S.tiles.nt[b][1] * S.tiles.nt[b][2];
+        os += (S.box[b].o1[0] - S.box[b].o0[0]) * (S.box[b].o1[1] - S.box[b].o0[1]) * (S.box[b].o1[2] - S.box[b].o0[2]);
+    }
+    S.tiles.tstart[nb] = ts;
+    S.ntiles = ts;
     return S;
 }
 
@@ -431,9 +619,10 @@ void fill_shell(const Plan& P, const ShellPlan& S, ShellArgs<T>& a) {
     for (int b = 0; b < S.nbox; ++b) a.box[b] = S.box[b];
 }
 
-// T >= 2 keeps every grown region free of self-overlap through the wrap.
+// T >= 2 keeps every grown region free of self-overlap through the wrap;
+// T <= 4 bounds the MFMA tiles' LDS.
 int check_shell(const Plan& P, int T) {
-    if (T < 2) return fail(3, "shell thickness %d < 2", T);
+    if (T < 2 || T > kMaxT) return fail(3, "shell thickness %d not in [2, %d]", T, kMaxT);
     if (2 * T + 4 > P.sx || 2 * T + 4 > P.sy || 2 * T + 4 > P.sz)
         return fail(3, "shell thickness %d too large for the grid", T);
     return 0;
@@ -460,19 +649,20 @@ int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, con
     if (!w || !E || !H || !hidden) return fail(2, "w, E, H and hidden are required");
     const ShellPlan S = make_shell(P, thickness);
     hipStream_t st = (hipStream_t)stream;
-    const int g1 = (S.hcells + 255) / 256, g2 = (S.out_cells + 255) / 256;
+    const int g2 = (S.out_cells + 255) / 256;
+    // hidden scratch holds the correction (out_cells x 3 <= hcells x 16)
     if (P.d.dtype == FDTD_F32) {
         ShellArgs<float> a;
         fill_shell<float>(P, S, a);
-        a.w = w; a.E = (const float*)E; a.H = (const float*)H; a.Eout = (float*)E; a.hid = hidden;
-        hipLaunchKernelGGL(shell_l1<float>, dim3(g1), dim3(256), 0, st, a, S.hcells);
-        hipLaunchKernelGGL(shell_l2<float>, dim3(g2), dim3(256), 0, st, a, S.out_cells);
+        a.w = w; a.E = (const float*)E; a.H = (const float*)H; a.Eout = (float*)E;
+        hipLaunchKernelGGL(shell_mfma<float>, dim3(S.ntiles), dim3(256), 0, st, a, S.tiles, hidden);
+        hipLaunchKernelGGL(shell_add<float>, dim3(g2), dim3(256), 0, st, a, (const float*)hidden, S.out_cells);
     } else {
         ShellArgs<double> a;
         fill_shell<double>(P, S, a);
-        a.w = w; a.E = (const double*)E; a.H = (const double*)H; a.Eout = (double*)E; a.hid = hidden;
-        hipLaunchKernelGGL(shell_l1<double>, dim3(g1), dim3(256), 0, st, a, S.hcells);
-        hipLaunchKernelGGL(shell_l2<double>, dim3(g2), dim3(256), 0, st, a, S.out_cells);
+        a.w = w; a.E = (const double*)E; a.H = (const double*)H; a.Eout = (double*)E;
+        hipLaunchKernelGGL(shell_mfma<double>, dim3(S.ntiles), dim3(256), 0, st, a, S.tiles, hidden);
+        hipLaunchKernelGGL(shell_add<double>, dim3(g2), dim3(256), 0, st, a, (const float*)hidden, S.out_cells);
     }
     return check_launch("fdtd_shell_apply");
 }

@@ -105,16 +105,25 @@ def _gpu_run(shape, T, w, E, H, Ebar=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4), ((13, 12, 17), 3)])
-def test_hip_forward_bit_exact(shape, T):
+@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4), ((13, 12, 17), 3),
+                                      ((20, 37, 25), 4)])
+def test_hip_forward_matches_oracle(shape, T):
+    """The MFMA inference kernel: same BF16 operands as the oracle's FP32
+    emulation, FP32 accumulation in another order, so the correction agrees
+    to accumulation rounding (and, rarely, one BF16 step of a hidden value);
+    cells outside the shell are untouched."""
     rng = numpy.random.default_rng(404)
     w = _rand_weights(rng)
     E, H = _fields(rng, shape)
     E, H = E.astype(numpy.float32), H.astype(numpy.float32)
     got = _gpu_run(shape, T, w, E, H)
-    want, _ = SN.forward(E, H, w, T, exact_f32=True)
-    bad = numpy.flatnonzero(got != want)
-    assert bad.size == 0, (bad.size, bad[:5])
+    want, y = SN.forward(E, H, w, T, exact_f32=True)
+    inner = numpy.zeros(shape, bool)
+    inner[T:-T, T:-T, T:-T] = True
+    assert numpy.array_equal(got[:, inner], E[:, inner])
+    err = numpy.abs((got - E) - y).max()
+    print("max |y_gpu - y_oracle| = %.3e of max |y| = %.3e" % (err, numpy.abs(y).max()))
+    assert err <= 2e-3 * numpy.abs(y).max()
 
 
 @pytest.mark.gpu
