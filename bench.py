@@ -199,12 +199,8 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001)):
         def run(n0, n):
             if runner is None:
                 sim.step_n(n, idx, wave[n0:n0 + n])
-                return
-            for k in range(n0, n0 + n):
-                sim.update_E([])
-                sim._E[idx] += wave[k]
-                runner.apply(sim._E, sim._H[0])
-                sim.update_H([])
+            else:
+                runner.step_n(n, idx, wave[n0:n0 + n])
         run(0, warmup)
         torch.cuda.synchronize()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -203,13 +203,20 @@ int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_pr
  * six boxes, then W2[box][3][16][27]; taps t = 9(dx+1) + 3(dy+1) + (dz+1).
  * ------------------------------------------------------------------------ */
 int64_t fdtd_shell_weight_elems(void);
-/* FP32 scratch for the hidden layer (and, in the backward, its adjoint). */
+/* FP32 scratch of the correction (the hidden layer or the correction
+ * itself, then the weights repacked in matrix-core fragment order); the
+ * backward also takes a second buffer of this size for the hidden adjoint. */
 int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness);
 /* FP32 scratch for the weight-gradient partials. */
 int64_t fdtd_shell_partials_elems(void);
 /* E += correction(E, H) on the shell cells (E read before it is corrected). */
 int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
                      float* hidden, void* stream);
+/* n steps of fdtd_step_n with the correction applied between the E update
+ * (and its source) and the H update; the same fields, source and swap rules
+ * as fdtd_step_n (two-pass Poynting excluded). */
+int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness, const uint16_t* w,
+                      float* hidden, int32_t n, int64_t src_index, const void* src_wave, void* stream);
 /* Adjoint of E_out = E + correction(E, H): given Ebar (adjoint of E_out),
  * Ebar_out += Ebar-contribution through the correction's E input (callers
  * initialise Ebar_out with Ebar for the identity part), Hbar_out += the H

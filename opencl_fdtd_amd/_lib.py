@@ -25,7 +25,7 @@ EXPORTS = (
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
     "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems",
-    "fdtd_shell_apply", "fdtd_shell_backward",
+    "fdtd_shell_apply", "fdtd_shell_step_n", "fdtd_shell_backward",
 )
 
 
@@ -108,6 +108,7 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_int64
     lib.fdtd_shell_apply.argtypes = [P, ctypes.c_int32, P, P, P, P, P]
     lib.fdtd_shell_backward.argtypes = [P, ctypes.c_int32] + [P] * 10 + [P]
+    lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int64, P, P]
     sized = ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials",
              "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems")
     for name in EXPORTS:

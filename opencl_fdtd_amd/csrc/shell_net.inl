@@ -386,155 +386,261 @@ __global__ __launch_bounds__(256) void shell_wgrad_sum(const float* part, int gr
 
 // ---- inference with MFMA (v_mfma_f32_16x16x32_bf16) ----------------------
 //
-// One workgroup (4 waves) per tile of a box: T x 8 x 16 output cells (the
-// thin axis first).  The input fields of the tile grown by 2 are staged in
-// LDS as BF16 [cell][8] (6 channels + 2 zero pads), the hidden layer of the
-// tile grown by 1 as BF16 [cell][16].  Both layers are GEMMs over (tap,
-// channel):
-//   layer 1: M = hidden cells, K = 28 taps x 8 channels (224; tap 27 and
-//            channels 6, 7 are zero), N = 16 hidden channels;
-//   layer 2: M = output cells, K = 28 taps x 16 hidden channels, N = 16
-//            (3 outputs used).
+// One workgroup (4 waves) per tile of a box.  A tile has three axes: a0 the
+// box's thin axis (T cells), a1 (8 cells) and a2 (14 cells), a2 innermost in
+// LDS; a2 is z for the x and y faces and y for the z faces (a1 is the
+// remaining axis).  The tile's input grown by 2 (T+4 x 12 x 18 cells) is
+// staged in LDS as BF16 [cell][8] (6 channels + 2 zero pads), its hidden
+// layer grown by 1 (T+2 x 10 x 16) as BF16 in two planes [8 channels][cell]
+// over the same LDS words.  Both layers are GEMMs over (tap, channel), with
+// M = 16 cells of one a2 row (a hidden row is exactly 16 cells; an output
+// row is 14, the last 2 rows of each MFMA are discarded):
+//   layer 1: K = 28 taps x 8 channels (tap 27 and channels 6, 7 are zero),
+//            N = 16 hidden channels;
+//   layer 2: K = 28 taps x 16 hidden channels, N = 16 (3 outputs used).
 // MFMA operands (16x16x32): lane l holds A row l%16 and B column l%16 at
 // k = 8*(l/16) + [0..7]; D column l%16 at rows 4*(l/16) + [0..3].  With the
-// K order above, a lane's 8 A values are one 16-byte LDS read.  The
-// correction is written to ybuf[out cell][3] and added to E by shell_add,
+// K order above, a lane's 8 A values are one 16-byte LDS read at a per-lane
+// offset (fixed per fragment) from the row's base.  The correction goes to
+// a tile-major scratch [tile][3][T*8*16] and is added to E by shell_add,
 // since neighbouring tiles read E around their own cells.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTileA = 8, kTileB = 16;
 constexpr int kMaxT = 4;
-constexpr int kInMax = (kMaxT + 4) * (kTileA + 4) * (kTileB + 4);
-constexpr int kHidMax = (kMaxT + 2) * (kTileA + 2) * (kTileB + 2);
+constexpr int kTileA1 = 8, kTileA2 = 14;                 // output cells along a1, a2
+constexpr int kRow = 16;                                  // MFMA rows = hidden a2 extent
+constexpr int kInMax = (kMaxT + 4) * (kTileA1 + 4) * (kTileA2 + 4);
+constexpr int kHidRows = (kMaxT + 2) * (kTileA1 + 2);     // hidden a2 rows (of 16 cells)
+constexpr int kHidPlane = kHidRows * kRow;                // cells per hidden plane
+constexpr int kTileRows = kMaxT * kTileA1 * kRow;         // correction slots per tile and output
+constexpr int kTileLds = kInMax > 2 * kHidPlane ? kInMax : 2 * kHidPlane;   // uint4 words
+static_assert(kTileA2 + 2 == kRow, "hidden rows must be one MFMA tall");
 
 struct ShellTiles {
-    int te[6][3];      // tile extents per axis
-    int nt[6][3];      // tiles per axis
+    int perm[6][3];    // grid axis of tile axes a0, a1, a2
+    int te[6][3];      // output extents along a0, a1, a2
+    int nt[6][3];      // tiles along a0, a1, a2
     int tstart[7];     // prefix of tiles per box
-    int ostart[6];     // prefix of output cells per box
 };
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    return (uint16_t)(__float_as_uint(bf16_round(f)) >> 16);
+// BF16 conversions with round-to-nearest-even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    bf16x2 r;
+    r[0] = (__bf16)lo;
+    r[1] = (__bf16)hi;
+    return __builtin_bit_cast(uint32_t, r);
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void shell_mfma(ShellArgs<T> a, ShellTiles tl, float* ybuf) {
-    __shared__ uint4 s_in[kInMax];
-    __shared__ uint4 s_hid[kHidMax * 2];
-    int b = 0;
-    while (b < a.nbox - 1 && (int)blockIdx.x >= tl.tstart[b + 1]) ++b;
-    const ShellBox& B = a.box[b];
-    int tix = blockIdx.x - tl.tstart[b];
-    int tcoord[3];
-    tcoord[2] = tix % tl.nt[b][2];
-    tcoord[1] = (tix / tl.nt[b][2]) % tl.nt[b][1];
-    tcoord[0] = tix / (tl.nt[b][2] * tl.nt[b][1]);
-    int o[3], te[3], ie[3], he[3];
-    for (int q = 0; q < 3; ++q) {
-        te[q] = tl.te[b][q];
-        o[q] = B.o0[q] + tcoord[q] * te[q];
-        ie[q] = te[q] + 4;
-        he[q] = te[q] + 2;
-    }
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const int nin = ie[0] * ie[1] * ie[2];
-    const int nhid = he[0] * he[1] * he[2];
-    const int nout = te[0] * te[1] * te[2];
-    // 1. input tile -> LDS (BF16, 8 per cell)
-    for (int c = threadIdx.x; c < nin; c += 256) {
-        const int lz = c % ie[2], ly = (c / ie[2]) % ie[1], lx = c / (ie[2] * ie[1]);
-        const size_t gi = gidx(a, o[0] - 2 + lx, o[1] - 2 + ly, o[2] - 2 + lz);
-        uint32_t w[4];
-        const uint16_t v0 = f2bf((float)a.E[gi]), v1 = f2bf((float)a.E[N + gi]), v2 = f2bf((float)a.E[2 * N + gi]);
-        const uint16_t v3 = f2bf((float)a.H[gi]), v4 = f2bf((float)a.H[N + gi]), v5 = f2bf((float)a.H[2 * N + gi]);
-        w[0] = (uint32_t)v0 | ((uint32_t)v1 << 16);
-        w[1] = (uint32_t)v2 | ((uint32_t)v3 << 16);
-        w[2] = (uint32_t)v4 | ((uint32_t)v5 << 16);
-        w[3] = 0;
-        s_in[c] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    // weight fragments of this lane
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// B fragments of both layers in lane order, per box: [kFrag1 + kFrag2][64
+// lanes][8] BF16, so a lane loads each fragment with one 16-byte read.
+// Layer 1 fragment j holds, for lane (g, n), W1[n][i][4j + g] for input
+// channel i < 6 (zero for i = 6, 7 and tap 27); layer 2 fragment j holds
+// W2[n][8(g & 1) + i][2j + g / 2] for output n < 3 (zero otherwise).
+constexpr int kFrag1 = 7, kFrag2 = 14;
+constexpr int kPackBox = (kFrag1 + kFrag2) * 64 * 8;      // BF16 values per box
+constexpr int kPackFloats = 6 * kPackBox / 2;             // FP32 words of scratch for 6 boxes
+
+__global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, uint16_t* packed) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nbox * kPackBox) return;
+    const int b = idx / kPackBox, r = idx % kPackBox;
+    const int j = r / 512, lane = (r / 8) % 64, i = r % 8;
     const int g = lane >> 4, n = lane & 15;
-    const uint16_t* w1 = a.w + (size_t)b * kW1;
-    const uint16_t* w2 = a.w + (size_t)a.nbox * kW1 + (size_t)b * kW2;
-    bf16x8 B1[7], B2[14];
-    for (int j = 0; j < 7; ++j) {
+    uint16_t v = 0;
+    if (j < kFrag1) {
         const int tap = 4 * j + g;
-        for (int i = 0; i < 8; ++i) {
-            const uint16_t v = (tap < kTaps && i < kIn) ? w1[(n * kIn + i) * kTaps + tap] : (uint16_t)0;
-            B1[j][i] = __builtin_bit_cast(__bf16, v);
-        }
+        if (tap < kTaps && i < kIn) v = w[(size_t)b * kW1 + (n * kIn + i) * kTaps + tap];
+    } else {
+        const int tap = 2 * (j - kFrag1) + (g >> 1);
+        const int k = 8 * (g & 1) + i;
+        if (tap < kTaps && n < kOut) v = w[(size_t)nbox * kW1 + (size_t)b * kW2 + (n * kHid + k) * kTaps + tap];
     }
-    for (int j = 0; j < 14; ++j) {
-        const int tap = 2 * j + (g >> 1);
-        for (int i = 0; i < 8; ++i) {
-            const int k = 8 * (g & 1) + i;
-            const uint16_t v = (tap < kTaps && n < kOut) ? w2[(n * kHid + k) * kTaps + tap] : (uint16_t)0;
-            B2[j][i] = __builtin_bit_cast(__bf16, v);
-        }
-    }
-    __syncthreads();
-    // 2. layer 1: hidden = conv1(input)
-    const int mt1 = (nhid + 15) / 16;
-    uint16_t* hid16 = reinterpret_cast<uint16_t*>(s_hid);
-    for (int mt = wave; mt < mt1; mt += 4) {
-        const int row = min(mt * 16 + n, nhid - 1);
-        const int hz = row % he[2], hy = (row / he[2]) % he[1], hx = row / (he[2] * he[1]);
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    packed[idx] = v;
+}
+
+// floor(c / d) for 0 <= c < 4096 and 1 <= d <= 64 from r = 1/d: (c + 0.5)/d
+// lies at least 1/(2d) >= 2^-7 from an integer and the product's error is
+// below 2^-9.
+__device__ __forceinline__ int qdiv(int c, float r) { return (int)(((float)c + 0.5f) * r); }
+
+// displacement along tile axis a of stencil tap t (offsets 0..2 per grid axis)
+__device__ __forceinline__ int tap_disp(int t, int grid_axis) {
+    return grid_axis == 0 ? t / 9 : (grid_axis == 1 ? (t / 3) % 3 : t % 3);
+}
+
+// periodic wrap of -2 <= c < 3s (a tile's input reaches 17 cells past its
+// origin, at most s + 15 < 3s with s >= 8)
+__device__ __forceinline__ int wrap2(int c, int s) {
+    if (c < 0) return c + s;
+    if (c >= s) c -= s;
+    return c >= s ? c - s : c;
+}
+
+constexpr int kInIter = (kInMax + 255) / 256;           // input cells per thread
+constexpr int kHidPerWave = (kHidRows + 3) / 4;         // hidden rows per wave
+
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf,
+                                                  int per_block) {
+    // the input tile [cell][8 x BF16], then (after layer 1) the hidden tile
+    // [2 planes][cell][8 x BF16] in the same words
+    __shared__ uint4 s_buf[kTileLds];
+    // workgroup -> a run of per_block consecutive tiles; hardware workgroup
+    // k runs on XCD k % 8, and each XCD gets a contiguous range of runs so
+    // the halos of neighbouring tiles meet in its L2
+    const int nb = gridDim.x, qq = nb >> 3, rr = nb & 7, xcd = blockIdx.x & 7, jj = blockIdx.x >> 3;
+    const int run = (xcd < rr ? xcd * (qq + 1) : rr * (qq + 1) + (xcd - rr) * qq) + jj;
+    const int first = run * per_block, last = min(first + per_block, tl.tstart[a.nbox]);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = lane >> 4, n = lane & 15;
+    for (int tile = first; tile < last; ++tile) {
+    int b = 0;
+    while (b < a.nbox - 1 && tile >= tl.tstart[b + 1]) ++b;
+    const ShellBox& B = a.box[b];
+    const int p0 = tl.perm[b][0], p1 = tl.perm[b][1], p2 = tl.perm[b][2];
+    const int te0 = tl.te[b][0], te1 = tl.te[b][1];
+    const int ie1 = te1 + 4, ie2 = kTileA2 + 4;
+    const int he0 = te0 + 2, he1 = te1 + 2;
+    const int tix = tile - tl.tstart[b];
+    const int t2 = tix % tl.nt[b][2], t1 = (tix / tl.nt[b][2]) % tl.nt[b][1], t0 = tix / (tl.nt[b][2] * tl.nt[b][1]);
+    int o[3];                                  // tile origin, grid axes
+    o[p0] = B.o0[p0] + t0 * te0;
+    o[p1] = B.o0[p1] + t1 * te1;
+    o[p2] = B.o0[p2] + t2 * kTileA2;
+    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kFrag2) * 64 + lane;
+    if (tile > first) __syncthreads();        // the previous tile's layer 2 is done with s_buf
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    // 1. input tile -> LDS (BF16, 8 per cell), loaded in grid order (z
+    // fastest); every load of a pass is issued before the first conversion
+    // (threads past the tile reload its last cell)
+    {
+        int ext[3], str[3];                    // input extents and LDS strides per grid axis
+        ext[p0] = te0 + 4; ext[p1] = ie1; ext[p2] = ie2;
+        str[p0] = ie1 * ie2; str[p1] = ie2; str[p2] = 1;
+        const int nin = ext[0] * ext[1] * ext[2];
+        const float r2 = 1.0f / ext[2], r1 = 1.0f / ext[1];
+        const T* E0 = a.E; const T* E1 = a.E + N; const T* E2 = a.E + 2 * N;
+        const T* H0 = a.H; const T* H1 = a.H + N; const T* H2 = a.H + 2 * N;
+        constexpr int U = sizeof(T) == 4 ? (kInIter + 1) / 2 : 2;   // cells per pass
+        for (int c0 = 0; c0 < nin; c0 += U * 256) {
+            float v[U][6];
+            int dst[U];
 #pragma unroll
-        for (int j = 0; j < 7; ++j) {
-            const int tap = min(4 * j + g, kTaps - 1);     // tap 27 has zero weights
-            const int ix = hx + 1 + tap / 9 - 1, iy = hy + 1 + (tap / 3) % 3 - 1, iz = hz + 1 + tap % 3 - 1;
-            const uint4 v = s_in[(ix * ie[1] + iy) * ie[2] + iz];
-            const bf16x8 av = __builtin_bit_cast(bf16x8, v);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, B1[j], acc, 0, 0, 0);
-        }
+            for (int u = 0; u < U; ++u) {
+                const int c = min(c0 + u * 256 + (int)threadIdx.x, nin - 1);
+                const int q = qdiv(c, r2), lz = c - q * ext[2];
+                const int lx = qdiv(q, r1), ly = q - lx * ext[1];
+                dst[u] = lx * str[0] + ly * str[1] + lz * str[2];
+                const uint32_t gi = (uint32_t)((wrap2(o[0] - 2 + lx, a.sx) * a.sy + wrap2(o[1] - 2 + ly, a.sy)) * a.sz +
+                                               wrap2(o[2] - 2 + lz, a.sz));
+                v[u][0] = (float)E0[gi]; v[u][1] = (float)E1[gi]; v[u][2] = (float)E2[gi];
+                v[u][3] = (float)H0[gi]; v[u][4] = (float)H1[gi]; v[u][5] = (float)H2[gi];
+            }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = mt * 16 + 4 * g + i;
-            if (r < nhid) hid16[r * kHid + n] = f2bf(acc[i]);
-        }
-    }
-    __syncthreads();
-    // 3. layer 2: y = conv2(hidden) on the tile's output cells
-    const int mt2 = (nout + 15) / 16;
-    for (int mt = wave; mt < mt2; mt += 4) {
-        const int row = min(mt * 16 + n, nout - 1);
-        const int oz = row % te[2], oy = (row / te[2]) % te[1], ox = row / (te[2] * te[1]);
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int j = 0; j < 14; ++j) {
-            const int tap = min(2 * j + (g >> 1), kTaps - 1);
-            const int hx = ox + 1 + tap / 9 - 1, hy = oy + 1 + (tap / 3) % 3 - 1, hz = oz + 1 + tap % 3 - 1;
-            const uint4 v = s_hid[((hx * he[1] + hy) * he[2] + hz) * 2 + (g & 1)];
-            const bf16x8 av = __builtin_bit_cast(bf16x8, v);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, B2[j], acc, 0, 0, 0);
-        }
-        if (n < kOut) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = mt * 16 + 4 * g + i;
-                if (r >= nout) continue;
-                const int rz = r % te[2], ry = (r / te[2]) % te[1], rx = r / (te[2] * te[1]);
-                const int gx = o[0] + rx, gy = o[1] + ry, gz = o[2] + rz;
-                if (gx >= B.o1[0] || gy >= B.o1[1] || gz >= B.o1[2]) continue;
-                const int bx = gx - B.o0[0], by = gy - B.o0[1], bz = gz - B.o0[2];
-                const int oi = tl.ostart[b] + (bx * (B.o1[1] - B.o0[1]) + by) * (B.o1[2] - B.o0[2]) + bz;
-                ybuf[(size_t)oi * kOut + n] = acc[i];
+            for (int u = 0; u < U; ++u) {
+                if (c0 + u * 256 + (int)threadIdx.x < nin)
+                    s_buf[dst[u]] = make_uint4(pack_bf16(v[u][0], v[u][1]), pack_bf16(v[u][2], v[u][3]),
+                                               pack_bf16(v[u][4], v[u][5]), 0u);
             }
         }
     }
+    bf16x8 B1[kFrag1];
+#pragma unroll
+    for (int j = 0; j < kFrag1; ++j) B1[j] = frag[j * 64];
+    __syncthreads();
+    // 2. layer 1: hidden row (h0, h1) = 16 cells along a2; a wave keeps its
+    // rows (BF16) in registers until every wave is done with the input tile
+    const int mt1 = he0 * he1;
+    uint32_t hreg[kHidPerWave][2];
+    {
+        int off1[kFrag1];                      // LDS word of this lane's input cell, from the row base
+#pragma unroll
+        for (int j = 0; j < kFrag1; ++j) {
+            const int tap = min(4 * j + g, kTaps - 1);     // tap 27 has zero weights
+            off1[j] = (tap_disp(tap, p0) * ie1 + tap_disp(tap, p1)) * ie2 + tap_disp(tap, p2) + n;
+        }
+        // rows wave + 4i; past the last row (T < 4) a wave repeats the last
+        // row, whose values it then writes again unchanged
+        int h0 = 0, h1 = wave;                 // row wave + 4i = h0 * he1 + h1 (he1 >= 10 > 4)
+#pragma unroll
+        for (int i = 0; i < kHidPerWave; ++i) {
+            if (i > 0) {
+                h1 += 4;
+                if (h1 >= he1) { h1 -= he1; ++h0; }
+            }
+            const int rh0 = h0 < he0 ? h0 : he0 - 1, rh1 = h0 < he0 ? h1 : he1 - 1;
+            const uint4* base = s_buf + (rh0 * ie1 + rh1) * ie2;
+            uint4 av[kFrag1];                  // every read of the row in flight before the first MFMA
+#pragma unroll
+            for (int j = 0; j < kFrag1; ++j) av[j] = base[off1[j]];
+            __builtin_amdgcn_sched_barrier(0);    // keeps the reads ahead of the MFMA chain
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int j = 0; j < kFrag1; ++j)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[j]), B1[j], acc, 0, 0, 0);
+            hreg[i][0] = pack_bf16(acc[0], acc[1]);
+            hreg[i][1] = pack_bf16(acc[2], acc[3]);
+        }
+    }
+    __syncthreads();
+    {
+        // cell (row mt, a2 = 4g + q), channel n -> plane n / 8, element n % 8
+        uint16_t* hid16 = reinterpret_cast<uint16_t*>(s_buf) + ((n >> 3) * kHidPlane + 4 * g) * 8 + (n & 7);
+#pragma unroll
+        for (int i = 0; i < kHidPerWave; ++i) {
+            const int mt = min(wave + 4 * i, mt1 - 1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                hid16[(mt * kRow + q) * 8] = (uint16_t)(hreg[i][q >> 1] >> (16 * (q & 1)));
+        }
+    }
+    __syncthreads();
+    // 3. layer 2: output row (o0, o1) = 14 cells along a2 (lanes 14, 15 of
+    // each MFMA read a valid cell and their rows are never used)
+    bf16x8 B2[kFrag2];                         // columns n >= 3 are zero: only 3 lanes of 16 load
+#pragma unroll
+    for (int j = 0; j < kFrag2; ++j) {
+        const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + j) * 64] : make_uint4(0u, 0u, 0u, 0u);
+        B2[j] = __builtin_bit_cast(bf16x8, v);
+    }
+    int off2[kFrag2];
+    {
+        const int nn = min(n, kTileA2 - 1);
+#pragma unroll
+        for (int j = 0; j < kFrag2; ++j) {
+            const int tap = min(2 * j + (g >> 1), kTaps - 1);
+            off2[j] = (tap_disp(tap, p0) * he1 + tap_disp(tap, p1)) * kRow + tap_disp(tap, p2) + nn +
+                      (g & 1) * kHidPlane;
+        }
+    }
+    const int mt2 = te0 * te1;
+    float* yt = ybuf + (size_t)tile * kOut * kTileRows + 4 * g;
+    int r0 = 0, r1 = wave;                     // row mt = r0 * te1 + r1 (te1 = 8 > 4)
+    for (int mt = wave; mt < mt2; mt += 4, r1 += 4) {
+        if (r1 >= te1) { r1 -= te1; ++r0; }
+        const uint4* base = s_buf + (r0 * he1 + r1) * kRow;
+        uint4 av[kFrag2];
+#pragma unroll
+        for (int j = 0; j < kFrag2; ++j) av[j] = base[off2[j]];
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < kFrag2; ++j)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[j]), B2[j], acc, 0, 0, 0);
+        if (n < kOut)      // slots 4g .. 4g+3 of row mt, output n
+            *reinterpret_cast<f32x4*>(yt + (size_t)n * kTileRows + mt * kRow) = acc;
+    }
+    }
 }
 
-// E += ybuf on the shell cells
+// E += correction on the shell cells: box cell l (grid axes) sits in tile
+// (l[p_a] / te_a) at slot ((l[p0] % te0) * te1 + l[p1] % te1) * 16 + l[p2] % te2
 template <class T>
-__global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, const float* ybuf, int total_out) {
+__global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, const float* ybuf, int total_out) {
     int cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= total_out) return;
-    const int oi = cell;
     int b = 0;
     for (; b < a.nbox; ++b) {
         const ShellBox& B = a.box[b];
@@ -545,11 +651,18 @@ __global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, const float* yb
     if (b >= a.nbox) return;
     const ShellBox& B = a.box[b];
     const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
-    const int lz = cell % nz, ly = (cell / nz) % ny, lx = cell / (nz * ny);
+    int l[3];
+    l[2] = cell % nz;
+    l[1] = (cell / nz) % ny;
+    l[0] = cell / (nz * ny);
+    const int u0 = l[tl.perm[b][0]], u1 = l[tl.perm[b][1]], u2 = l[tl.perm[b][2]];
+    const int t0 = u0 / tl.te[b][0], t1 = u1 / tl.te[b][1], t2 = u2 / tl.te[b][2];
+    const int tile = tl.tstart[b] + (t0 * tl.nt[b][1] + t1) * tl.nt[b][2] + t2;
+    const int slot = ((u0 - t0 * tl.te[b][0]) * tl.te[b][1] + (u1 - t1 * tl.te[b][1])) * kRow + (u2 - t2 * tl.te[b][2]);
     const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const size_t gi = gidx(a, B.o0[0] + lx, B.o0[1] + ly, B.o0[2] + lz);
+    const size_t gi = gidx(a, B.o0[0] + l[0], B.o0[1] + l[1], B.o0[2] + l[2]);
 #pragma unroll
-    for (int o = 0; o < kOut; ++o) a.Eout[o * N + gi] += (T)ybuf[(size_t)oi * kOut + o];
+    for (int o = 0; o < kOut; ++o) a.Eout[o * N + gi] += (T)ybuf[((size_t)tile * kOut + o) * kTileRows + slot];
 }
 
 struct ShellPlan {
@@ -558,6 +671,7 @@ struct ShellPlan {
     int hcells, out_cells;
     ShellTiles tiles;
     int ntiles;
+    size_t scratch_floats;   // hidden values (backward) or tile-major correction (forward)
 };
 
 ShellPlan make_shell(const Plan& P, int T) {
@@ -590,24 +704,27 @@ ShellPlan make_shell(const Plan& P, int T) {
     }
     S.hcells = off;
     S.out_cells = outc;
-    // MFMA tiles: the box's thin axis takes T, the other two 8 and 16
-    int ts = 0, os = 0;
+    // MFMA tiles: a0 = the box's thin axis (T), a2 = z for x and y faces, y
+    // for z faces (14), a1 = the remaining axis (8)
+    int ts = 0;
     for (int b = 0; b < nb; ++b) {
         const int r = b / 2;
-        int k = 0;
+        const int a2 = (r == 2) ? 1 : 2;
+        const int a1 = 3 - r - a2;
+        const int perm[3] = {r, a1, a2};
+        const int te[3] = {T, kTileA1, kTileA2};
         for (int q = 0; q < 3; ++q) {
-            const int te = (q == r) ? T : (k++ == 0 ? kTileA : kTileB);
-            const int ext = S.box[b].o1[q] - S.box[b].o0[q];
-            S.tiles.te[b][q] = te;
-            S.tiles.nt[b][q] = (ext + te - 1) / te;
+            const int ext = S.box[b].o1[perm[q]] - S.box[b].o0[perm[q]];
+            S.tiles.perm[b][q] = perm[q];
+            S.tiles.te[b][q] = te[q];
+            S.tiles.nt[b][q] = (ext + te[q] - 1) / te[q];
         }
         S.tiles.tstart[b] = ts;
-        S.tiles.ostart[b] = os;
         ts += S.tiles.nt[b][0] * S.tiles.nt[b][1] * S.tiles.nt[b][2];
-        os += (S.box[b].o1[0] - S.box[b].o0[0]) * (S.box[b].o1[1] - S.box[b].o0[1]) * (S.box[b].o1[2] - S.box[b].o0[2]);
     }
     S.tiles.tstart[nb] = ts;
     S.ntiles = ts;
+    S.scratch_floats = std::max((size_t)S.hcells * kHid, (size_t)ts * kOut * kTileRows);
     return S;
 }
 
@@ -628,6 +745,66 @@ int check_shell(const Plan& P, int T) {
     return 0;
 }
 
+// Scratch layout: [0, scratch_floats) the correction, tile-major
+// [tile][3][kTileRows] FP32 between the two forward launches (the backward
+// keeps hidden values there), then the packed weight fragments.
+uint16_t* packed_weights(const ShellPlan& S, float* scratch) {
+    return reinterpret_cast<uint16_t*>(scratch + S.scratch_floats);
+}
+
+// workgroups of shell_mfma resident on the current device at once
+int shell_slots(bool f32) {
+    static int cached[2][64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int& c = cached[f32 ? 0 : 1][dev];
+    if (c == 0) {
+        int cu = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) cu = 256;
+        const hipError_t e = f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_mfma<float>, 256, 0)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_mfma<double>, 256, 0);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+        c = cu * per_cu;
+    }
+    return c;
+}
+
+void launch_shell_pack(const ShellPlan& S, const uint16_t* w, float* scratch, hipStream_t st) {
+    hipLaunchKernelGGL(shell_pack, dim3((S.nbox * kPackBox + 255) / 256), dim3(256), 0, st, w, S.nbox,
+                       packed_weights(S, scratch));
+}
+
+// The correction kernels on stream st (weights already packed in scratch).
+void launch_shell_apply(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H,
+                        float* scratch, hipStream_t st) {
+    const int g2 = (S.out_cells + 255) / 256;
+    const uint16_t* pk = packed_weights(S, scratch);
+#ifndef FDTD_SHELL_RUN
+#define FDTD_SHELL_RUN 0
+#endif
+    // FDTD_SHELL_RUN = 0: one tile per workgroup; otherwise runs of
+    // ceil(tiles / resident workgroups) tiles
+    int per_block = 1;
+    if (FDTD_SHELL_RUN) {
+        const int slots = shell_slots(P.d.dtype == FDTD_F32);
+        per_block = (S.ntiles + slots - 1) / slots;
+    }
+    const int grid = (S.ntiles + per_block - 1) / per_block;
+    if (P.d.dtype == FDTD_F32) {
+        ShellArgs<float> a;
+        fill_shell<float>(P, S, a);
+        a.w = w; a.E = (const float*)E; a.H = (const float*)H; a.Eout = (float*)E;
+        hipLaunchKernelGGL(shell_mfma<float>, dim3(grid), dim3(256), 0, st, a, S.tiles, pk, scratch, per_block);
+        hipLaunchKernelGGL(shell_add<float>, dim3(g2), dim3(256), 0, st, a, S.tiles, (const float*)scratch, S.out_cells);
+    } else {
+        ShellArgs<double> a;
+        fill_shell<double>(P, S, a);
+        a.w = w; a.E = (const double*)E; a.H = (const double*)H; a.Eout = (double*)E;
+        hipLaunchKernelGGL(shell_mfma<double>, dim3(grid), dim3(256), 0, st, a, S.tiles, pk, scratch, per_block);
+        hipLaunchKernelGGL(shell_add<double>, dim3(g2), dim3(256), 0, st, a, S.tiles, (const float*)scratch, S.out_cells);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -636,7 +813,7 @@ int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness) {
     if (!plan) return -1;
     const Plan& P = *(const Plan*)plan;
     if (check_shell(P, thickness)) return -1;
-    return (int64_t)make_shell(P, thickness).hcells * kHid;
+    return (int64_t)make_shell(P, thickness).scratch_floats + kPackFloats;
 }
 
 int64_t fdtd_shell_weight_elems(void) { return 6 * (int64_t)(kW1 + kW2); }
@@ -648,23 +825,46 @@ int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, con
     if (int rc = check_shell(P, thickness)) return rc;
     if (!w || !E || !H || !hidden) return fail(2, "w, E, H and hidden are required");
     const ShellPlan S = make_shell(P, thickness);
-    hipStream_t st = (hipStream_t)stream;
-    const int g2 = (S.out_cells + 255) / 256;
-    // hidden scratch holds the correction (out_cells x 3 <= hcells x 16)
-    if (P.d.dtype == FDTD_F32) {
-        ShellArgs<float> a;
-        fill_shell<float>(P, S, a);
-        a.w = w; a.E = (const float*)E; a.H = (const float*)H; a.Eout = (float*)E;
-        hipLaunchKernelGGL(shell_mfma<float>, dim3(S.ntiles), dim3(256), 0, st, a, S.tiles, hidden);
-        hipLaunchKernelGGL(shell_add<float>, dim3(g2), dim3(256), 0, st, a, (const float*)hidden, S.out_cells);
-    } else {
-        ShellArgs<double> a;
-        fill_shell<double>(P, S, a);
-        a.w = w; a.E = (const double*)E; a.H = (const double*)H; a.Eout = (double*)E;
-        hipLaunchKernelGGL(shell_mfma<double>, dim3(S.ntiles), dim3(256), 0, st, a, S.tiles, hidden);
-        hipLaunchKernelGGL(shell_add<double>, dim3(g2), dim3(256), 0, st, a, (const float*)hidden, S.out_cells);
-    }
+    launch_shell_pack(S, w, hidden, (hipStream_t)stream);
+    launch_shell_apply(P, S, w, E, H, hidden, (hipStream_t)stream);
     return check_launch("fdtd_shell_apply");
+}
+
+int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness, const uint16_t* w, float* hidden,
+                      int32_t n, int64_t src_index, const void* src_wave, void* stream) {
+    if (!plan) return fail(1, "plan is NULL");
+    const Plan& P = *(const Plan*)plan;
+    if (int rc = check_shell(P, thickness)) return rc;
+    if (!w || !hidden) return fail(2, "w and hidden are required");
+    if (n < 0) return fail(3, "n = %d < 0", n);
+    if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
+        return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
+    if (P.d.poynting == FDTD_POYNTING_TWO_PASS) return fail(3, "two-pass Poynting is not supported with the shell");
+    if (int rc = check_fields(P, f, true)) return rc;
+    const ShellPlan S = make_shell(P, thickness);
+    hipStream_t st = (hipStream_t)stream;
+    fdtd_fields cur = *f;
+    const size_t esz = P.d.dtype == FDTD_F32 ? 4 : 8;
+    const bool swap = (f->H_out && f->H_out != f->H);
+    if (n > 0) launch_shell_pack(S, w, hidden, st);
+    for (int k = 0; k < n; ++k) {
+        const void* amp = src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
+        int rc = (P.d.dtype == FDTD_F32) ? launch_e<float>(P, &cur, st, (const float*)amp, src_index)
+                                         : launch_e<double>(P, &cur, st, (const double*)amp, src_index);
+        if (rc) return rc;
+        launch_shell_apply(P, S, w, cur.E, cur.H, hidden, st);
+        rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
+        if (rc) return rc;
+        if (swap) {
+            std::swap(cur.H, cur.H_out);
+            std::swap(cur.psi_h, cur.psi_h_out);
+            if (P.d.bloch_mask && cur.G_out && cur.G_out != cur.G) {
+                std::swap(cur.G, cur.G_out);
+                std::swap(cur.psi_g, cur.psi_g_out);
+            }
+        }
+    }
+    return check_launch("fdtd_shell_step_n");
 }
 
 int64_t fdtd_shell_partials_elems(void) { return (int64_t)64 * kW1; }

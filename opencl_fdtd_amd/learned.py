@@ -91,14 +91,16 @@ def run_interior(sc: Scenario, pad: int, pmls: List[Dict], sample_every: int = 1
     sim = Simulation(eps, pmls, dt=sc.dt, do_poynting=False, float_type=float_type)
     wave = torch.as_tensor(sc.wave(), dtype=sim._tdtype, device=sim.device)
     out = []
+    if correction is None:
+        for n0 in range(0, sc.steps - sample_every + 1, sample_every):
+            sim.step_n(sample_every, idx, wave[n0:n0 + sample_every])
+            out.append(sim.E.view(3, *shape)[(slice(None),) + sl].clone())
+        return torch.stack(out)
     for n in range(sc.steps):
-        if correction is None:
-            sim.step_n(1, idx, wave[n:n + 1])
-        else:
-            sim.update_E([])
-            sim.E[idx] += wave[n]
-            correction(sim, n)
-            sim.update_H([])
+        sim.update_E([])
+        sim.E[idx] += wave[n]
+        correction(sim, n)
+        sim.update_H([])
         if (n + 1) % sample_every == 0:
             out.append(sim.E.view(3, *shape)[(slice(None),) + sl].clone())
     return torch.stack(out)
@@ -188,6 +190,26 @@ class ShellRunner:
                                              ctypes.c_void_p(self.hidden.data_ptr()), self._st()),
                    "fdtd_shell_apply")
 
+    def step_n(self, n: int, src_index: Optional[int] = None, src_wave: Optional[torch.Tensor] = None) -> None:
+        """``n`` solver steps with the correction between E (+ source) and H,
+        launched natively (fdtd_shell_step_n); as Simulation.step_n."""
+        sim = self.sim
+        wave_ptr = None
+        if src_wave is not None:
+            wave = src_wave.to(device=sim.device, dtype=sim._tdtype).contiguous()
+            if wave.numel() < n:
+                raise ValueError("src_wave has %d values, need %d" % (wave.numel(), n))
+            self._wave_keepalive = wave
+            wave_ptr = ctypes.c_void_p(wave.data_ptr())
+        f = sim._fields()
+        _lib.check(self.lib.fdtd_shell_step_n(sim.plan, ctypes.byref(f), self.net.T,
+                                              ctypes.c_void_p(self.net.bf16.data_ptr()),
+                                              ctypes.c_void_p(self.hidden.data_ptr()), int(n),
+                                              int(src_index if src_index is not None else -1), wave_ptr,
+                                              self._st()), "fdtd_shell_step_n")
+        if len(sim._H) == 2 and n % 2 == 1:
+            sim._h = 1 - sim._h
+
     def backward(self, E, H, Ebar, Ebar_out, Hbar_out, wgrad) -> None:
         if self.hbar is None:
             self.hbar = torch.zeros_like(self.hidden)
@@ -211,13 +233,9 @@ def run_learned(sc: Scenario, net: ShellNet, sample_every: int = 10, float_type=
     runner = ShellRunner(sim, net)
     wave = torch.as_tensor(sc.wave(), dtype=sim._tdtype, device=sim.device)
     out = []
-    for n in range(sc.steps):
-        sim.update_E([])
-        sim._E[idx] += wave[n]
-        runner.apply(sim._E, sim._H[0])
-        sim.update_H([])
-        if (n + 1) % sample_every == 0:
-            out.append(sim._E.view(3, *shape)[(slice(None),) + sl].clone())
+    for n0 in range(0, sc.steps - sample_every + 1, sample_every):
+        runner.step_n(sample_every, idx, wave[n0:n0 + sample_every])
+        out.append(sim._E.view(3, *shape)[(slice(None),) + sl].clone())
     return torch.stack(out)
 
 

@@ -106,7 +106,8 @@ def _gpu_run(shape, T, w, E, H, Ebar=None):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4), ((13, 12, 17), 3),
-                                      ((20, 37, 25), 4)])
+                                      ((20, 37, 25), 4), ((8, 9, 30), 2), ((31, 8, 8), 2),
+                                      ((33, 29, 47), 3)])
 def test_hip_forward_matches_oracle(shape, T):
     """The MFMA inference kernel: same BF16 operands as the oracle's FP32
     emulation, FP32 accumulation in another order, so the correction agrees
@@ -143,3 +144,31 @@ def test_hip_backward_matches_oracle(shape, T):
     n1 = 6 * SN.W1_SIZE
     assert numpy.abs(gw[:n1] - ow[:n1]).max() <= 2e-5 * numpy.abs(ow[:n1]).max()
     assert numpy.abs(gw[n1:] - ow[n1:]).max() <= 1e-2 * numpy.abs(ow[n1:]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("poynting", [False, True])
+def test_shell_step_n_matches_stepwise(poynting):
+    """fdtd_shell_step_n (one native launch loop) against the same steps
+    made one call at a time: E step + source, correction, H step."""
+    import torch
+    from opencl_fdtd_amd import Simulation, learned as L
+    sc = L.make_scenario(7, interior=(24, 20, 28), steps=41, near_face=6, src_margin=4)
+    w = torch.as_tensor(_rand_weights(numpy.random.default_rng(8), 0.02), dtype=torch.float32)
+    net = L.ShellNet(4, weights=w)
+    shape, eps, idx, sl = L.embed(sc, 4)
+    wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device="cuda")
+    a = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=poynting)
+    b = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=poynting)
+    ra, rb = L.ShellRunner(a, net), L.ShellRunner(b, net)
+    ra.step_n(sc.steps, idx, wave)
+    for n in range(sc.steps):
+        b.update_E([])
+        b.E[idx] += wave[n]
+        rb.apply(b.E, b.H)
+        b.update_H([])
+    torch.cuda.synchronize()
+    assert float(torch.abs(a.E).max()) > 0
+    assert torch.equal(a.E, b.E) and torch.equal(a.H, b.H)
+    if poynting:
+        assert torch.equal(a.S, b.S)
