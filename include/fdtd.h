@@ -210,11 +210,13 @@ int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness);
 /* FP32 scratch for the weight-gradient partials. */
 int64_t fdtd_shell_partials_elems(void);
 /* E += correction(E, H) on the shell cells (E read before it is corrected). */
-int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
+int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, void* E, const void* H,
                      float* hidden, void* stream);
-/* n steps of fdtd_step_n with the correction applied between the E update
- * (and its source) and the H update; the same fields, source and swap rules
- * as fdtd_step_n (two-pass Poynting excluded). */
+/* n steps of the shell solver, each: E update and source (as fdtd_step_n);
+ * H update from that E; E += correction(E, H) with E and H as the H update
+ * read them (the addition is made by the next step's E update, and by a
+ * final pass after step n).  Swap rules as fdtd_step_n.  Real fields only
+ * (no Bloch boundaries), two-pass Poynting excluded. */
 int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness, const uint16_t* w,
                       float* hidden, int32_t n, int64_t src_index, const void* src_wave, void* stream);
 /* Adjoint of E_out = E + correction(E, H): given Ebar (adjoint of E_out),

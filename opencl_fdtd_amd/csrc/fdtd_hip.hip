@@ -146,6 +146,11 @@ struct EArgs {
     // fused point source: E_comp[src] += *src_amp after the update
     const T* src_amp;
     int src_x, src_p, src_comp;
+    // pending learned-shell correction: E += ycorr before the update on the
+    // cells within ycT of a face, stored per component in the shell's box
+    // order (shell_index); null: none
+    const float* ycorr[3];
+    int ycT;
 };
 
 template <class T>
@@ -378,7 +383,36 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, con
     sts_<FDTD_NT_E>(s.E[2], oi, ez);
 }
 
-template <class T, bool PAIR, bool SEAM>
+// The learned shell's cell order: the x faces (T x sy x sz each), then the
+// y faces ((sx-2T) x T x sz), then the z faces ((sx-2T) x (sy-2T) x T), C
+// order inside a box (the boxes of shell_net.inl).
+// Per-thread part of that index for a fixed plane cell (y, z): index =
+// xf + x * sy * sz on the x faces, base + (x - T) * step elsewhere (when
+// (y, z) is within T of a y or z face; in_yz false otherwise).
+struct ShellCol {
+    uint32_t xf, base, step;
+    bool in_yz;
+};
+
+__device__ __forceinline__ ShellCol shell_col(int y, int z, int T, int sx, int sy, int sz) {
+    const uint32_t bx = (uint32_t)T * sy * sz;
+    const int mx = sx - 2 * T, my = sy - 2 * T;
+    const uint32_t by = (uint32_t)mx * T * sz;
+    const uint32_t bz = (uint32_t)mx * my * T;
+    ShellCol c;
+    c.xf = (uint32_t)y * sz + z;
+    c.in_yz = true;
+    if (y < T) { c.base = 2 * bx + (uint32_t)y * sz + z; c.step = (uint32_t)T * sz; }
+    else if (y >= sy - T) { c.base = 2 * bx + by + (uint32_t)(y - (sy - T)) * sz + z; c.step = (uint32_t)T * sz; }
+    else if (z < T) { c.base = 2 * bx + 2 * by + (uint32_t)(y - T) * T + z; c.step = (uint32_t)my * T; }
+    else if (z >= sz - T) { c.base = 2 * bx + 2 * by + bz + (uint32_t)(y - T) * T + (z - (sz - T)); c.step = (uint32_t)my * T; }
+    else { c.base = 0; c.step = 0; c.in_yz = false; }
+    return c;
+}
+
+// SHELL: add the pending learned-shell correction (EArgs::ycorr) to E
+// before the update (real fields only).
+template <class T, bool PAIR, bool SEAM, bool SHELL = false>
 __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     const Geom<T>& g = a.g;
     int pb, ck;
@@ -407,6 +441,8 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     }
     const T iy = g.inv_h[1] ? g.inv_h[1][c.y] : g.inv_u;
     const T iz = g.inv_h[2] ? g.inv_h[2][c.z] : g.inv_u;
+    ShellCol sc{};
+    if (SHELL) sc = shell_col(c.y, c.z, a.ycT, g.sx, g.sy, g.sz);
 
     EIn<T> cur[NS], nxt[NS];
     T e0, e1, e2, n0 = (T)0, n1 = (T)0, n2 = (T)0;
@@ -420,6 +456,22 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     }
     for (int x = x0; x < x1; ++x) {
         const uint32_t ob = (uint32_t)x * pbytes;
+        // pending shell correction, loaded ahead of the plane's fields so
+        // the loads overlap
+        bool in_shell = false;
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        if (SHELL) {
+            const bool xlo = x < a.ycT, xhi = x >= g.sx - a.ycT;
+            in_shell = xlo || xhi || sc.in_yz;
+            if (in_shell) {
+                const uint32_t i = xlo   ? sc.xf + (uint32_t)x * g.plane
+                                   : xhi ? (uint32_t)a.ycT * g.plane + sc.xf + (uint32_t)(x - (g.sx - a.ycT)) * g.plane
+                                         : sc.base + (uint32_t)(x - a.ycT) * sc.step;
+                y0 = a.ycorr[0][i];
+                y1 = a.ycorr[1][i];
+                y2 = a.ycorr[2][i];
+            }
+        }
         if (!FDTD_PREFETCH && x > x0) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) e_fetch<T, SEAM>(g, a.s[k], c, ob, cur[k]);
@@ -434,6 +486,11 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
             n0 = lds_<FDTD_NT_E>(a.eps[0], on + c.op);
             n1 = lds_<FDTD_NT_E>(a.eps[1], on + c.op);
             n2 = lds_<FDTD_NT_E>(a.eps[2], on + c.op);
+        }
+        if (SHELL && in_shell) {
+            cur[0].ex = cur[0].ex + (T)y0;
+            cur[0].ey = cur[0].ey + (T)y1;
+            cur[0].ez = cur[0].ez + (T)y2;
         }
         const T ix = g.inv_h[0] ? g.inv_h[0][x] : g.inv_u;
         int src_comp = -1;
@@ -819,9 +876,14 @@ int check_launch(const char* what) {
 }
 
 template <class T>
-int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index) {
+int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index,
+             const float* ycorr = nullptr, int ycT = 0, int64_t ycells = 0) {
     EArgs<T> a;
     memset(&a, 0, sizeof(a));
+    if (ycorr) {
+        for (int r = 0; r < 3; ++r) a.ycorr[r] = ycorr + (size_t)r * ycells;
+        a.ycT = ycT;
+    }
     fill_geom<T>(P, P.le, f, a.g);
     const int64_t N = P.N;
     comp3c<T>(f->eps, N, a.eps);
@@ -852,7 +914,9 @@ int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_a
     }
     const dim3 grid(P.le.pblocks * P.le.nchunk);
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
-    if (!pair) hipLaunchKernelGGL((e_march<T, false, false>), grid, dim3(kBlock), 0, st, a);
+    if (ycorr && pair) return fail(3, "the shell correction is for real fields (no Bloch boundaries)");
+    if (ycorr) hipLaunchKernelGGL((e_march<T, false, false, true>), grid, dim3(kBlock), 0, st, a);
+    else if (!pair) hipLaunchKernelGGL((e_march<T, false, false>), grid, dim3(kBlock), 0, st, a);
     else if (!seam) hipLaunchKernelGGL((e_march<T, true, false>), grid, dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((e_march<T, true, true>), grid, dim3(kBlock), 0, st, a);
     return check_launch("fdtd_update_e");

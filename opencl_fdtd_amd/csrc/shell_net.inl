@@ -391,37 +391,47 @@ __global__ __launch_bounds__(256) void shell_wgrad_sum(const float* part, int gr
 // LDS; a2 is z for the x and y faces and y for the z faces (a1 is the
 // remaining axis).  The tile's input grown by 2 (T+4 x 12 x 18 cells) is
 // staged in LDS as BF16 [cell][8] (6 channels + 2 zero pads), its hidden
-// layer grown by 1 (T+2 x 10 x 16) as BF16 in two planes [8 channels][cell]
-// over the same LDS words.  Both layers are GEMMs over (tap, channel), with
-// M = 16 cells of one a2 row (a hidden row is exactly 16 cells; an output
-// row is 14, the last 2 rows of each MFMA are discarded):
-//   layer 1: K = 28 taps x 8 channels (tap 27 and channels 6, 7 are zero),
-//            N = 16 hidden channels;
-//   layer 2: K = 28 taps x 16 hidden channels, N = 16 (3 outputs used).
+// layer grown by 1 (T+2 x 10 x 16) as BF16 in two planes [cell][8 channels]
+// over the same LDS words.
+//
+// Each layer is a GEMM with the weights as the A operand (M = the layer's
+// output channels) and 16 cells of one a2 row as B (N; a hidden row is
+// exactly 16 cells, an output row 14 and two discarded), K running over
+// (tap, input channel).  A fragment (K = 32) covers one tap pair (d0, d1)
+// along a0, a1 with its three taps along a2 in the lane groups:
+//   layer 1: 9 fragments, group g < 3 = tap d2 = g x 8 input channels
+//            (6 used), group 3 zero;
+//   layer 2: 18 fragments, two per pair: (d2 = 0 channels 0-7, 8-15,
+//            d2 = 1 channels 0-7, 8-15) and (d2 = 2 channels 0-7, 8-15,
+//            zero, zero).
+// The fragment of pair (d0, d1) for row r reads the same cells as pair
+// (d0, d1 + 1) for row r - 1, so a wave walking a run of rows along a1 loads
+// each B fragment once and feeds it to up to 3 rows' MFMAs.
 // MFMA operands (16x16x32): lane l holds A row l%16 and B column l%16 at
-// k = 8*(l/16) + [0..7]; D column l%16 at rows 4*(l/16) + [0..3].  With the
-// K order above, a lane's 8 A values are one 16-byte LDS read at a per-lane
-// offset (fixed per fragment) from the row's base.  The correction goes to
-// a tile-major scratch [tile][3][T*8*16] and is added to E by shell_add,
-// since neighbouring tiles read E around their own cells.
+// k = 8*(l/16) + [0..7]; D column l%16 at rows 4*(l/16) + [0..3], i.e. for
+// cell l%16 the output channels 4*(l/16) + [0..3].  The correction goes to a
+// scratch in box order [3][shell cells] and is added to E later (by the next
+// E update, or by shell_add), since neighbouring tiles read E around their
+// own cells.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxT = 4;
 constexpr int kTileA1 = 8, kTileA2 = 14;                 // output cells along a1, a2
-constexpr int kRow = 16;                                  // MFMA rows = hidden a2 extent
-constexpr int kInMax = (kMaxT + 4) * (kTileA1 + 4) * (kTileA2 + 4);
+constexpr int kRow = 16;                                  // MFMA columns = hidden a2 extent
+constexpr int kInA0 = (kTileA1 + 4) * (kTileA2 + 4) + 1;  // input a0 stride (padded: no bank conflicts)
+constexpr int kInMax = (kMaxT + 4) * kInA0;
 constexpr int kHidRows = (kMaxT + 2) * (kTileA1 + 2);     // hidden a2 rows (of 16 cells)
 constexpr int kHidPlane = kHidRows * kRow;                // cells per hidden plane
-constexpr int kTileRows = kMaxT * kTileA1 * kRow;         // correction slots per tile and output
 constexpr int kTileLds = kInMax > 2 * kHidPlane ? kInMax : 2 * kHidPlane;   // uint4 words
-static_assert(kTileA2 + 2 == kRow, "hidden rows must be one MFMA tall");
+static_assert(kTileA2 + 2 == kRow, "hidden rows must be one MFMA wide");
 
 struct ShellTiles {
     int perm[6][3];    // grid axis of tile axes a0, a1, a2
     int te[6][3];      // output extents along a0, a1, a2
     int nt[6][3];      // tiles along a0, a1, a2
     int tstart[7];     // prefix of tiles per box
+    int ostart[7];     // prefix of output cells per box
 };
 
 // BF16 conversions with round-to-nearest-even (v_cvt_pk_bf16_f32)
@@ -433,29 +443,36 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, r);
 }
 
-// B fragments of both layers in lane order, per box: [kFrag1 + kFrag2][64
-// lanes][8] BF16, so a lane loads each fragment with one 16-byte read.
-// Layer 1 fragment j holds, for lane (g, n), W1[n][i][4j + g] for input
-// channel i < 6 (zero for i = 6, 7 and tap 27); layer 2 fragment j holds
-// W2[n][8(g & 1) + i][2j + g / 2] for output n < 3 (zero otherwise).
-constexpr int kFrag1 = 7, kFrag2 = 14;
+// grid-axis displacement (0..2) of tap t along grid axis q, and the tap of
+// tile-axis displacements (d0, d1, d2) for a tile with axis permutation p
+__host__ __device__ __forceinline__ int tap_of(const int* p, int d0, int d1, int d2) {
+    int d[3];
+    d[p[0]] = d0; d[p[1]] = d1; d[p[2]] = d2;
+    return (d[0] * 3 + d[1]) * 3 + d[2];
+}
+
+// A fragments in lane order, per box: [kFrag1 + kFrag2][64 lanes][8] BF16,
+// so a lane loads each fragment with one 16-byte read (layouts above).
+constexpr int kFrag1 = 9, kFrag2 = 18;
 constexpr int kPackBox = (kFrag1 + kFrag2) * 64 * 8;      // BF16 values per box
 constexpr int kPackFloats = 6 * kPackBox / 2;             // FP32 words of scratch for 6 boxes
 
-__global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, uint16_t* packed) {
+__global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, ShellTiles tl, uint16_t* packed) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= nbox * kPackBox) return;
     const int b = idx / kPackBox, r = idx % kPackBox;
-    const int j = r / 512, lane = (r / 8) % 64, i = r % 8;
+    const int f = r / 512, lane = (r / 8) % 64, i = r % 8;
     const int g = lane >> 4, n = lane & 15;
+    const int* p = tl.perm[b];
     uint16_t v = 0;
-    if (j < kFrag1) {
-        const int tap = 4 * j + g;
-        if (tap < kTaps && i < kIn) v = w[(size_t)b * kW1 + (n * kIn + i) * kTaps + tap];
+    if (f < kFrag1) {
+        if (g < 3 && i < kIn) v = w[(size_t)b * kW1 + (n * kIn + i) * kTaps + tap_of(p, f / 3, f % 3, g)];
     } else {
-        const int tap = 2 * (j - kFrag1) + (g >> 1);
-        const int k = 8 * (g & 1) + i;
-        if (tap < kTaps && n < kOut) v = w[(size_t)nbox * kW1 + (size_t)b * kW2 + (n * kHid + k) * kTaps + tap];
+        const int pair = (f - kFrag1) >> 1, h = (f - kFrag1) & 1;
+        const int d2 = h == 0 ? (g >> 1) : 2;
+        if (n < kOut && (h == 0 || g < 2))
+            v = w[(size_t)nbox * kW1 + (size_t)b * kW2 + (n * kHid + 8 * (g & 1) + i) * kTaps +
+                  tap_of(p, pair / 3, pair % 3, d2)];
     }
     packed[idx] = v;
 }
@@ -465,11 +482,6 @@ __global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, u
 // below 2^-9.
 __device__ __forceinline__ int qdiv(int c, float r) { return (int)(((float)c + 0.5f) * r); }
 
-// displacement along tile axis a of stencil tap t (offsets 0..2 per grid axis)
-__device__ __forceinline__ int tap_disp(int t, int grid_axis) {
-    return grid_axis == 0 ? t / 9 : (grid_axis == 1 ? (t / 3) % 3 : t % 3);
-}
-
 // periodic wrap of -2 <= c < 3s (a tile's input reaches 17 cells past its
 // origin, at most s + 15 < 3s with s >= 8)
 __device__ __forceinline__ int wrap2(int c, int s) {
@@ -478,48 +490,41 @@ __device__ __forceinline__ int wrap2(int c, int s) {
     return c >= s ? c - s : c;
 }
 
-constexpr int kInIter = (kInMax + 255) / 256;           // input cells per thread
-constexpr int kHidPerWave = (kHidRows + 3) / 4;         // hidden rows per wave
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const uint4& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+constexpr int kInIter = (kInMax + 255) / 256;           // input cells per thread (upper bound)
 
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf,
-                                                  int per_block) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf) {
     // the input tile [cell][8 x BF16], then (after layer 1) the hidden tile
     // [2 planes][cell][8 x BF16] in the same words
     __shared__ uint4 s_buf[kTileLds];
-    // workgroup -> a run of per_block consecutive tiles; hardware workgroup
-    // k runs on XCD k % 8, and each XCD gets a contiguous range of runs so
-    // the halos of neighbouring tiles meet in its L2
+    // hardware workgroup k runs on XCD k % 8; each XCD gets a contiguous
+    // range of tiles so the halos of neighbouring tiles meet in its L2
     const int nb = gridDim.x, qq = nb >> 3, rr = nb & 7, xcd = blockIdx.x & 7, jj = blockIdx.x >> 3;
-    const int run = (xcd < rr ? xcd * (qq + 1) : rr * (qq + 1) + (xcd - rr) * qq) + jj;
-    const int first = run * per_block, last = min(first + per_block, tl.tstart[a.nbox]);
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int g = lane >> 4, n = lane & 15;
-    for (int tile = first; tile < last; ++tile) {
+    const int tile = (xcd < rr ? xcd * (qq + 1) : rr * (qq + 1) + (xcd - rr) * qq) + jj;
     int b = 0;
     while (b < a.nbox - 1 && tile >= tl.tstart[b + 1]) ++b;
     const ShellBox& B = a.box[b];
     const int p0 = tl.perm[b][0], p1 = tl.perm[b][1], p2 = tl.perm[b][2];
-    const int te0 = tl.te[b][0], te1 = tl.te[b][1];
-    const int ie1 = te1 + 4, ie2 = kTileA2 + 4;
-    const int he0 = te0 + 2, he1 = te1 + 2;
+    const int te0 = tl.te[b][0];
+    const int ie2 = kTileA2 + 4, he1 = kTileA1 + 2, he0 = te0 + 2;
     const int tix = tile - tl.tstart[b];
     const int t2 = tix % tl.nt[b][2], t1 = (tix / tl.nt[b][2]) % tl.nt[b][1], t0 = tix / (tl.nt[b][2] * tl.nt[b][1]);
     int o[3];                                  // tile origin, grid axes
     o[p0] = B.o0[p0] + t0 * te0;
-    o[p1] = B.o0[p1] + t1 * te1;
+    o[p1] = B.o0[p1] + t1 * kTileA1;
     o[p2] = B.o0[p2] + t2 * kTileA2;
-    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kFrag2) * 64 + lane;
-    if (tile > first) __syncthreads();        // the previous tile's layer 2 is done with s_buf
     const size_t N = (size_t)a.sx * a.sy * a.sz;
     // 1. input tile -> LDS (BF16, 8 per cell), loaded in grid order (z
-    // fastest); every load of a pass is issued before the first conversion
-    // (threads past the tile reload its last cell)
+    // fastest) in passes whose loads are all issued before the first
+    // conversion (threads past the tile reload its last cell)
     {
         int ext[3], str[3];                    // input extents and LDS strides per grid axis
-        ext[p0] = te0 + 4; ext[p1] = ie1; ext[p2] = ie2;
-        str[p0] = ie1 * ie2; str[p1] = ie2; str[p2] = 1;
+        ext[p0] = te0 + 4; ext[p1] = kTileA1 + 4; ext[p2] = ie2;
+        str[p0] = kInA0; str[p1] = ie2; str[p2] = 1;
         const int nin = ext[0] * ext[1] * ext[2];
         const float r2 = 1.0f / ext[2], r1 = 1.0f / ext[1];
         const T* E0 = a.E; const T* E1 = a.E + N; const T* E2 = a.E + 2 * N;
@@ -547,122 +552,140 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
             }
         }
     }
-    bf16x8 B1[kFrag1];
-#pragma unroll
-    for (int j = 0; j < kFrag1; ++j) B1[j] = frag[j * 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = lane >> 4, n = lane & 15;
+    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kFrag2) * 64 + lane;
     __syncthreads();
-    // 2. layer 1: hidden row (h0, h1) = 16 cells along a2; a wave keeps its
-    // rows (BF16) in registers until every wave is done with the input tile
-    const int mt1 = he0 * he1;
-    uint32_t hreg[kHidPerWave][2];
+    // 2. layer 1: wave w takes a0 rows h0 = w/2, w/2 + 2, w/2 + 4 (< T+2) and
+    // along a1 the five rows h1 = 5 (w % 2) + [0, 5); B fragment (h0 + d0,
+    // y) feeds rows y - d1.  Results stay in registers (BF16, channels
+    // 4g .. 4g+3 of cell n) until every wave is done with the input tile.
+    constexpr int kR1 = 5, kH0 = 3;
+    uint32_t hreg[kH0][kR1][2];
     {
-        int off1[kFrag1];                      // LDS word of this lane's input cell, from the row base
+        bf16x8 W1[kFrag1];
 #pragma unroll
-        for (int j = 0; j < kFrag1; ++j) {
-            const int tap = min(4 * j + g, kTaps - 1);     // tap 27 has zero weights
-            off1[j] = (tap_disp(tap, p0) * ie1 + tap_disp(tap, p1)) * ie2 + tap_disp(tap, p2) + n;
-        }
-        // rows wave + 4i; past the last row (T < 4) a wave repeats the last
-        // row, whose values it then writes again unchanged
-        int h0 = 0, h1 = wave;                 // row wave + 4i = h0 * he1 + h1 (he1 >= 10 > 4)
+        for (int f = 0; f < kFrag1; ++f) W1[f] = frag[f * 64];
+        const int h1s = kR1 * (wave & 1);
+        const int lofs = n + min(g, 2);        // this lane's cell along a2 (group 3: any valid cell)
 #pragma unroll
-        for (int i = 0; i < kHidPerWave; ++i) {
-            if (i > 0) {
-                h1 += 4;
-                if (h1 >= he1) { h1 -= he1; ++h0; }
+        for (int k = 0; k < kH0; ++k) {
+            const int h0 = (wave >> 1) + 2 * k;
+            f32x4 acc[kR1];
+#pragma unroll
+            for (int r = 0; r < kR1; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (h0 < he0) {
+#pragma unroll
+                for (int d0 = 0; d0 < 3; ++d0) {
+                    const uint4* rowp = s_buf + (h0 + d0) * kInA0 + h1s * ie2 + lofs;
+#pragma unroll
+                    for (int y = 0; y < kR1 + 2; ++y) {
+                        const uint4 bv = rowp[y * ie2];
+#pragma unroll
+                        for (int d1 = 0; d1 < 3; ++d1) {
+                            const int r = y - d1;
+                            if (r >= 0 && r < kR1) acc[r] = mfma(W1[d0 * 3 + d1], bv, acc[r]);
+                        }
+                    }
+                }
             }
-            const int rh0 = h0 < he0 ? h0 : he0 - 1, rh1 = h0 < he0 ? h1 : he1 - 1;
-            const uint4* base = s_buf + (rh0 * ie1 + rh1) * ie2;
-            uint4 av[kFrag1];                  // every read of the row in flight before the first MFMA
 #pragma unroll
-            for (int j = 0; j < kFrag1; ++j) av[j] = base[off1[j]];
-            __builtin_amdgcn_sched_barrier(0);    // keeps the reads ahead of the MFMA chain
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int j = 0; j < kFrag1; ++j)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[j]), B1[j], acc, 0, 0, 0);
-            hreg[i][0] = pack_bf16(acc[0], acc[1]);
-            hreg[i][1] = pack_bf16(acc[2], acc[3]);
+            for (int r = 0; r < kR1; ++r) {
+                hreg[k][r][0] = pack_bf16(acc[r][0], acc[r][1]);
+                hreg[k][r][1] = pack_bf16(acc[r][2], acc[r][3]);
+            }
         }
     }
     __syncthreads();
     {
-        // cell (row mt, a2 = 4g + q), channel n -> plane n / 8, element n % 8
-        uint16_t* hid16 = reinterpret_cast<uint16_t*>(s_buf) + ((n >> 3) * kHidPlane + 4 * g) * 8 + (n & 7);
+        // hidden cell (h0, h1, n) channels 4g .. 4g+3 -> plane g / 2, half g % 2
+        char* hb = reinterpret_cast<char*>(s_buf) + (size_t)(g >> 1) * kHidPlane * 16 + n * 16 + (g & 1) * 8;
+        const int h1s = kR1 * (wave & 1);
 #pragma unroll
-        for (int i = 0; i < kHidPerWave; ++i) {
-            const int mt = min(wave + 4 * i, mt1 - 1);
+        for (int k = 0; k < kH0; ++k) {
+            const int h0 = (wave >> 1) + 2 * k;
+            if (h0 >= he0) continue;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                hid16[(mt * kRow + q) * 8] = (uint16_t)(hreg[i][q >> 1] >> (16 * (q & 1)));
+            for (int r = 0; r < kR1; ++r)
+                *reinterpret_cast<uint2*>(hb + ((h0 * he1 + h1s + r) * kRow) * 16) = make_uint2(hreg[k][r][0], hreg[k][r][1]);
         }
     }
     __syncthreads();
-    // 3. layer 2: output row (o0, o1) = 14 cells along a2 (lanes 14, 15 of
-    // each MFMA read a valid cell and their rows are never used)
-    bf16x8 B2[kFrag2];                         // columns n >= 3 are zero: only 3 lanes of 16 load
+    // 3. layer 2: wave w takes output a0 row o0 = w (< T) and its 8 rows
+    // along a1; B fragment (o0 + d0, y, half) feeds rows y - d1.  Lanes 14,
+    // 15 of each MFMA read a valid cell and their columns are never used.
+    const int o0 = wave;
+    if (o0 < te0) {
+        bf16x8 W2[kFrag2];                     // rows n >= 3 are zero: only 3 lanes of 16 load
 #pragma unroll
-    for (int j = 0; j < kFrag2; ++j) {
-        const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + j) * 64] : make_uint4(0u, 0u, 0u, 0u);
-        B2[j] = __builtin_bit_cast(bf16x8, v);
-    }
-    int off2[kFrag2];
-    {
+        for (int f = 0; f < kFrag2; ++f) {
+            const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + f) * 64] : make_uint4(0u, 0u, 0u, 0u);
+            W2[f] = __builtin_bit_cast(bf16x8, v);
+        }
+        constexpr int kR2 = kTileA1;
+        f32x4 acc[kR2];
+#pragma unroll
+        for (int r = 0; r < kR2; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int nn = min(n, kTileA2 - 1);
+        const uint4* plane = s_buf + (g & 1) * kHidPlane + nn;
 #pragma unroll
-        for (int j = 0; j < kFrag2; ++j) {
-            const int tap = min(2 * j + (g >> 1), kTaps - 1);
-            off2[j] = (tap_disp(tap, p0) * he1 + tap_disp(tap, p1)) * kRow + tap_disp(tap, p2) + nn +
-                      (g & 1) * kHidPlane;
+        for (int d0 = 0; d0 < 3; ++d0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int d2 = h == 0 ? (g >> 1) : 2;
+                const uint4* rowp = plane + (o0 + d0) * he1 * kRow + d2;
+#pragma unroll
+                for (int y = 0; y < kR2 + 2; ++y) {
+                    const uint4 bv = rowp[y * kRow];
+#pragma unroll
+                    for (int d1 = 0; d1 < 3; ++d1) {
+                        const int r = y - d1;
+                        if (r >= 0 && r < kR2) acc[r] = mfma(W2[(d0 * 3 + d1) * 2 + h], bv, acc[r]);
+                    }
+                }
+            }
         }
-    }
-    const int mt2 = te0 * te1;
-    float* yt = ybuf + (size_t)tile * kOut * kTileRows + 4 * g;
-    int r0 = 0, r1 = wave;                     // row mt = r0 * te1 + r1 (te1 = 8 > 4)
-    for (int mt = wave; mt < mt2; mt += 4, r1 += 4) {
-        if (r1 >= te1) { r1 -= te1; ++r0; }
-        const uint4* base = s_buf + (r0 * he1 + r1) * kRow;
-        uint4 av[kFrag2];
+        // outputs 0..2 of cell (o0, r, n) (tile axes) to the correction, in
+        // box order: ycorr[component][ostart[b] + box-local C index]
+        int ext[3], bst[3];
+        for (int q = 0; q < 3; ++q) ext[q] = B.o1[q] - B.o0[q];
+        bst[0] = ext[1] * ext[2]; bst[1] = ext[2]; bst[2] = 1;
+        const int l0 = o[p0] - B.o0[p0] + o0, l1 = o[p1] - B.o0[p1], l2 = o[p2] - B.o0[p2] + n;
+        if (g == 0 && n < kTileA2 && l0 < ext[p0] && l2 < ext[p2]) {
+            const int out = tl.ostart[a.nbox];
+            float* yc = ybuf + tl.ostart[b] + l0 * bst[p0] + l1 * bst[p1] + l2 * bst[p2];
 #pragma unroll
-        for (int j = 0; j < kFrag2; ++j) av[j] = base[off2[j]];
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int j = 0; j < kFrag2; ++j)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[j]), B2[j], acc, 0, 0, 0);
-        if (n < kOut)      // slots 4g .. 4g+3 of row mt, output n
-            *reinterpret_cast<f32x4*>(yt + (size_t)n * kTileRows + mt * kRow) = acc;
-    }
+            for (int r = 0; r < kR2; ++r) {
+                if (l1 + r < ext[p1]) {
+                    yc[r * bst[p1]] = acc[r][0];
+                    yc[out + r * bst[p1]] = acc[r][1];
+                    yc[2 * out + r * bst[p1]] = acc[r][2];
+                }
+            }
+        }
     }
 }
 
-// E += correction on the shell cells: box cell l (grid axes) sits in tile
-// (l[p_a] / te_a) at slot ((l[p0] % te0) * te1 + l[p1] % te1) * 16 + l[p2] % te2
+// E += correction on the shell cells, one thread per cell in box order
+// (z fastest within a box), the order of the correction's scratch.
 template <class T>
-__global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, const float* ybuf, int total_out) {
+__global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, const float* ycorr) {
     int cell = blockIdx.x * 256 + threadIdx.x;
-    if (cell >= total_out) return;
+    const int out = tl.ostart[a.nbox];
+    if (cell >= out) return;
+    const float y0 = ycorr[cell], y1 = ycorr[out + cell], y2 = ycorr[2 * out + cell];
     int b = 0;
-    for (; b < a.nbox; ++b) {
-        const ShellBox& B = a.box[b];
-        const int n = (B.o1[0] - B.o0[0]) * (B.o1[1] - B.o0[1]) * (B.o1[2] - B.o0[2]);
-        if (cell < n) break;
-        cell -= n;
-    }
-    if (b >= a.nbox) return;
+    while (b < a.nbox - 1 && cell >= tl.ostart[b + 1]) ++b;
+    cell -= tl.ostart[b];
     const ShellBox& B = a.box[b];
     const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
-    int l[3];
-    l[2] = cell % nz;
-    l[1] = (cell / nz) % ny;
-    l[0] = cell / (nz * ny);
-    const int u0 = l[tl.perm[b][0]], u1 = l[tl.perm[b][1]], u2 = l[tl.perm[b][2]];
-    const int t0 = u0 / tl.te[b][0], t1 = u1 / tl.te[b][1], t2 = u2 / tl.te[b][2];
-    const int tile = tl.tstart[b] + (t0 * tl.nt[b][1] + t1) * tl.nt[b][2] + t2;
-    const int slot = ((u0 - t0 * tl.te[b][0]) * tl.te[b][1] + (u1 - t1 * tl.te[b][1])) * kRow + (u2 - t2 * tl.te[b][2]);
+    const int lz = cell % nz, ly = (cell / nz) % ny, lx = cell / (nz * ny);
     const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const size_t gi = gidx(a, B.o0[0] + l[0], B.o0[1] + l[1], B.o0[2] + l[2]);
-#pragma unroll
-    for (int o = 0; o < kOut; ++o) a.Eout[o * N + gi] += (T)ybuf[((size_t)tile * kOut + o) * kTileRows + slot];
+    const size_t gi = ((size_t)(B.o0[0] + lx) * a.sy + (B.o0[1] + ly)) * a.sz + (B.o0[2] + lz);
+    a.Eout[gi] += (T)y0;
+    a.Eout[N + gi] += (T)y1;
+    a.Eout[2 * N + gi] += (T)y2;
 }
 
 struct ShellPlan {
@@ -714,7 +737,7 @@ ShellPlan make_shell(const Plan& P, int T) {
         const int perm[3] = {r, a1, a2};
         const int te[3] = {T, kTileA1, kTileA2};
         for (int q = 0; q < 3; ++q) {
-            const int ext = S.box[b].o1[perm[q]] - S.box[b].o0[perm[q]];
+            const int ext = S.box[b].o1[perm[q]] - S.box[b].o0[perm[q]];   // T along a0: one tile
             S.tiles.perm[b][q] = perm[q];
             S.tiles.te[b][q] = te[q];
             S.tiles.nt[b][q] = (ext + te[q] - 1) / te[q];
@@ -722,9 +745,15 @@ ShellPlan make_shell(const Plan& P, int T) {
         S.tiles.tstart[b] = ts;
         ts += S.tiles.nt[b][0] * S.tiles.nt[b][1] * S.tiles.nt[b][2];
     }
+    int oc = 0;
+    for (int b = 0; b < nb; ++b) {
+        S.tiles.ostart[b] = oc;
+        oc += (S.box[b].o1[0] - S.box[b].o0[0]) * (S.box[b].o1[1] - S.box[b].o0[1]) * (S.box[b].o1[2] - S.box[b].o0[2]);
+    }
+    S.tiles.ostart[nb] = oc;
     S.tiles.tstart[nb] = ts;
     S.ntiles = ts;
-    S.scratch_floats = std::max((size_t)S.hcells * kHid, (size_t)ts * kOut * kTileRows);
+    S.scratch_floats = std::max((size_t)S.hcells * kHid, (size_t)3 * S.out_cells);
     return S;
 }
 
@@ -745,64 +774,48 @@ int check_shell(const Plan& P, int T) {
     return 0;
 }
 
-// Scratch layout: [0, scratch_floats) the correction, tile-major
-// [tile][3][kTileRows] FP32 between the two forward launches (the backward
+// Scratch layout: [0, scratch_floats) the correction, [3][out_cells] FP32
+// in box order, between its computation and its addition to E (the backward
 // keeps hidden values there), then the packed weight fragments.
 uint16_t* packed_weights(const ShellPlan& S, float* scratch) {
     return reinterpret_cast<uint16_t*>(scratch + S.scratch_floats);
 }
 
-// workgroups of shell_mfma resident on the current device at once
-int shell_slots(bool f32) {
-    static int cached[2][64];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    int& c = cached[f32 ? 0 : 1][dev];
-    if (c == 0) {
-        int cu = 0, per_cu = 0;
-        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) cu = 256;
-        const hipError_t e = f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_mfma<float>, 256, 0)
-                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_mfma<double>, 256, 0);
-        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-        c = cu * per_cu;
-    }
-    return c;
-}
-
 void launch_shell_pack(const ShellPlan& S, const uint16_t* w, float* scratch, hipStream_t st) {
-    hipLaunchKernelGGL(shell_pack, dim3((S.nbox * kPackBox + 255) / 256), dim3(256), 0, st, w, S.nbox,
+    hipLaunchKernelGGL(shell_pack, dim3((S.nbox * kPackBox + 255) / 256), dim3(256), 0, st, w, S.nbox, S.tiles,
                        packed_weights(S, scratch));
 }
 
-// The correction kernels on stream st (weights already packed in scratch).
-void launch_shell_apply(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H,
-                        float* scratch, hipStream_t st) {
-    const int g2 = (S.out_cells + 255) / 256;
+// The correction (weights already packed in scratch) into the scratch, and
+// its addition to E: two launches, so that a caller can run something else
+// between them.
+template <class T>
+ShellArgs<T> shell_args(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H) {
+    ShellArgs<T> a;
+    fill_shell<T>(P, S, a);
+    a.w = w; a.E = (const T*)E; a.H = (const T*)H; a.Eout = (T*)E;
+    return a;
+}
+
+void launch_shell_mfma(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H,
+                       float* scratch, hipStream_t st) {
     const uint16_t* pk = packed_weights(S, scratch);
-#ifndef FDTD_SHELL_RUN
-#define FDTD_SHELL_RUN 0
-#endif
-    // FDTD_SHELL_RUN = 0: one tile per workgroup; otherwise runs of
-    // ceil(tiles / resident workgroups) tiles
-    int per_block = 1;
-    if (FDTD_SHELL_RUN) {
-        const int slots = shell_slots(P.d.dtype == FDTD_F32);
-        per_block = (S.ntiles + slots - 1) / slots;
-    }
-    const int grid = (S.ntiles + per_block - 1) / per_block;
-    if (P.d.dtype == FDTD_F32) {
-        ShellArgs<float> a;
-        fill_shell<float>(P, S, a);
-        a.w = w; a.E = (const float*)E; a.H = (const float*)H; a.Eout = (float*)E;
-        hipLaunchKernelGGL(shell_mfma<float>, dim3(grid), dim3(256), 0, st, a, S.tiles, pk, scratch, per_block);
-        hipLaunchKernelGGL(shell_add<float>, dim3(g2), dim3(256), 0, st, a, S.tiles, (const float*)scratch, S.out_cells);
-    } else {
-        ShellArgs<double> a;
-        fill_shell<double>(P, S, a);
-        a.w = w; a.E = (const double*)E; a.H = (const double*)H; a.Eout = (double*)E;
-        hipLaunchKernelGGL(shell_mfma<double>, dim3(grid), dim3(256), 0, st, a, S.tiles, pk, scratch, per_block);
-        hipLaunchKernelGGL(shell_add<double>, dim3(g2), dim3(256), 0, st, a, S.tiles, (const float*)scratch, S.out_cells);
-    }
+    if (P.d.dtype == FDTD_F32)
+        hipLaunchKernelGGL(shell_mfma<float>, dim3(S.ntiles), dim3(256), 0, st, shell_args<float>(P, S, w, E, H),
+                           S.tiles, pk, scratch);
+    else
+        hipLaunchKernelGGL(shell_mfma<double>, dim3(S.ntiles), dim3(256), 0, st, shell_args<double>(P, S, w, E, H),
+                           S.tiles, pk, scratch);
+}
+
+void launch_shell_add(const Plan& P, const ShellPlan& S, void* E, const float* scratch, hipStream_t st) {
+    const dim3 grid((S.out_cells + 255) / 256);
+    if (P.d.dtype == FDTD_F32)
+        hipLaunchKernelGGL(shell_add<float>, grid, dim3(256), 0, st, shell_args<float>(P, S, nullptr, E, nullptr),
+                           S.tiles, scratch);
+    else
+        hipLaunchKernelGGL(shell_add<double>, grid, dim3(256), 0, st, shell_args<double>(P, S, nullptr, E, nullptr),
+                           S.tiles, scratch);
 }
 
 }  // namespace
@@ -818,15 +831,17 @@ int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness) {
 
 int64_t fdtd_shell_weight_elems(void) { return 6 * (int64_t)(kW1 + kW2); }
 
-int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
+int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, void* E, const void* H,
                      float* hidden, void* stream) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
     if (int rc = check_shell(P, thickness)) return rc;
     if (!w || !E || !H || !hidden) return fail(2, "w, E, H and hidden are required");
     const ShellPlan S = make_shell(P, thickness);
-    launch_shell_pack(S, w, hidden, (hipStream_t)stream);
-    launch_shell_apply(P, S, w, E, H, hidden, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    launch_shell_pack(S, w, hidden, st);
+    launch_shell_mfma(P, S, w, E, H, hidden, st);
+    launch_shell_add(P, S, E, hidden, st);
     return check_launch("fdtd_shell_apply");
 }
 
@@ -840,6 +855,7 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
     if (P.d.poynting == FDTD_POYNTING_TWO_PASS) return fail(3, "two-pass Poynting is not supported with the shell");
+    if (P.d.bloch_mask) return fail(3, "the shell correction is for real fields (no Bloch boundaries)");
     if (int rc = check_fields(P, f, true)) return rc;
     const ShellPlan S = make_shell(P, thickness);
     hipStream_t st = (hipStream_t)stream;
@@ -847,23 +863,25 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     const size_t esz = P.d.dtype == FDTD_F32 ? 4 : 8;
     const bool swap = (f->H_out && f->H_out != f->H);
     if (n > 0) launch_shell_pack(S, w, hidden, st);
+    // step k: E update (+ the correction of step k-1, added before the
+    // update) and source; correction y(E, H) into the scratch; H update from
+    // the uncorrected E.  The last step's correction is added at the end.
     for (int k = 0; k < n; ++k) {
         const void* amp = src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
-        int rc = (P.d.dtype == FDTD_F32) ? launch_e<float>(P, &cur, st, (const float*)amp, src_index)
-                                         : launch_e<double>(P, &cur, st, (const double*)amp, src_index);
+        const float* pending = k > 0 ? hidden : nullptr;
+        int rc = (P.d.dtype == FDTD_F32)
+                     ? launch_e<float>(P, &cur, st, (const float*)amp, src_index, pending, thickness, S.out_cells)
+                     : launch_e<double>(P, &cur, st, (const double*)amp, src_index, pending, thickness, S.out_cells);
         if (rc) return rc;
-        launch_shell_apply(P, S, w, cur.E, cur.H, hidden, st);
+        launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
         rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
         if (rc) return rc;
         if (swap) {
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
-            if (P.d.bloch_mask && cur.G_out && cur.G_out != cur.G) {
-                std::swap(cur.G, cur.G_out);
-                std::swap(cur.psi_g, cur.psi_g_out);
-            }
         }
     }
+    if (n > 0) launch_shell_add(P, S, cur.E, hidden, st);
     return check_launch("fdtd_shell_step_n");
 }
 

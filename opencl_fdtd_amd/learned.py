@@ -124,7 +124,11 @@ W2_SIZE = N_OUT * N_HID * N_TAP
 N_WEIGHTS = 6 * (W1_SIZE + W2_SIZE)
 
 
-BASE_PML = {"cfs_alpha": 0.05}   # the CPML the shell runs under the correction
+# The CPML the shell runs under the correction: a 4-cell CPML tuned for the
+# interior-field error against a 48-cell padded run (graded sigma of order 2,
+# kappa_max = sqrt(6), CFS alpha 0.15), far better at 4 cells than the
+# defaults of simulation.py:161-168.
+BASE_PML = {"ln_R_per_layer": -2.2, "m": 2.0, "cfs_alpha": 0.15, "epsilon_eff": 6.0, "ma": 1}
 
 
 class ShellNet:
@@ -168,7 +172,12 @@ class ShellNet:
 
 
 class ShellRunner:
-    """Applies a ShellNet to one Simulation (the thin domain with CPML-T)."""
+    """Applies a ShellNet to one Simulation (the thin domain with CPML-T).
+
+    One step of the shell solver: E update and source (E_s); H update from
+    E_s; E = E_s + correction(E_s, H before the update).  ``step`` makes one
+    such step call by call, ``step_n`` makes n natively (the correction is
+    then added by the next step's E update)."""
 
     def __init__(self, sim: Simulation, net: ShellNet):
         self.sim, self.net = sim, net
@@ -190,9 +199,23 @@ class ShellRunner:
                                              ctypes.c_void_p(self.hidden.data_ptr()), self._st()),
                    "fdtd_shell_apply")
 
+    def step(self, src_index: Optional[int] = None, amp=None, keep: Optional[List[torch.Tensor]] = None) -> None:
+        """One shell step call by call; ``keep`` receives a copy of E after
+        the E update and source (the correction's input)."""
+        sim = self.sim
+        sim.update_E([])
+        if src_index is not None:
+            sim._E[src_index] += amp
+        if keep is not None:
+            keep.append(sim._E.clone())
+        Ec = sim._E.clone()
+        self.apply(Ec, sim._H[sim._h])
+        sim.update_H([])
+        sim._E.copy_(Ec)
+
     def step_n(self, n: int, src_index: Optional[int] = None, src_wave: Optional[torch.Tensor] = None) -> None:
-        """``n`` solver steps with the correction between E (+ source) and H,
-        launched natively (fdtd_shell_step_n); as Simulation.step_n."""
+        """``n`` shell steps launched natively (fdtd_shell_step_n); source as
+        Simulation.step_n."""
         sim = self.sim
         wave_ptr = None
         if src_wave is not None:
@@ -274,23 +297,14 @@ class ShellTrainer:
         runner = ShellRunner(sim, self.net)
         shape = sim.shape
         wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device=sim.device)
-        wave_host = wave.double().cpu().numpy()
-        E, H = sim._E, sim._H[0]
+        E = sim._E
         Ev = lambda t: t.view(3, *shape)[(slice(None),) + sl]
-
-        def step(n, keep=None):
-            sim.update_E([])
-            E[idx] += wave[n]
-            if keep is not None:
-                keep.append(E.clone())           # E after the E step and source, before the correction
-            runner.apply(E, H)
-            sim.update_H([])
 
         cps, loss = [], torch.zeros((), dtype=torch.float64, device=sim.device)
         for n in range(sc.steps):
             if n % self.K == 0:
                 cps.append(sim.checkpoint())
-            step(n)
+            runner.step(idx, wave[n])
             loss += torch.sum((Ev(E) - wide[n]).double() ** 2)
         # reverse
         N3 = 3 * sim.N
@@ -311,28 +325,28 @@ class ShellTrainer:
         for s in reversed(range(len(cps))):
             n0, n1 = s * self.K, min(s * self.K + self.K, sc.steps)
             sim.restore(cps[s])
-            Es, Hs, Em = [E.clone()], [H.clone()], []
+            Es, Hs, Em = [E.clone()], [sim._H[sim._h].clone()], []
             for n in range(n0, n1):
-                step(n, Em)
+                runner.step(idx, wave[n], keep=Em)
                 Es.append(E.clone())
-                Hs.append(H.clone())
+                Hs.append(sim._H[sim._h].clone())
             for n in reversed(range(n0, n1)):
                 k = n - n0
-                # loss adjoint of E after step n
+                # step n: Em = Estep(E_k, H_k) + source; H_k+1 = Hstep(H_k, Em);
+                # E_k+1 = Em + y(Em, H_k).  Eb: adjoint of E_k+1 (+ the loss).
                 Ev(Eb).add_((2.0 / Z) * (Ev(Es[k + 1]) - wide[n]))
-                b.E_prev, b.E_next = Es[k].data_ptr(), Es[k + 1].data_ptr()
+                Eb_mid.copy_(Eb)                 # adjoint of E_k+1, the correction's output
+                b.E_prev, b.E_next = Es[k].data_ptr(), Em[k].data_ptr()
                 b.H_prev, b.H_next = Hs[k].data_ptr(), Hs[k + 1].data_ptr()
                 b.Eb, b.Hb = Eb.data_ptr(), Hb.data_ptr()
                 b.psib_e, b.psib_e_out = pe[cur].data_ptr(), pe[1 - cur].data_ptr()
                 b.psib_h, b.psib_h_out = ph[cur].data_ptr(), ph[1 - cur].data_ptr()
-                b.parts = 1                      # the H step
+                b.parts = 1                      # the H step: Hb -> adjoint of H_k, Eb += its Em part
                 _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint H")
-                Eb_mid.copy_(Eb)                 # the correction: E_out = E_mid + y(E_mid, H_k)
-                runner.backward(Em[k], Hs[k], Eb, Eb_mid, Hb, wgrad)
-                Eb, Eb_mid = Eb_mid, Eb
-                b.Eb = Eb.data_ptr()
-                b.E_next = Em[k].data_ptr()
-                b.parts = 2                      # the E step
+                # the correction: Eb += its Em part, Hb += its H_k part (the H step's
+                # Jacobian in H is the identity, so adding after its adjoint is exact)
+                runner.backward(Em[k], Hs[k], Eb_mid, Eb, Hb, wgrad)
+                b.parts = 2                      # the E step: Eb -> adjoint of E_k, Hb += its part
                 _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint E")
                 cur = 1 - cur
             del Es, Hs, Em
@@ -349,18 +363,25 @@ class ShellTrainer:
         self.net.sync()
 
     def train(self, iterations: int, log: Optional[Callable[[int, float], None]] = None,
-              decay_every: int = 0, decay: float = 0.5) -> List[float]:
+              decay_every: int = 0, decay: float = 0.5, batch: int = 1) -> List[float]:
+        """``iterations`` Adam steps, each on the mean loss and gradient of
+        ``batch`` scenarios taken in turn; returns the mean losses."""
         hist = []
         lr0 = self.lr
+        k = 0
         for it in range(iterations):
             if decay_every:
                 self.lr = lr0 * decay ** (it // decay_every)
-            case = self.cases[it % len(self.cases)]
-            L, g = self.loss_and_grad(case)
-            self.adam(g)
-            hist.append(L)
+            Ls, gs = 0.0, None
+            for _ in range(batch):
+                L, g = self.loss_and_grad(self.cases[k % len(self.cases)])
+                k += 1
+                Ls += L / batch
+                gs = g / batch if gs is None else gs.add_(g, alpha=1.0 / batch)
+            self.adam(gs)
+            hist.append(Ls)
             if log:
-                log(it, L)
+                log(it, Ls)
         return hist
 
 
@@ -375,18 +396,19 @@ import time  # noqa: E402
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "shell4.npz")
 
 
-def train_scenarios(n: int, seed0: int = 100, interior=(48, 48, 48), steps: int = 300) -> List[Scenario]:
+def train_scenarios(n: int, seed0: int = 100, interior=(64, 64, 48), steps: int = 300) -> List[Scenario]:
     """Training trajectories: small interiors with eps ~ U[1, 4] near the
-    faces and random dipoles (position, polarisation, wavelength U[12, 40])."""
-    return [make_scenario(seed0 + s, interior=interior, steps=steps, near_face=12, width_wl=1.0)
+    faces and random dipoles (position anywhere 4 or more cells inside,
+    polarisation, wavelength U[12, 40])."""
+    return [make_scenario(seed0 + s, interior=interior, steps=steps, near_face=16, src_margin=4, width_wl=1.0)
             for s in range(n)]
 
 
-def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print) -> ShellNet:
+def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4) -> ShellNet:
     net = ShellNet(4, seed=1)
     tr = ShellTrainer(net, train_scenarios(n_scen), checkpoint_every=60, lr=lr)
     t0 = time.time()
-    tr.train(iterations, decay_every=max(iterations // 3, 1),
+    tr.train(iterations, decay_every=max(iterations // 3, 1), batch=batch,
              log=lambda it, L: log("it %d loss %.2f dB lr %.1e (%.0f s)" % (it, 10 * numpy.log10(L), tr.lr,
                                                                             time.time() - t0)))
     os.makedirs(os.path.dirname(out), exist_ok=True)
@@ -420,10 +442,12 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=0, help="training iterations")
     ap.add_argument("--lr", type=float, default=5e-5)
     ap.add_argument("--scenarios", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=4, help="scenarios per Adam step")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
     a = ap.parse_args(argv)
-    net = train_main(a.train, a.lr, a.scenarios, a.weights) if a.train else ShellNet.load(a.weights)
+    net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch) if a.train
+           else ShellNet.load(a.weights))
     if a.evaluate:
         evaluate_main(net, range(1000, 1000 + a.evaluate))
 
