@@ -177,7 +177,7 @@ def precision_report():
     return rows
 
 
-def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001)):
+def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), profile_only: bool = False):
     """Interior 256x256x128: interior Gcell-updates/s of CPML-16, CPML-8, the
     learned 4-cell shell and its CPML-4 base (Poynting off), and reflection
     (dB vs a 48-cell padded run) of each on unseen scenarios."""
@@ -214,6 +214,9 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001)):
                       "total_gcell_s": int(numpy.prod(shape)) * steps / ms / 1e6}
         print(json.dumps({name: rows[name]}), file=sys.stderr, flush=True)
 
+    if profile_only:                           # the learned-4 loop alone (for kernel profiles)
+        timed("learned4", 4, net.faces(), net)
+        return rows
     timed("cpml16", 16, L.cpml_faces(16))
     timed("cpml8", 8, L.cpml_faces(8))
     timed("learned4", 4, net.faces(), net)
@@ -239,6 +242,8 @@ def main():
                     help="per-config FP32 vs FP64 and FP32 vs oracle report instead of the benchmark")
     ap.add_argument("--learned", action="store_true",
                     help="learned 4-cell shell vs CPML-8/16: interior speed and reflection")
+    ap.add_argument("--learned-profile", action="store_true",
+                    help="with --learned: only the timed learned-4 loop (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
@@ -249,7 +254,7 @@ def main():
         precision_report()
         return
     if args.learned:
-        learned_report()
+        learned_report(profile_only=args.learned_profile)
         return
 
     import torch

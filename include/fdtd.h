@@ -28,6 +28,9 @@
  *   fdtd_adjoint_step, fdtd_plane_sum, fdtd_energy, fdtd_reduce_partials
  *                      <- no reference counterpart (gradients and FP64
  *                         reductions over the same step; see below)
+ *   fdtd_shell_*       <- no reference counterpart (the learned 4-cell
+ *                         absorbing shell that stands in for a thick CPML;
+ *                         see below)
  *
  * Field layout (kernels/common.cl:34-61, fdfd_tools.vec at simulation.py:347):
  * a vector field is 3 contiguous components of N = sx*sy*sz values each, and
@@ -195,12 +198,15 @@ int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_pr
 
 /* ------------------------------------------------------------------------
  * Learned absorbing shell.  No counterpart in the reference: a thin shell of
- * T cells on every face runs the CPML-T update, and after each E update (and
- * source) a linear 2-layer 3x3x3 convolution (6 -> 16 -> 3 channels: E, H in,
- * a correction to E out) is added to E on the shell cells.  Six weight sets,
- * one per face box; BF16 weights and activations, FP32 accumulation.
+ * T cells on every face runs the CPML-T update, and each step a linear
+ * 2-layer 3x3x3 convolution (6 -> 16 -> 3 channels: E, H in, a correction
+ * to E out) of the fields after the E update (and source) is added to E on
+ * the shell cells.  Six weight sets, one per face box, each with one
+ * layer-2 head per depth (index along the box's thin axis); BF16 weights
+ * and activations, FP32 accumulation.
  * Weights: fdtd_shell_weight_elems() BF16 values, W1[box][16][6][27] for the
- * six boxes, then W2[box][3][16][27]; taps t = 9(dx+1) + 3(dy+1) + (dz+1).
+ * six boxes, then W2[box][depth 0..3][3][16][27]; taps
+ * t = 9(dx+1) + 3(dy+1) + (dz+1).
  * ------------------------------------------------------------------------ */
 int64_t fdtd_shell_weight_elems(void);
 /* FP32 scratch of the correction (the hidden layer or the correction

@@ -70,6 +70,9 @@ namespace {
 #ifndef FDTD_E_WAVES
 #define FDTD_E_WAVES 1               // __launch_bounds__ min waves per SIMD, E kernel
 #endif
+#ifndef FDTD_H_TARGET
+#define FDTD_H_TARGET 0              // fused-S H workgroups per launch (0: as many threads as the E launch)
+#endif
 #ifndef FDTD_H_WAVES
 #define FDTD_H_WAVES 6               // same, H kernel
 #endif
@@ -1069,9 +1072,10 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         // whole rows per workgroup plus one halo row
         if (P->sz <= 128)
             P->lh = make_launch(P->plane, P->sx, FDTD_HB_SMALL, ((FDTD_HB_SMALL - P->sz) / P->sz) * P->sz, 2,
-                                kTargetBlocks * kBlock / FDTD_HB_SMALL);
+                                FDTD_H_TARGET ? FDTD_H_TARGET : kTargetBlocks * kBlock / FDTD_HB_SMALL);
         else if (P->sz <= 341)
-            P->lh = make_launch(P->plane, P->sx, 1024, ((1024 - P->sz) / P->sz) * P->sz, 2, kTargetBlocks / 4);
+            P->lh = make_launch(P->plane, P->sx, 1024, ((1024 - P->sz) / P->sz) * P->sz, 2,
+                                FDTD_H_TARGET ? FDTD_H_TARGET : kTargetBlocks / 4);
         else
             P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, 3, kTargetBlocks);
     } else {

@@ -5,8 +5,10 @@
 // shell cells:
 //   x = (Ex, Ey, Ez, Hx, Hy, Hz)                         6 input channels
 //   h_k(j) = sum_{c,t} W1[k][c][t] x_c(j + t)            16 hidden channels
-//   y_a(i) = sum_{k,t} W2[a][k][t] h_k(i + t)            3 outputs
+//   y_a(i) = sum_{k,t} W2[d(i)][a][k][t] h_k(i + t)      3 outputs
 //   E_a(i) += y_a(i)  for i in the shell
+// with d(i) the depth of i: its index along its box's thin axis (a layer-2
+// head per depth).
 // t runs over the 27 offsets of a 3x3x3 stencil (periodic wrap, as the step).
 // There is no activation: the correction must be linear in the fields, as
 // Maxwell's equations are.  Weights and activations are BF16 (inputs rounded
@@ -23,8 +25,9 @@ constexpr int kHid = 16;
 constexpr int kIn = 6;
 constexpr int kOut = 3;
 constexpr int kTaps = 27;
+constexpr int kDepth = 4;                   // layer-2 heads per box, one per shell depth (T <= 4)
 constexpr int kW1 = kHid * kIn * kTaps;    // per box
-constexpr int kW2 = kOut * kHid * kTaps;   // per box
+constexpr int kW2 = kOut * kHid * kTaps;   // per head
 
 // Along an axis the box spans completely (full), its grown regions are the
 // same periodic range (local coordinate = global coordinate, wrapping);
@@ -146,42 +149,6 @@ __global__ __launch_bounds__(256) void shell_l1(ShellArgs<T> a, int total) {
     for (int k = 0; k < kHid; ++k) h[k] = bf16_round(acc[k]);
 }
 
-// layer 2 over the output cells, adding the correction to Eout
-template <class T>
-__global__ __launch_bounds__(256) void shell_l2(ShellArgs<T> a, int total_out) {
-    int cell = blockIdx.x * 256 + threadIdx.x;
-    if (cell >= total_out) return;
-    int b = 0;
-    for (; b < a.nbox; ++b) {
-        const ShellBox& B = a.box[b];
-        const int n = (B.o1[0] - B.o0[0]) * (B.o1[1] - B.o0[1]) * (B.o1[2] - B.o0[2]);
-        if (cell < n) break;
-        cell -= n;
-    }
-    if (b >= a.nbox) return;
-    const ShellBox& B = a.box[b];
-    const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
-    const int lz = cell % nz, ly = (cell / nz) % ny, lx = cell / (nz * ny);
-    const uint16_t* w2 = a.w + (size_t)a.nbox * kW1 + (size_t)b * kW2;
-    float y[kOut] = {0.f, 0.f, 0.f};
-    for (int t = 0; t < kTaps; ++t) {
-        const int hx = hid_local(B, 0, B.o0[0] + lx + t / 9 - 1);
-        const int hy = hid_local(B, 1, B.o0[1] + ly + (t / 3) % 3 - 1);
-        const int hz = hid_local(B, 2, B.o0[2] + lz + t % 3 - 1);
-        const float* h = a.hid + ((size_t)B.hoff + ((size_t)hx * B.e[1] + hy) * B.e[2] + hz) * kHid;
-#pragma unroll
-        for (int k = 0; k < kHid; ++k) {
-            const float hv = h[k];
-#pragma unroll
-            for (int o = 0; o < kOut; ++o) y[o] += bf16_to_f(w2[(o * kHid + k) * kTaps + t]) * hv;
-        }
-    }
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const size_t gi = gidx(a, B.o0[0] + lx, B.o0[1] + ly, B.o0[2] + lz);
-#pragma unroll
-    for (int o = 0; o < kOut; ++o) a.Eout[o * N + gi] += (T)y[o];
-}
-
 // ---- backward (training): adjoint of E_out = E + mask * y(E, H) ----------
 
 // hbar_k(j) = sum_{o,t} W2[o][k][t] ybar_o(j - t), ybar = Ebar on the box's
@@ -193,7 +160,8 @@ __global__ __launch_bounds__(256) void shell_l2_t(ShellArgs<T> a, int total) {
     if (cell >= total || !locate(a, cell, &b, &lx, &ly, &lz)) return;
     const ShellBox& B = a.box[b];
     const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const uint16_t* w2 = a.w + (size_t)a.nbox * kW1 + (size_t)b * kW2;
+    const uint16_t* w2b = a.w + (size_t)a.nbox * kW1 + (size_t)b * kDepth * kW2;
+    const int r = b / 2;                       // the box's thin axis: depth = output-local index along it
     float acc[kHid];
 #pragma unroll
     for (int k = 0; k < kHid; ++k) acc[k] = 0.f;
@@ -204,6 +172,7 @@ __global__ __launch_bounds__(256) void shell_l2_t(ShellArgs<T> a, int total) {
         const int oy = out_local(B, 1, gy - ((t / 3) % 3 - 1));
         const int oz = out_local(B, 2, gz - (t % 3 - 1));
         if (ox < 0 || oy < 0 || oz < 0) continue;
+        const uint16_t* w2 = w2b + (size_t)(r == 0 ? ox : (r == 1 ? oy : oz)) * kW2;
         const size_t gi = gidx(a, B.o0[0] + ox, B.o0[1] + oy, B.o0[2] + oz);
         float yb[kOut];
 #pragma unroll
@@ -315,9 +284,11 @@ __global__ __launch_bounds__(256) void shell_wgrad1(ShellArgs<T> a) {
     }
 }
 
+// (layer 2: all kDepth heads, each summing only the cells at its depth)
 template <class T>
 __global__ __launch_bounds__(256) void shell_wgrad2(ShellArgs<T> a) {
     __shared__ float sy[kChunk2][kOut];
+    __shared__ int sd[kChunk2];
     __shared__ float shc[kChunk2][kHid * kTaps];
     const ShellBox& B = a.box[a.part_box];
     const size_t N = (size_t)a.sx * a.sy * a.sz;
@@ -325,8 +296,9 @@ __global__ __launch_bounds__(256) void shell_wgrad2(ShellArgs<T> a) {
     const int nout = (B.o1[0] - B.o0[0]) * ny * nz;
     const int per = (nout + gridDim.x - 1) / gridDim.x;
     const int c0 = blockIdx.x * per, c1 = min(c0 + per, nout);
-    constexpr int NW = kOut * kHid * kTaps;  // 1296
+    constexpr int NW = kDepth * kOut * kHid * kTaps;  // 4 x 1296
     constexpr int PER = (NW + 255) / 256;
+    const int r = a.part_box / 2;
     float acc[PER];
 #pragma unroll
     for (int m = 0; m < PER; ++m) acc[m] = 0.f;
@@ -335,12 +307,15 @@ __global__ __launch_bounds__(256) void shell_wgrad2(ShellArgs<T> a) {
         for (int e = threadIdx.x; e < kChunk2 * kOut; e += 256) {
             const int l = e / kOut, o = e % kOut;
             float v = 0.f;
+            int dep = -1;
             if (l < n) {
                 const int cl = base + l;
                 const int lz = cl % nz, ly = (cl / nz) % ny, lx = cl / (nz * ny);
                 v = (float)a.Ebar[o * N + gidx(a, B.o0[0] + lx, B.o0[1] + ly, B.o0[2] + lz)];
+                dep = r == 0 ? lx : (r == 1 ? ly : lz);
             }
             sy[l][o] = v;
+            if (o == 0) sd[l] = dep;
         }
         for (int e = threadIdx.x; e < kChunk2 * kHid * kTaps; e += 256) {
             const int l = e / (kHid * kTaps), kt = e % (kHid * kTaps), k = kt / kTaps, t = kt % kTaps;
@@ -360,9 +335,10 @@ __global__ __launch_bounds__(256) void shell_wgrad2(ShellArgs<T> a) {
         for (int m = 0; m < PER; ++m) {
             const int wi = threadIdx.x + 256 * m;
             if (wi < NW) {
-                const int o = wi / (kHid * kTaps), kt = wi % (kHid * kTaps);
+                const int dep = wi / (kOut * kHid * kTaps), w = wi % (kOut * kHid * kTaps);
+                const int o = w / (kHid * kTaps), kt = w % (kHid * kTaps);
                 float s_ = 0.f;
-                for (int l = 0; l < kChunk2; ++l) s_ += sy[l][o] * shc[l][kt];
+                for (int l = 0; l < kChunk2; ++l) s_ += sd[l] == dep ? sy[l][o] * shc[l][kt] : 0.f;
                 acc[m] += s_;
             }
         }
@@ -425,6 +401,7 @@ constexpr int kHidRows = (kMaxT + 2) * (kTileA1 + 2);     // hidden a2 rows (of 
 constexpr int kHidPlane = kHidRows * kRow;                // cells per hidden plane
 constexpr int kTileLds = kInMax > 2 * kHidPlane ? kInMax : 2 * kHidPlane;   // uint4 words
 static_assert(kTileA2 + 2 == kRow, "hidden rows must be one MFMA wide");
+static_assert(kDepth >= kMaxT, "one layer-2 head per shell depth");
 
 struct ShellTiles {
     int perm[6][3];    // grid axis of tile axes a0, a1, a2
@@ -451,10 +428,11 @@ __host__ __device__ __forceinline__ int tap_of(const int* p, int d0, int d1, int
     return (d[0] * 3 + d[1]) * 3 + d[2];
 }
 
-// A fragments in lane order, per box: [kFrag1 + kFrag2][64 lanes][8] BF16,
-// so a lane loads each fragment with one 16-byte read (layouts above).
-constexpr int kFrag1 = 9, kFrag2 = 18;
-constexpr int kPackBox = (kFrag1 + kFrag2) * 64 * 8;      // BF16 values per box
+// A fragments in lane order, per box: [kFrag1 + kDepth * kFrag2][64
+// lanes][8] BF16 (layer 1, then layer 2 head by head), so a lane loads each
+// fragment with one 16-byte read (layouts above).
+constexpr int kFrag1 = 9, kFrag2 = 18;                    // per box; kFrag2 per layer-2 head
+constexpr int kPackBox = (kFrag1 + kDepth * kFrag2) * 64 * 8;   // BF16 values per box
 constexpr int kPackFloats = 6 * kPackBox / 2;             // FP32 words of scratch for 6 boxes
 
 __global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, ShellTiles tl, uint16_t* packed) {
@@ -468,10 +446,11 @@ __global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, S
     if (f < kFrag1) {
         if (g < 3 && i < kIn) v = w[(size_t)b * kW1 + (n * kIn + i) * kTaps + tap_of(p, f / 3, f % 3, g)];
     } else {
-        const int pair = (f - kFrag1) >> 1, h = (f - kFrag1) & 1;
+        const int dep = (f - kFrag1) / kFrag2, f2 = (f - kFrag1) % kFrag2;
+        const int pair = f2 >> 1, h = f2 & 1;
         const int d2 = h == 0 ? (g >> 1) : 2;
         if (n < kOut && (h == 0 || g < 2))
-            v = w[(size_t)nbox * kW1 + (size_t)b * kW2 + (n * kHid + 8 * (g & 1) + i) * kTaps +
+            v = w[(size_t)nbox * kW1 + ((size_t)b * kDepth + dep) * kW2 + (n * kHid + 8 * (g & 1) + i) * kTaps +
                   tap_of(p, pair / 3, pair % 3, d2)];
     }
     packed[idx] = v;
@@ -555,7 +534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = lane >> 4, n = lane & 15;
-    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kFrag2) * 64 + lane;
+    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kDepth * kFrag2) * 64 + lane;
     __syncthreads();
     // 2. layer 1: wave w takes a0 rows h0 = w/2, w/2 + 2, w/2 + 4 (< T+2) and
     // along a1 the five rows h1 = 5 (w % 2) + [0, 5); B fragment (h0 + d0,
@@ -612,7 +591,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         }
     }
     __syncthreads();
-    // 3. layer 2: wave w takes output a0 row o0 = w (< T) and its 8 rows
+    // 3. layer 2: wave w takes output a0 row o0 = w (< T), i.e. depth o0 of
+    // the box (a tile is the box's whole depth) and its head, and its 8 rows
     // along a1; B fragment (o0 + d0, y, half) feeds rows y - d1.  Lanes 14,
     // 15 of each MFMA read a valid cell and their columns are never used.
     const int o0 = wave;
@@ -620,7 +600,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         bf16x8 W2[kFrag2];                     // rows n >= 3 are zero: only 3 lanes of 16 load
 #pragma unroll
         for (int f = 0; f < kFrag2; ++f) {
-            const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + f) * 64] : make_uint4(0u, 0u, 0u, 0u);
+            const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + o0 * kFrag2 + f) * 64]
+                                     : make_uint4(0u, 0u, 0u, 0u);
             W2[f] = __builtin_bit_cast(bf16x8, v);
         }
         constexpr int kR2 = kTileA1;
@@ -829,7 +810,7 @@ int64_t fdtd_shell_hidden_elems(const void* plan, int32_t thickness) {
     return (int64_t)make_shell(P, thickness).scratch_floats + kPackFloats;
 }
 
-int64_t fdtd_shell_weight_elems(void) { return 6 * (int64_t)(kW1 + kW2); }
+int64_t fdtd_shell_weight_elems(void) { return 6 * (int64_t)(kW1 + kDepth * kW2); }
 
 int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, void* E, const void* H,
                      float* hidden, void* stream) {
@@ -885,7 +866,7 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     return check_launch("fdtd_shell_step_n");
 }
 
-int64_t fdtd_shell_partials_elems(void) { return (int64_t)64 * kW1; }
+int64_t fdtd_shell_partials_elems(void) { return (int64_t)64 * std::max(kW1, kDepth * kW2); }
 
 int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
                         float* hidden, float* hbar, const void* Ebar, void* Ebar_out, void* Hbar_out,
@@ -923,8 +904,9 @@ int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, 
                                groups, kW1, wgrad + (size_t)b * kW1);
             // W2 gradient of box b
             hipLaunchKernelGGL(shell_wgrad2<T>, dim3(groups), dim3(256), 0, st, a);
-            hipLaunchKernelGGL(shell_wgrad_sum, dim3((kW2 + 255) / 256), dim3(256), 0, st, (const float*)partials,
-                               groups, kW2, wgrad + (size_t)S.nbox * kW1 + (size_t)b * kW2);
+            hipLaunchKernelGGL(shell_wgrad_sum, dim3((kDepth * kW2 + 255) / 256), dim3(256), 0, st,
+                               (const float*)partials, groups, kDepth * kW2,
+                               wgrad + (size_t)S.nbox * kW1 + (size_t)b * kDepth * kW2);
         }
         return 0;
     };

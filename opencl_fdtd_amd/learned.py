@@ -119,9 +119,10 @@ def reflection_db(test: torch.Tensor, wide: torch.Tensor) -> float:
 from . import _lib  # noqa: E402
 
 N_HID, N_IN, N_OUT, N_TAP = 16, 6, 3, 27
+N_DEPTH = 4                        # layer-2 heads per box: one per depth of the shell (T <= 4)
 W1_SIZE = N_HID * N_IN * N_TAP
-W2_SIZE = N_OUT * N_HID * N_TAP
-N_WEIGHTS = 6 * (W1_SIZE + W2_SIZE)
+W2_SIZE = N_OUT * N_HID * N_TAP    # one head
+N_WEIGHTS = 6 * (W1_SIZE + N_DEPTH * W2_SIZE)
 
 
 # The CPML the shell runs under the correction: a 4-cell CPML tuned for the
@@ -168,6 +169,8 @@ class ShellNet:
     def load(cls, path: str, device="cuda") -> "ShellNet":
         with numpy.load(path, allow_pickle=False) as z:
             base = {str(k): float(v) for k, v in zip(z["base_pml_keys"], z["base_pml_values"])}
+            if z["weights"].size != N_WEIGHTS:
+                raise ValueError("%s holds %d weights, this net has %d" % (path, z["weights"].size, N_WEIGHTS))
             return cls(int(z["thickness"]), weights=torch.as_tensor(z["weights"]), device=device, base_pml=base)
 
 
@@ -281,6 +284,7 @@ class ShellTrainer:
         self.m = torch.zeros_like(net.master)
         self.v = torch.zeros_like(net.master)
         self.t = 0
+        self.next_case = 0                     # scenarios are taken in turn across train() calls
         self.cases = []
         for sc in scenarios:
             wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=1)
@@ -368,14 +372,13 @@ class ShellTrainer:
         ``batch`` scenarios taken in turn; returns the mean losses."""
         hist = []
         lr0 = self.lr
-        k = 0
         for it in range(iterations):
             if decay_every:
                 self.lr = lr0 * decay ** (it // decay_every)
             Ls, gs = 0.0, None
             for _ in range(batch):
-                L, g = self.loss_and_grad(self.cases[k % len(self.cases)])
-                k += 1
+                L, g = self.loss_and_grad(self.cases[self.next_case % len(self.cases)])
+                self.next_case += 1
                 Ls += L / batch
                 gs = g / batch if gs is None else gs.add_(g, alpha=1.0 / batch)
             self.adam(gs)
@@ -404,14 +407,51 @@ def train_scenarios(n: int, seed0: int = 100, interior=(64, 64, 48), steps: int 
             for s in range(n)]
 
 
-def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4) -> ShellNet:
+def validation_scenarios(n: int = 4, seed0: int = 500, interior=(128, 128, 96), steps: int = 400):
+    """Held-out scenarios for model selection, with their padded references."""
+    scs = [make_scenario(seed0 + s, interior=interior, steps=steps) for s in range(n)]
+    return [(sc, run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=5)) for sc in scs]
+
+
+def validation_db(net: ShellNet, val) -> float:
+    """Mean reflection (dB vs the padded runs) of the net on ``val``."""
+    return float(numpy.mean([reflection_db(run_learned(sc, net, sample_every=5), wide) for sc, wide in val]))
+
+
+def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4,
+               val_every: int = 8, interior=(64, 64, 48), steps: int = 300) -> ShellNet:
+    """Train a 4-cell shell net from W2 = 0 (the base CPML-4); every
+    ``val_every`` Adam steps measure the held-out reflection and keep the
+    best weights (the base itself when nothing beats it)."""
     net = ShellNet(4, seed=1)
-    tr = ShellTrainer(net, train_scenarios(n_scen), checkpoint_every=60, lr=lr)
+    tr = ShellTrainer(net, train_scenarios(n_scen, interior=interior, steps=steps), checkpoint_every=60, lr=lr)
+    val = validation_scenarios()
+    best_db = validation_db(net, val)
+    best = net.master.clone()
+    log("validation: base CPML-4 %.2f dB" % best_db)
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+
+    def save_best():
+        keep = net.master.clone()
+        net.master.copy_(best)
+        net.save(out)
+        net.master.copy_(keep)
     t0 = time.time()
-    tr.train(iterations, decay_every=max(iterations // 3, 1), batch=batch,
-             log=lambda it, L: log("it %d loss %.2f dB lr %.1e (%.0f s)" % (it, 10 * numpy.log10(L), tr.lr,
-                                                                            time.time() - t0)))
-    os.makedirs(os.path.dirname(out), exist_ok=True)
+    decay_every = max(iterations // 3, 1)
+    for it in range(0, iterations, val_every):
+        n = min(val_every, iterations - it)
+        lr_it = lr * 0.5 ** (it // decay_every)
+        tr.lr = lr_it
+        tr.train(n, batch=batch,
+                 log=lambda k, L: log("it %d loss %.2f dB lr %.1e (%.0f s)" % (it + k, 10 * numpy.log10(L), tr.lr,
+                                                                              time.time() - t0)))
+        db = validation_db(net, val)
+        if db < best_db:
+            best_db, best = db, net.master.clone()
+            save_best()                        # the best so far survives an interrupted run
+        log("validation after %d: %.2f dB (best %.2f)" % (it + n, db, best_db))
+    net.master.copy_(best)
+    net.sync()
     net.save(out)
     return net
 
@@ -440,16 +480,23 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train / evaluate the learned 4-cell absorbing shell")
     ap.add_argument("--train", type=int, default=0, help="training iterations")
-    ap.add_argument("--lr", type=float, default=5e-5)
-    ap.add_argument("--scenarios", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=5e-6)
+    ap.add_argument("--scenarios", type=int, default=48)
     ap.add_argument("--batch", type=int, default=4, help="scenarios per Adam step")
+    ap.add_argument("--interior", default="64,64,48", help="training interior (cells)")
+    ap.add_argument("--steps", type=int, default=300, help="steps per training scenario")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
     a = ap.parse_args(argv)
-    net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch) if a.train
-           else ShellNet.load(a.weights))
+    interior = tuple(int(v) for v in a.interior.split(","))
+    net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps)
+           if a.train else ShellNet.load(a.weights))
     if a.evaluate:
-        evaluate_main(net, range(1000, 1000 + a.evaluate))
+        rows = evaluate_main(net, range(1000, 1000 + a.evaluate))
+        gain = [r["cpml4_base"] - r["learned4"] for r in rows]
+        print(json.dumps({"scenarios": len(rows), "mean_gain_db": float(numpy.mean(gain)),
+                          "improved": int(sum(g > 0 for g in gain)),
+                          "worst_learned_db": max(r["learned4"] for r in rows)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -2,10 +2,12 @@
 
 Restates the shell correction (no counterpart in the reference): on a shell
 of T cells per face, split into six boxes (x faces first, then y, then z),
-  h_k(j) = sum_{c,t} W1[b][k][c][t] x_c(j + t)      x = (E, H), 16 hidden
-  y_o(i) = sum_{k,t} W2[b][o][k][t] h_k(i + t)      3 outputs, i in box b
+  h_k(j) = sum_{c,t} W1[b][k][c][t] x_c(j + t)        x = (E, H), 16 hidden
+  y_o(i) = sum_{k,t} W2[b][d(i)][o][k][t] h_k(i + t)  3 outputs, i in box b
   E_out = E + y on the shell cells,
-with t over the 27 offsets of a periodic 3x3x3 stencil and weights in BF16.
+with t over the 27 offsets of a periodic 3x3x3 stencil, weights in BF16, and
+d(i) the depth of i: its index along the box's thin axis (0..T-1; four
+output heads are stored whatever T).
 
 ``forward(..., exact_f32=True)`` reproduces the GPU arithmetic: inputs and
 hidden values rounded to BF16, products and sums in float32, in the GPU's
@@ -18,8 +20,10 @@ from __future__ import annotations
 import numpy
 
 N_HID, N_IN, N_OUT, N_TAP = 16, 6, 3, 27
+N_DEPTH = 4                                    # output heads per box (T <= 4)
 W1_SIZE = N_HID * N_IN * N_TAP
-W2_SIZE = N_OUT * N_HID * N_TAP
+W2_SIZE = N_OUT * N_HID * N_TAP                # one head
+N_WEIGHTS = 6 * (W1_SIZE + N_DEPTH * W2_SIZE)
 
 
 def offsets():
@@ -60,11 +64,20 @@ def bf16_round(a):
 
 
 def split_weights(w, nbox=6):
-    """Flat weight vector (float) -> W1 [box][16][6][27], W2 [box][3][16][27]."""
+    """Flat weight vector (float) -> W1 [box][16][6][27], W2 [box][depth][3][16][27]."""
     w = numpy.asarray(w)
     W1 = w[:nbox * W1_SIZE].reshape(nbox, N_HID, N_IN, N_TAP)
-    W2 = w[nbox * W1_SIZE:].reshape(nbox, N_OUT, N_HID, N_TAP)
+    W2 = w[nbox * W1_SIZE:].reshape(nbox, N_DEPTH, N_OUT, N_HID, N_TAP)
     return W1, W2
+
+
+def depth_masks(shape, box, b):
+    """Masks of the box's output cells at each depth along its thin axis."""
+    r = b // 2
+    (lo, hi) = box
+    idx = numpy.arange(shape[r]).reshape([-1 if q == r else 1 for q in range(3)])
+    m = box_mask(shape, box)
+    return [m & ((idx - lo[r]) == d) for d in range(N_DEPTH)]
 
 
 def _shift(a, d):
@@ -94,13 +107,15 @@ def forward(E, H, w, T, exact_f32=True):
         if exact_f32:
             h = bf16_round(h)
         hs = [_shift(h, d) for d in offsets()]
-        yb = numpy.zeros((3,) + shape, dt)
-        for t in range(N_TAP):
-            for k in range(N_HID):
-                for o in range(N_OUT):
-                    yb[o] = yb[o] + dt(W2[b, o, k, t]) * hs[t][k]
-        m = box_mask(shape, box)
-        y[:, m] = yb[:, m]
+        for dep, m in enumerate(depth_masks(shape, box, b)):
+            if not m.any():
+                continue
+            yb = numpy.zeros((3,) + shape, dt)
+            for t in range(N_TAP):
+                for k in range(N_HID):
+                    for o in range(N_OUT):
+                        yb[o] = yb[o] + dt(W2[b, dep, o, k, t]) * hs[t][k]
+            y[:, m] = yb[:, m]
     Eout = (E.astype(dt) + y).astype(dt)
     return Eout, y
 
@@ -118,17 +133,17 @@ def backward(E, H, w, T, Ebar):
     n = int(numpy.prod(shape))
     X = numpy.array(xs).reshape(N_TAP, N_IN, n)                  # x_c(j + t)
     for b, box in enumerate(boxes(shape, T)):
-        m = box_mask(shape, box)
         h = numpy.einsum("kct,tcn->kn", W1[b], X).reshape((N_HID,) + shape)
         Hs = numpy.array([_shift(h, d) for d in offsets()]).reshape(N_TAP, N_HID, n)
-        yb = numpy.where(m, Ebar, 0.0)
-        # W2bar[o][k][t] = sum_i ybar_o(i) h_k(i + t)
-        W2b[b] = numpy.einsum("on,tkn->okt", yb.reshape(N_OUT, n), Hs)
-        # hbar_k(j) = sum_{o,t} W2[o][k][t] ybar_o(j - t)
         hbar = numpy.zeros_like(h)
-        for t, d in enumerate(offsets()):
-            hbar += numpy.einsum("ok,on->kn", W2[b][:, :, t],
-                                 _shift(yb, (-d[0], -d[1], -d[2])).reshape(N_OUT, n)).reshape(h.shape)
+        for dep, m in enumerate(depth_masks(shape, box, b)):
+            yb = numpy.where(m, Ebar, 0.0)
+            # W2bar[d][o][k][t] = sum_{i at depth d} ybar_o(i) h_k(i + t)
+            W2b[b, dep] = numpy.einsum("on,tkn->okt", yb.reshape(N_OUT, n), Hs)
+            # hbar_k(j) = sum_{o,t} W2[d(j - t)][o][k][t] ybar_o(j - t)
+            for t, d in enumerate(offsets()):
+                hbar += numpy.einsum("ok,on->kn", W2[b, dep][:, :, t],
+                                     _shift(yb, (-d[0], -d[1], -d[2])).reshape(N_OUT, n)).reshape(h.shape)
         # W1bar[k][c][t] = sum_j hbar_k(j) x_c(j + t)
         W1b[b] = numpy.einsum("kn,tcn->kct", hbar.reshape(N_HID, n), X)
         for t, d in enumerate(offsets()):

@@ -10,7 +10,7 @@ from oracle import shell_net as SN
 
 
 def _rand_weights(rng, scale=0.05):
-    return SN.bf16_round(rng.standard_normal(6 * (SN.W1_SIZE + SN.W2_SIZE)) * scale).astype(numpy.float64)
+    return SN.bf16_round(rng.standard_normal(SN.N_WEIGHTS) * scale).astype(numpy.float64)
 
 
 def _fields(rng, shape):
@@ -56,18 +56,46 @@ def test_oracle_weight_gradient_finite_differences():
         fd = (f(w + h * d) - f(w - h * d)) / (2 * h)
         assert abs(fd - float(wbar @ d)) <= 1e-8 * max(abs(fd), 1)
     # a few single weights in W1 and W2
-    for idx in (5, SN.W1_SIZE * 3 + 77, 6 * SN.W1_SIZE + 10, 6 * SN.W1_SIZE + 5 * SN.W2_SIZE + 400):
+    for idx in (5, SN.W1_SIZE * 3 + 77, 6 * SN.W1_SIZE + 10, 6 * SN.W1_SIZE + 21 * SN.W2_SIZE + 400,
+                6 * SN.W1_SIZE + 9 * SN.W2_SIZE + 1000):
         d = numpy.zeros(w.size)
         d[idx] = 1.0
         fd = (f(w + 1e-3 * d) - f(w - 1e-3 * d)) / 2e-3
         assert abs(fd - wbar[idx]) <= 1e-8 * max(abs(fd), 1)
 
 
+def test_oracle_depth_heads():
+    """Layer-2 head d acts only on the cells at depth d of their box (index
+    d along the box's thin axis, counted from the box's low side)."""
+    rng = numpy.random.default_rng(406)
+    shape, T = (11, 10, 12), 3
+    E, H = _fields(rng, shape)
+    w = _rand_weights(rng)
+    n1 = 6 * SN.W1_SIZE
+    for b in (0, 3, 5):
+        for d in range(T):
+            ww = w.copy()
+            ww[n1:] = 0.0
+            head = n1 + (b * SN.N_DEPTH + d) * SN.W2_SIZE
+            ww[head:head + SN.W2_SIZE] = w[head:head + SN.W2_SIZE]
+            _, y = SN.forward(E, H, ww, T, exact_f32=False)
+            (lo, hi) = SN.boxes(shape, T)[b]
+            want = SN.box_mask(shape, (lo, hi))
+            r = b // 2
+            sel = [slice(None)] * 3
+            keep = numpy.zeros(shape, bool)
+            sel[r] = lo[r] + d
+            keep[tuple(sel)] = True
+            want &= keep
+            support = numpy.abs(y).sum(axis=0) > 0
+            assert numpy.array_equal(support, want), (b, d)
+
+
 def test_oracle_zero_weights_identity():
     rng = numpy.random.default_rng(403)
     shape = (8, 8, 8)
     E, H = _fields(rng, shape)
-    Eo, y = SN.forward(E, H, numpy.zeros(6 * (SN.W1_SIZE + SN.W2_SIZE)), 2)
+    Eo, y = SN.forward(E, H, numpy.zeros(SN.N_WEIGHTS), 2)
     assert numpy.array_equal(Eo, E.astype(numpy.float32)) and not y.any()
 
 
@@ -216,3 +244,22 @@ def test_trainer_gradient_directional_derivative():
     net.sync()
     tr.K = 7
     assert torch.equal(tr.loss_and_grad(tr.cases[0])[1], g)
+
+
+@pytest.mark.gpu
+def test_zero_net_is_the_base_cpml():
+    """With zero weights the shell step is exactly its base CPML-4 (the
+    fallback): fields equal to the plain solver's after many steps."""
+    import torch
+    from opencl_fdtd_amd import Simulation, learned as L
+    sc = L.make_scenario(9, interior=(30, 26, 34), steps=60, near_face=6, src_margin=4)
+    net = L.ShellNet(4, weights=torch.zeros(L.N_WEIGHTS))
+    shape, eps, idx, sl = L.embed(sc, 4)
+    wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device="cuda")
+    a = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=False)
+    b = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=False)
+    L.ShellRunner(a, net).step_n(sc.steps, idx, wave)
+    b.step_n(sc.steps, idx, wave)
+    torch.cuda.synchronize()
+    assert float(torch.abs(b.E).max()) > 0
+    assert torch.equal(a.E, b.E) and torch.equal(a.H, b.H)
