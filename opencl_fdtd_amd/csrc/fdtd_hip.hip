@@ -70,8 +70,11 @@ namespace {
 #ifndef FDTD_E_WAVES
 #define FDTD_E_WAVES 1               // __launch_bounds__ min waves per SIMD, E kernel
 #endif
+#ifndef FDTD_PREFETCH_H
+#define FDTD_PREFETCH_H FDTD_PREFETCH   // same for the H kernel
+#endif
 #ifndef FDTD_H_TARGET
-#define FDTD_H_TARGET 0              // fused-S H workgroups per launch (0: as many threads as the E launch)
+#define FDTD_H_TARGET 2048           // fused-S H workgroups per launch (measured: 2048 beats 1024-8192)
 #endif
 #ifndef FDTD_H_WAVES
 #define FDTD_H_WAVES 6               // same, H kernel
@@ -664,11 +667,11 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
     int x = xs;
     for (int it = 0; it < n_it; ++it) {
         const int xn = (x == g.sx - 1) ? 0 : x + 1;
-        if (!FDTD_PREFETCH && it > 0) {
+        if (!FDTD_PREFETCH_H && it > 0) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, x, pbytes, cur[k]);
         }
-        if (FDTD_PREFETCH && it + 1 < n_it) {
+        if (FDTD_PREFETCH_H && it + 1 < n_it) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, xn, pbytes, nxt[k]);
         }
@@ -738,7 +741,7 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
             qx[k] = cur[k].exn;
             qy[k] = cur[k].eyn;
             qz[k] = cur[k].ezn;
-            if (FDTD_PREFETCH) cur[k] = nxt[k];
+            if (FDTD_PREFETCH_H) cur[k] = nxt[k];
         }
         x = xn;
     }

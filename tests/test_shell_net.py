@@ -263,3 +263,48 @@ def test_zero_net_is_the_base_cpml():
     torch.cuda.synchronize()
     assert float(torch.abs(b.E).max()) > 0
     assert torch.equal(a.E, b.E) and torch.equal(a.H, b.H)
+
+
+@pytest.mark.gpu
+def test_trained_shell_gain_over_its_base():
+    """The shipped net against the method without it (the same CPML-4, zero
+    weights) on unseen 256x256x128 scenarios, 600 steps: reflection (interior
+    field error over the run, vs the scenario padded by 48 cells of CPML)
+    improves in every scenario, by at least 1 dB on average, and stays at or
+    below -40 dB."""
+    import torch
+    from opencl_fdtd_amd import learned as L
+    net = L.ShellNet.load(L.WEIGHTS)
+    base = L.ShellNet(net.T, weights=torch.zeros(L.N_WEIGHTS), base_pml=net.base_pml)
+    gains = []
+    for seed in (1000, 1001, 1002, 1003):
+        sc = L.make_scenario(seed, interior=(256, 256, 128), steps=600)
+        wide = L.run_interior(sc, L.WIDE_PAD, L.cpml_faces(L.WIDE_PAD), sample_every=5)
+        db_net = L.reflection_db(L.run_learned(sc, net, sample_every=5), wide)
+        db_base = L.reflection_db(L.run_learned(sc, base, sample_every=5), wide)
+        print("seed %d: base %.2f dB, learned %.2f dB" % (seed, db_base, db_net))
+        assert db_net <= -40.0
+        gains.append(db_base - db_net)
+    assert min(gains) > 0.0
+    assert float(numpy.mean(gains)) >= 1.0
+
+
+@pytest.mark.gpu
+def test_trained_shell_long_run_is_stable():
+    """4000 leapfrog steps after a pulse: the interior energy with the
+    trained shell decays like that of its base CPML-4 (no late growth)."""
+    import torch
+    from opencl_fdtd_amd import learned as L
+    net = L.ShellNet.load(L.WEIGHTS)
+    base = L.ShellNet(net.T, weights=torch.zeros(L.N_WEIGHTS), base_pml=net.base_pml)
+    sc = L.make_scenario(21, interior=(64, 64, 48), steps=4000)
+    energy = {}
+    for name, nn in (("learned", net), ("base", base)):
+        e = [float(torch.sum(t.double() ** 2)) for t in L.run_learned(sc, nn, sample_every=250)]
+        energy[name] = e
+        print(name, ["%.3e" % v for v in e])
+    e = energy["learned"]
+    peak = max(e)
+    assert e[-1] < 1e-3 * peak
+    assert e[-1] <= e[-5]                          # still decaying over the last 1000 steps
+    assert e[-1] <= 2.0 * energy["base"][-1]
