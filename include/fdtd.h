@@ -25,6 +25,9 @@
  *                         (fdtd.py:168-169), with f(t) read from device memory
  *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
  *                         (E, F, source, H, G, S) without host round trips
+ *   fdtd_update_he     <- sim.update_H + sim.update_S of one iteration of that
+ *                         loop followed by sim.update_E and the soft source of
+ *                         the next, in one pass over the grid
  *   fdtd_adjoint_step, fdtd_plane_sum, fdtd_energy, fdtd_reduce_partials
  *                      <- no reference counterpart (gradients and FP64
  *                         reductions over the same step; see below)
@@ -51,7 +54,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 1
+#define FDTD_ABI_VERSION 2
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -113,6 +116,9 @@ typedef struct {
     void* coef;       /* fdtd_coef_elems() values: per face p0e,p1e,p2e,p0h,p1h,p2h (T each) */
     void* inv_d;      /* nonuniform only: 1/dx_e along x,y,z then 1/dx_h along x,y,z,
                          lengths sx,sy,sz,sx,sy,sz (simulation.py:136-142, 179-182) */
+    void* E_out;      /* optional second E buffer (3N, != E): written by fdtd_update_he;
+                         lets fdtd_step_n fuse each H half-step with the next E half-step.
+                         NULL: no fusion */
 } fdtd_fields;
 
 int fdtd_abi_version(void);
@@ -145,6 +151,21 @@ int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp
  * roles every step: after an odd n the newest H is in f->H_out. */
 int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
                 const void* src_wave, void* stream);
+
+/* The H half-step with S followed by the next E half-step in one pass:
+ * H_out = H-step(E, H); S; E_out = E-step(E, H_out); then, when src_amp is
+ * not NULL, E_out[src_index] += src_amp[0].  E is left unchanged (E_out holds
+ * the new E), psi_e is updated in place, psi_h -> psi_h_out.  Bit-identical to
+ * fdtd_update_h, then fdtd_update_e on E_out := E, then fdtd_add_point.
+ * Needs fdtd_update_he_supported(plan) (fused Poynting, no Bloch boundaries,
+ * rows of at most 341 cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.
+ * With f->E_out set and a supported plan, fdtd_step_n uses it for all but
+ * the first E and last H half-steps and arranges the count so that the new E
+ * ends in f->E. */
+int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,
+                   void* stream);
+/* 1 when fdtd_update_he can run with this plan, else 0. */
+int fdtd_update_he_supported(const void* plan);
 
 /* ------------------------------------------------------------------------
  * Discrete adjoint of the step and FP64 reductions.  No counterpart in the

@@ -79,6 +79,9 @@ namespace {
 #ifndef FDTD_H_WAVES
 #define FDTD_H_WAVES 6               // same, H kernel
 #endif
+#ifndef FDTD_HE_WAVES
+#define FDTD_HE_WAVES 4              // same, fused H+E kernel
+#endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
 constexpr int kTargetBlocks = FDTD_TARGET_BLOCKS;
@@ -213,6 +216,17 @@ __device__ __forceinline__ void block_coords(int pblocks, int nchunk, int* pb, i
     *ck = lb / pblocks;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences
+// global memory, which makes every wave wait for all of its outstanding
+// global stores (vmcnt(0)) at each barrier; the marching kernels exchange
+// data through LDS only and never read global memory another thread of the
+// launch writes, so the LDS-scope fence is all they need.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <class T>
 __device__ __forceinline__ bool slab(const FaceDev<T>& f, int c, int s, int* ir) {
     if (c < f.lo || c >= f.hi) return false;
@@ -231,17 +245,29 @@ __device__ __forceinline__ uint32_t psi_index(const Geom<T>& g, const FaceDev<T>
 }
 
 // CPML of one face on the derivatives of one cell; update_e.cl:64-100 (ISE)
-// and update_h.cl:80-116.  d_ab = d(field_a)/d(b).
-template <int R, class T, bool ISE>
-__device__ __forceinline__ void apply_face(const Geom<T>& g, const FaceDev<T>& f, int ir, uint32_t ip,
-                                           const T* psi_in, T* psi_out, bool write,
-                                           T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy,
-                                           T& px, T& py, T& pz) {
+// and update_h.cl:80-116.  d_ab = d(field_a)/d(b).  face_step takes the
+// face's coefficients at depth ir and the old psi pair (ou, ov) and stores
+// the new pair at byte offsets bu, bv when write is set.
+template <class T>
+struct FaceCoef {
+    T p0, p1, p2;
+};
+
+template <class T, bool ISE>
+__device__ __forceinline__ FaceCoef<T> face_coef(const Geom<T>& g, const FaceDev<T>& f, int ir) {
     const T* c = g.coef + f.coef_off + (ISE ? 0 : 3 * f.thick);
-    const T p0 = c[ir], p1 = c[f.thick + ir], p2 = c[2 * f.thick + ir];
-    const uint32_t bu = ((uint32_t)f.psi_off + ip) * (uint32_t)sizeof(T);
-    const uint32_t bv = bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T);
-    const T ou = ld(psi_in, bu), ov = ld(psi_in, bv);
+    FaceCoef<T> k;
+    k.p0 = c[ir];
+    k.p1 = c[f.thick + ir];
+    k.p2 = c[2 * f.thick + ir];
+    return k;
+}
+
+template <int R, class T, bool ISE>
+__device__ __forceinline__ void face_step(const FaceCoef<T>& k, T ou, T ov, T* psi_out, uint32_t bu,
+                                          uint32_t bv, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
+                                          T& dzy, T& px, T& py, T& pz) {
+    const T p0 = k.p0, p1 = k.p1, p2 = k.p2;
     T su, sv;
     if (R == 0) {                      // r = x, (u, v) = (y, z)
         dzx *= p2; dyx *= p2;
@@ -263,6 +289,18 @@ __device__ __forceinline__ void apply_face(const Geom<T>& g, const FaceDev<T>& f
         st(psi_out, bu, su);
         st(psi_out, bv, sv);
     }
+}
+
+template <int R, class T, bool ISE>
+__device__ __forceinline__ void apply_face(const Geom<T>& g, const FaceDev<T>& f, int ir, uint32_t ip,
+                                           const T* psi_in, T* psi_out, bool write,
+                                           T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy,
+                                           T& px, T& py, T& pz) {
+    const FaceCoef<T> k = face_coef<T, ISE>(g, f, ir);
+    const uint32_t bu = ((uint32_t)f.psi_off + ip) * (uint32_t)sizeof(T);
+    const uint32_t bv = bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T);
+    const T ou = ld(psi_in, bu), ov = ld(psi_in, bv);
+    face_step<R, T, ISE>(k, ou, ov, psi_out, bu, bv, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
 }
 
 template <class T>
@@ -333,6 +371,129 @@ __device__ __forceinline__ Cell make_cell(const Geom<T>& g, int p) {
 }
 
 // ---------------------------------------------------------------------------
+// CPML with the psi pairs loaded early.  The marching kernels issue the loads
+// of a cell's psi values together with its field loads at the top of a plane
+// iteration (one memory round trip per plane instead of one more per slab),
+// then apply the faces with cpml_apply.  A PsiHit holds, for one axis, the
+// first face slot whose slab contains the cell (-1: none), the depth ir into
+// it, and whether the other slot's slab contains it too (slabs of one axis
+// overlap only on grids thinner than both; that second face is then applied
+// with its own loads).  Same faces, same order, same arithmetic as cpml_cell.
+// ---------------------------------------------------------------------------
+struct PsiHit {
+    int code;            // -1: none; else ir << 2 | both << 1 | slot
+    __device__ __forceinline__ bool on() const { return code >= 0; }
+    __device__ __forceinline__ int slot() const { return code & 1; }
+    __device__ __forceinline__ bool both() const { return (code & 2) != 0; }
+    __device__ __forceinline__ int ir() const { return code >> 2; }
+};
+
+template <class T>
+__device__ __forceinline__ PsiHit slab_hit(const Geom<T>& g, int r, int c, int s) {
+    int ir0 = 0, ir1 = 0;
+    const bool h0 = g.face[r][0].thick && slab(g.face[r][0], c, s, &ir0);
+    const bool h1 = g.face[r][1].thick && slab(g.face[r][1], c, s, &ir1);
+    PsiHit h;
+    h.code = h0 ? (ir0 << 2) | (h1 ? 2 : 0) : h1 ? (ir1 << 2) | 1 : -1;
+    return h;
+}
+
+// field by field, so a per-lane slot selects between uniform values instead
+// of turning into per-lane loads of the argument block
+template <class T>
+__device__ __forceinline__ FaceDev<T> face_of(const Geom<T>& g, int r, int slot) {
+    const FaceDev<T>& a = g.face[r][0];
+    const FaceDev<T>& b = g.face[r][1];
+    const bool s0 = slot == 0;
+    const auto pick = [s0](int u, int v) {
+        return s0 ? __builtin_amdgcn_readfirstlane(u) : __builtin_amdgcn_readfirstlane(v);
+    };
+    FaceDev<T> f;
+    f.pol = pick(a.pol, b.pol);
+    f.thick = pick(a.thick, b.thick);
+    f.lo = pick(a.lo, b.lo);
+    f.hi = pick(a.hi, b.hi);
+    f.psi_off = pick(a.psi_off, b.psi_off);
+    f.face_elems = pick(a.face_elems, b.face_elems);
+    f.coef_off = pick(a.coef_off, b.coef_off);
+    return f;
+}
+
+template <int R, class T>
+__device__ __forceinline__ uint32_t psi_byte(const Geom<T>& g, const FaceDev<T>& f, int x, const Cell& c, int ir) {
+    return ((uint32_t)f.psi_off + psi_index<R>(g, f, x, c.y, c.z, c.p, ir)) * (uint32_t)sizeof(T);
+}
+
+template <class T>
+struct PsiPair {
+    T u[3], v[3];
+};
+
+template <int R, class T>
+__device__ __forceinline__ void psi_fetch_axis(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
+                                               PsiHit h, PsiPair<T>& q) {
+    q.u[R] = (T)0;
+    q.v[R] = (T)0;
+    if (h.on()) {
+        const FaceDev<T> f = face_of(g, R, h.slot());
+        const uint32_t bu = psi_byte<R>(g, f, x, c, h.ir());
+        q.u[R] = ld(psi_in, bu);
+        q.v[R] = ld(psi_in, bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T));
+    }
+}
+
+// hit[0..2]: the cell's PsiHit along x, y, z
+template <class T>
+__device__ __forceinline__ void psi_fetch(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
+                                          const PsiHit* hit, PsiPair<T>& q) {
+    psi_fetch_axis<0>(g, psi_in, x, c, hit[0], q);
+    psi_fetch_axis<1>(g, psi_in, x, c, hit[1], q);
+    psi_fetch_axis<2>(g, psi_in, x, c, hit[2], q);
+}
+
+template <int R, class T, bool ISE>
+__device__ __forceinline__ void cpml_axis(const Geom<T>& g, int x, const Cell& c, PsiHit h,
+                                          const FaceCoef<T>& k, const PsiPair<T>& q, const T* psi_in,
+                                          T* psi_out, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
+                                          T& dzy, T& px, T& py, T& pz) {
+    if (!h.on()) return;
+    const FaceDev<T> f = face_of(g, R, h.slot());
+    const uint32_t bu = psi_byte<R>(g, f, x, c, h.ir());
+    face_step<R, T, ISE>(k, q.u[R], q.v[R], psi_out, bu, bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T),
+                         write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    if (h.both()) {
+        const FaceDev<T>& f1 = g.face[R][1];
+        const int cr = R == 0 ? x : R == 1 ? c.y : c.z;
+        const int sr = R == 0 ? g.sx : R == 1 ? g.sy : g.sz;
+        int ir1 = 0;
+        (void)slab(f1, cr, sr, &ir1);
+        apply_face<R, T, ISE>(g, f1, ir1, psi_index<R>(g, f1, x, c.y, c.z, c.p, ir1), psi_in, psi_out, write,
+                              dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    }
+}
+
+// The coefficients of the cell's slab on each axis (loaded together with
+// the psi pairs, so that they share their round trip).
+template <class T, bool ISE>
+__device__ __forceinline__ void cpml_coefs(const Geom<T>& g, const PsiHit* hit, FaceCoef<T>* k) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        k[r].p0 = k[r].p1 = k[r].p2 = (T)0;
+        if (hit[r].on()) k[r] = face_coef<T, ISE>(g, face_of(g, r, hit[r].slot()), hit[r].ir());
+    }
+}
+
+template <class T, bool ISE>
+__device__ __forceinline__ void cpml_apply(const Geom<T>& g, int x, const Cell& c, const PsiHit* hit,
+                                           const FaceCoef<T>* k, const PsiPair<T>& q, const T* psi_in,
+                                           T* psi_out, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
+                                           T& dzy, T& px, T& py, T& pz) {
+    cpml_axis<0, T, ISE>(g, x, c, hit[0], k[0], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    cpml_axis<1, T, ISE>(g, x, c, hit[1], k[1], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    cpml_axis<2, T, ISE>(g, x, c, hit[2], k[2], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+}
+
+// ---------------------------------------------------------------------------
 // E half-step, update_e.cl:20-107.
 // ---------------------------------------------------------------------------
 
@@ -366,10 +527,11 @@ __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, cons
     }
 }
 
-template <class T>
+template <class T, bool PRE = false>
 __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, const Cell& c, int x, uint32_t oi,
                                          const EIn<T>& v, T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy,
-                                         T iz, int src_comp, T amp) {
+                                         T iz, int src_comp, T amp, const PsiHit* hit = nullptr,
+                                         const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr) {
     T dxy = (v.hx - v.hx_ym) * iy;
     T dxz = (v.hx - v.hx_zm) * iz;
     T dyx = (v.hy - hy_xm) * ix;
@@ -377,7 +539,9 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, con
     T dzx = (v.hz - hz_xm) * ix;
     T dzy = (v.hz - v.hz_ym) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
-    cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    if (PRE)
+        cpml_apply<T, true>(g, x, c, hit, kc, *q, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     T ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
     T ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
     T ez = v.ez + g.dt / eps2 * (dyx - dxy + pz);
@@ -563,10 +727,11 @@ __device__ __forceinline__ void h_fetch(const Geom<T>& g, const HSys<T>& s, cons
 
 // The H update of cell (x, c).  ex, ey, ez: E at the cell.  Stores H_out and
 // psi_out when write is set.
-template <class T, bool SEAM>
+template <class T, bool SEAM, bool PRE = false>
 __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s, const Cell& c, int x,
                                              uint32_t pbytes, T ex, T ey, T ez, const HIn<T>& v, T ix, T iy,
-                                             T iz, bool write) {
+                                             T iz, bool write, const PsiHit* hit = nullptr,
+                                             const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr) {
     T ey1 = v.eyn, ez1 = v.ezn;
     if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
         ey1 = seam_mix(ey1, ld(s.P[1], c.op), s.sa[0], s.sb[0]);
@@ -579,8 +744,12 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s,
     T dzx = (ez1 - ez) * ix;
     T dzy = (v.ez_y - ez) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
-    cpml_cell<T, false>(g, x, c.y, c.z, c.p, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py,
-                        pz);
+    if (PRE)
+        cpml_apply<T, false>(g, x, c, hit, kc, *q, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                             py, pz);
+    else
+        cpml_cell<T, false>(g, x, c.y, c.z, c.p, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                            py, pz);
     HCell<T> r;
     r.hx0 = v.hx;
     r.hy0 = v.hy;
@@ -706,7 +875,7 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
                 lds[b][1][t] = oSzy;
                 lds[b][2][t] = oSxz;
                 lds[b][3][t] = oSyz;
-                __syncthreads();
+                lds_barrier();
                 if (own) {
                     nxy = lds[b][0][t - g.sz];
                     nzy = lds[b][1][t - g.sz];
@@ -743,6 +912,155 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
             qz[k] = cur[k].ezn;
             if (FDTD_PREFETCH_H) cur[k] = nxt[k];
         }
+        x = xn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// H half-step with S, then the next E half-step, in one march (he_march).
+//
+// One pass over the grid instead of two: E^n, H^{n-1/2} and eps are read
+// once, H^{n+1/2}, S and E^{n+1} written once (72 bytes per FP32 cell instead
+// of 96).  The workgroup layout is MODE 2's: whole rows plus one halo row,
+// and a dry plane x0-1 in front of the chunk.  At plane x every thread
+// evaluates H^{n+1/2}(x) and the oS products; they go through LDS, and the
+// owned cells then finish S(x) and do the E update of plane x, which needs
+// H^{n+1/2} at x (own), x-1 (previous iteration, registers), y-1 (the row
+// before, or the halo row) and z-1 (same row) -- the neighbours whose oS
+// products S needs too.  E is read from E and written to E_out, H from H and
+// written to H_out, so no workgroup ever reads a value another one has
+// already overwritten; psi_E is touched by its own cell only (in place).
+// Per cell the arithmetic is h_update + S (as in h_march MODE 2) followed by
+// e_update (as in e_march), so the results are bit-identical to the two
+// kernels run one after the other.
+// ---------------------------------------------------------------------------
+template <class T>
+struct HEArgs {
+    HArgs<T> h;
+    const T* eps[3];
+    T* Eout[3];
+    T* psi_e;
+    const T* src_amp;        // E_out[src] += *src_amp after the E update; null: none
+    int src_x, src_p, src_comp;
+};
+
+template <class T, int HB>
+__global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
+    __shared__ T lds[2][7][HB];
+    const Geom<T>& g = a.h.g;
+    const HSys<T>& hs = a.h.s[0];
+    int pb, ck;
+    block_coords(g.pblocks, g.nchunk, &pb, &ck);
+    const int t = threadIdx.x;
+    const int halo = g.sz;
+    const int praw = pb * g.tile - halo + t;
+    const bool valid = (t < g.tile + halo) && praw < g.plane;
+    const int p = praw < 0 ? praw + g.plane : praw;
+    const bool own = valid && t >= halo;
+    const Cell c = make_cell<T>(g, valid ? p : 0);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const int x0 = ck * g.xchunk;
+    const int x1 = min(x0 + g.xchunk, g.sx);
+    const T sf = g.dt * (T)0.125;
+    const T iy = g.inv_e[1] ? g.inv_e[1][c.y] : g.inv_u;     // H step
+    const T iz = g.inv_e[2] ? g.inv_e[2][c.z] : g.inv_u;
+    const T jy = g.inv_h[1] ? g.inv_h[1][c.y] : g.inv_u;     // E step
+    const T jz = g.inv_h[2] ? g.inv_h[2][c.z] : g.inv_u;
+    const int xs = (x0 == 0) ? g.sx - 1 : x0 - 1;
+    const int n_it = (x1 - x0) + 1;
+    // the E half-step writes E_out and psi_E through an ESys (its H and
+    // partner fields are not used: H comes from this kernel's registers/LDS)
+    ESys<T> es;
+    for (int r = 0; r < 3; ++r) { es.E[r] = a.Eout[r]; es.H[r] = nullptr; es.P[r] = nullptr; es.sa[r] = es.sb[r] = (T)0; }
+    es.psi = a.psi_e;
+    const int src_comp = (own && a.src_amp != nullptr && p == a.src_p) ? a.src_comp : -1;
+    PsiHit hit[3];
+    hit[1] = slab_hit(g, 1, c.y, g.sy);
+    hit[2] = slab_hit(g, 2, c.z, g.sz);
+
+    T qx, qy, qz;                 // E^n at the current plane
+    HIn<T> cur;
+    {
+        const uint32_t ob = (uint32_t)xs * pbytes;
+        qx = ld(hs.E[0], ob + c.op);
+        qy = ld(hs.E[1], ob + c.op);
+        qz = ld(hs.E[2], ob + c.op);
+    }
+    T oSyx_m = (T)0, oSzx_m = (T)0;   // oS products of the -x neighbour
+    T hy_m = (T)0, hz_m = (T)0;       // H^{n+1/2} of the -x neighbour
+    int x = xs;
+    for (int it = 0; it < n_it; ++it) {
+        const int xn = (x == g.sx - 1) ? 0 : x + 1;
+        const bool dry = it == 0;   // uniform
+        const uint32_t oi = (uint32_t)x * pbytes + c.op;
+        // every load of the plane is issued here, before any of them is used:
+        // psi pairs, eps, then the H-step inputs
+        hit[0] = slab_hit(g, 0, x, g.sx);
+        PsiPair<T> qh{}, qe{};
+        FaceCoef<T> kh[3], ke[3];
+        T e0 = (T)1, e1 = (T)1, e2 = (T)1;
+        const T jx = g.inv_h[0] ? g.inv_h[0][x] : g.inv_u;
+        const bool src = src_comp >= 0 && x == a.src_x;
+        const T amp = src ? *a.src_amp : (T)0;
+        cpml_coefs<T, false>(g, hit, kh);
+        cpml_coefs<T, true>(g, hit, ke);
+        if (valid) psi_fetch(g, hs.psi_in, x, c, hit, qh);
+        if (own && !dry) {
+            psi_fetch(g, (const T*)a.psi_e, x, c, hit, qe);
+            e0 = lds_<FDTD_NT_E>(a.eps[0], oi);
+            e1 = lds_<FDTD_NT_E>(a.eps[1], oi);
+            e2 = lds_<FDTD_NT_E>(a.eps[2], oi);
+        }
+        h_fetch<T, false>(g, hs, c, x, pbytes, cur);
+        const T ix = g.inv_e[0] ? g.inv_e[0][x] : g.inv_u;
+        T oSxy = (T)0, oSxz = (T)0, oSyz = (T)0, oSyx = (T)0, oSzx = (T)0, oSzy = (T)0;
+        T hx1 = (T)0, hy1 = (T)0, hz1 = (T)0;
+        if (valid) {
+            const HCell<T> h = h_update<T, false, true>(g, hs, c, x, pbytes, qx, qy, qz, cur, ix, iy, iz,
+                                                        own && !dry, hit, kh, &qh);
+            const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+            oSxy = h.aEyx * aHz;
+            oSxz = -h.aEzx * aHy;
+            oSyz = h.aEzy * aHx;
+            oSyx = -h.aExy * aHz;
+            oSzx = h.aExz * aHy;
+            oSzy = -h.aEyz * aHx;
+            hx1 = h.hx1; hy1 = h.hy1; hz1 = h.hz1;
+        }
+        const int b = it & 1;
+        lds[b][0][t] = oSxy;
+        lds[b][1][t] = oSzy;
+        lds[b][2][t] = oSxz;
+        lds[b][3][t] = oSyz;
+        lds[b][4][t] = hx1;
+        lds[b][5][t] = hy1;
+        lds[b][6][t] = hz1;
+        lds_barrier();
+        if (own && !dry) {
+            const int ty = t - g.sz;
+            const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
+            const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
+            const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
+            sts_<FDTD_NT_H>(a.h.S[0], oi, (oSxy + oSxz + nxy + nxz) * sf);
+            sts_<FDTD_NT_H>(a.h.S[1], oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+            sts_<FDTD_NT_H>(a.h.S[2], oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            EIn<T> v;
+            v.hx = hx1; v.hy = hy1; v.hz = hz1;
+            v.hx_ym = lds[b][4][ty];
+            v.hz_ym = lds[b][6][ty];
+            v.hx_zm = lds[b][4][tz];
+            v.hy_zm = lds[b][5][tz];
+            v.ex = qx; v.ey = qy; v.ez = qz;
+            e_update<T, true>(g, es, c, x, oi, v, hy_m, hz_m, e0, e1, e2, jx, jy, jz, src ? src_comp : -1, amp,
+                              hit, ke, &qe);
+        }
+        oSyx_m = oSyx;
+        oSzx_m = oSzx;
+        hy_m = hy1;
+        hz_m = hz1;
+        qx = cur.exn;
+        qy = cur.eyn;
+        qz = cur.ezn;
         x = xn;
     }
 }
@@ -985,6 +1303,42 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     return check_launch("fdtd_update_h");
 }
 
+bool he_supported(const Plan& P) {
+    return P.d.poynting == FDTD_POYNTING_FUSED && P.d.bloch_mask == 0 && P.lh.mode == 2;
+}
+
+template <class T>
+int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index) {
+    HEArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    fill_h<T>(P, P.lh, f, a.h);
+    const int64_t N = P.N;
+    comp3c<T>(f->eps, N, a.eps);
+    comp3<T>(f->E_out, N, a.Eout);
+    a.psi_e = (T*)f->psi_e;
+    a.src_amp = src_amp;
+    if (src_amp) {
+        const int64_t comp = src_index / N;
+        const int64_t i = src_index - comp * N;
+        a.src_comp = (int)comp;
+        a.src_x = (int)(i / P.plane);
+        a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
+    }
+    const Launch& L = P.lh;
+    const dim3 grid(L.pblocks * L.nchunk);
+    if (L.block == FDTD_HB_SMALL && FDTD_HB_SMALL != 1024)
+        hipLaunchKernelGGL((he_march<T, FDTD_HB_SMALL>), grid, dim3(FDTD_HB_SMALL), 0, st, a);
+    else hipLaunchKernelGGL((he_march<T, 1024>), grid, dim3(1024), 0, st, a);
+    return check_launch("fdtd_update_he");
+}
+
+int check_he_fields(const Plan& P, const fdtd_fields* f) {
+    if (!he_supported(P))
+        return fail(3, "fdtd_update_he needs fused Poynting, no Bloch boundaries and rows of at most 341 cells");
+    if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he needs E_out != E");
+    return 0;
+}
+
 int check_fields(const Plan& P, const fdtd_fields* f, bool for_h) {
     if (!f) return fail(2, "fields is NULL");
     if (!f->E || !f->H || !f->eps) return fail(2, "E, H and eps are required");
@@ -1187,17 +1541,16 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     fdtd_fields cur = *f;
     const size_t esz = P.d.dtype == FDTD_F32 ? 4 : 8;
     const bool swap = (f->H_out && f->H_out != f->H);
-    for (int k = 0; k < n; ++k) {
-        const void* amp = src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
-        int rc = (P.d.dtype == FDTD_F32) ? launch_e<float>(P, &cur, st, (const float*)amp, src_index)
-                                         : launch_e<double>(P, &cur, st, (const double*)amp, src_index);
-        if (rc) return rc;
-        rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
-        if (rc) return rc;
-        if (P.d.poynting == FDTD_POYNTING_TWO_PASS) {
-            rc = fdtd_update_s(plan, &cur, stream);
-            if (rc) return rc;
-        }
+    const auto amp_of = [&](int k) -> const void* {
+        return src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
+    };
+    const auto half_e = [&](int k) {
+        return (P.d.dtype == FDTD_F32) ? launch_e<float>(P, &cur, st, (const float*)amp_of(k), src_index)
+                                       : launch_e<double>(P, &cur, st, (const double*)amp_of(k), src_index);
+    };
+    const auto half_h = [&]() {
+        int rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
+        if (rc == 0 && P.d.poynting == FDTD_POYNTING_TWO_PASS) rc = fdtd_update_s(plan, &cur, stream);
         if (swap) {
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
@@ -1206,9 +1559,53 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
                 std::swap(cur.psi_g, cur.psi_g_out);
             }
         }
+        return rc;
+    };
+    // Half-steps E_0 H_0 E_1 H_1 ... E_{n-1} H_{n-1}.  Fused: E_0, then
+    // (H_{k-1}, E_k) in one he_march each, then H_{n-1}.  Every he_march
+    // moves E to the other buffer, so their count is kept even (H_0, E_1
+    // unfused when n-1 is odd) and E ends in f->E.
+    const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P);
+    int k = 0;
+    if (fuse) {
+        if (int rc = half_e(0)) return rc;
+        k = 1;
+        if ((n - 1) % 2 == 1) {
+            if (int rc = half_h()) return rc;
+            if (int rc = half_e(1)) return rc;
+            k = 2;
+        }
+        for (; k < n; ++k) {
+            int rc = (P.d.dtype == FDTD_F32) ? launch_he<float>(P, &cur, st, (const float*)amp_of(k), src_index)
+                                             : launch_he<double>(P, &cur, st, (const double*)amp_of(k), src_index);
+            if (rc) return rc;
+            std::swap(cur.H, cur.H_out);
+            std::swap(cur.psi_h, cur.psi_h_out);
+            std::swap(cur.E, cur.E_out);
+        }
+        return half_h();
+    }
+    for (k = 0; k < n; ++k) {
+        if (int rc = half_e(k)) return rc;
+        if (int rc = half_h()) return rc;
     }
     return 0;
 }
+
+int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,
+                   void* stream) {
+    if (!plan) return fail(1, "plan is NULL");
+    const Plan& P = *(const Plan*)plan;
+    if (int rc = check_fields(P, f, true)) return rc;
+    if (int rc = check_he_fields(P, f)) return rc;
+    if (src_amp && (src_index < 0 || src_index >= 3 * P.N))
+        return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
+    hipStream_t st = (hipStream_t)stream;
+    if (P.d.dtype == FDTD_F32) return launch_he<float>(P, f, st, (const float*)src_amp, src_index);
+    return launch_he<double>(P, f, st, (const double*)src_amp, src_index);
+}
+
+int fdtd_update_he_supported(const void* plan) { return plan && he_supported(*(const Plan*)plan) ? 1 : 0; }
 
 }  // extern "C"
 

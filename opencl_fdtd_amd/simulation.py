@@ -22,6 +22,12 @@ Differences from the reference, all documented behaviour:
   E/F, which the paired step leaves untouched); ``update_F``/``update_G`` are
   event-only.  ``bloch_mode='phase'`` (default) applies the phase factor at
   the seam; ``'decoupled'`` reproduces the reference as shipped.
+* ``step_n`` (n steps without host round trips) runs each H half-step and
+  the next E half-step as one pass over the grid when the plan allows it
+  (fused Poynting, no Bloch boundaries, rows of at most 341 cells;
+  ``fuse_steps=False`` turns it off and saves the second E buffer it needs).
+  E ends in the same buffer it started in; results are bit-identical.
+  ``update_HE`` exposes that pass once; it moves E to the other buffer.
 * An unstable ``dt`` warns and is then used (the reference warns and leaves
   ``dt`` unset); malformed initial fields raise (the reference builds the
   exception without raising it).
@@ -114,7 +120,8 @@ class Simulation:
                  do_poynting: bool = True,
                  do_fieldsrc: bool = False,
                  poynting_mode: str = "fused",
-                 bloch_mode: str = "phase"):
+                 bloch_mode: str = "phase",
+                 fuse_steps: bool = True):
         if initial_fields is None:
             initial_fields = {}
         self._lib = _lib.load()
@@ -239,6 +246,11 @@ class Simulation:
         plan = ctypes.c_void_p()
         _lib.check(self._lib.fdtd_plan_create(ctypes.byref(desc), ctypes.byref(plan)), "fdtd_plan_create")
         self._plan = plan
+        # second E buffer: step_n then runs each H half-step and the next E
+        # half-step as one pass over the grid (fdtd_update_he)
+        self._E_out = None
+        if fuse_steps and self._lib.fdtd_update_he_supported(plan):
+            self._E_out = torch.empty_like(self._E)
         n_psi = int(self._lib.fdtd_psi_elems(plan))
         self._coef = _vec([numpy.concatenate(coefs)] if coefs else [numpy.zeros(1)], self._tdtype, self.device)
         zeros = lambda: torch.zeros(max(n_psi, 1), dtype=self._tdtype, device=self.device)
@@ -322,6 +334,7 @@ class Simulation:
         f.psi_g_out = f.psi_g
         f.coef = ptr(self._coef)
         f.inv_d = ptr(self._inv_d)
+        f.E_out = ptr(self._E_out)
         return f
 
     def _launch(self, fn, events, what, *extra):
@@ -360,6 +373,27 @@ class Simulation:
                             ctypes.c_double(c), ctypes.c_double(s), box)
 
     # ------------------------------------------------------------------ fast path
+    def update_HE(self, events: Sequence = (), src_index: Optional[int] = None, src_amp=None) -> Event:
+        """``update_H`` (with S) of one step and ``update_E`` of the next,
+        then ``E[src_index] += src_amp[0]`` (a one-value device tensor), in
+        one pass over the grid.  Needs ``fuse_steps`` and a plan that
+        supports it (fused Poynting, no Bloch boundaries, sz <= 341).  The
+        new E is in a second buffer afterwards: read ``sim.E`` again."""
+        if self._E_out is None:
+            raise _lib.FdtdError("update_HE is not available for this simulation (fdtd_update_he_supported)")
+        amp = None
+        if src_amp is not None:
+            if src_index is None:
+                raise Exception("src_index is required with src_amp")
+            amp = src_amp.to(device=self.device, dtype=self._tdtype).contiguous()
+            self._amp_keepalive = amp
+        ev = self._launch(self._lib.fdtd_update_he, events, "fdtd_update_he",
+                          ctypes.c_int64(int(src_index) if amp is not None else -1),
+                          ctypes.c_void_p(amp.data_ptr() if amp is not None else None))
+        self._h = 1 - self._h
+        self._E, self._E_out = self._E_out, self._E
+        return ev
+
     def step_n(self, n: int, src_index: Optional[int] = None, src_wave=None, events: Sequence = ()) -> Event:
         """``n`` iterations of the reference caller loop (fdtd.py:162-175)
         enqueued without host synchronisation: E (+F), ``E[src_index] +=
@@ -408,6 +442,11 @@ class Simulation:
             self._G.copy_(st["G"])
             self._psi_f.copy_(st["psi_f"])
             self._psi_g.copy_(st["psi_g"])
+
+    @property
+    def fused_steps(self) -> bool:
+        """True when step_n runs the fused H+E pass (see update_HE)."""
+        return self._E_out is not None
 
     @property
     def plan(self) -> ctypes.c_void_p:

@@ -1,0 +1,128 @@
+"""The fused H+E pass (fdtd_update_he, used by step_n) against the unfused
+kernels and the CPU oracle.
+
+The fused pass does, per cell, the same arithmetic in the same order as the
+H kernel followed by the E kernel, so every comparison here is bit-exact:
+fields, S and both CPML psi blocks.
+"""
+import numpy
+import pytest
+import torch
+
+from tests.helpers import build_pair, fetch, random_fields, step_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(shape, T, seed, nonuniform=False, pec=False):
+    rng = numpy.random.default_rng(seed)
+    eps = [rng.uniform(1, 6, shape) for _ in range(3)]
+    if pec:
+        for e in eps:
+            e[shape[0] // 2, :, :shape[2] // 3] = numpy.inf
+    pmls = [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    dxes = None
+    if nonuniform:
+        dxes = [[rng.uniform(0.8, 1.25, s) for s in shape] for _ in range(2)]
+    return eps, pmls, init, dxes
+
+
+def _sim(eps, pmls, init, dxes, dtype, fuse):
+    from opencl_fdtd_amd import Simulation
+    return Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], dxes=dxes,
+                      initial_fields=init, float_type=dtype, fuse_steps=fuse)
+
+
+def _state(sim):
+    return {"E": fetch(sim, "E"), "H": fetch(sim, "H"), "S": fetch(sim, "S"),
+            "psi_e": sim._psi_e.cpu().numpy(),
+            "psi_h": sim._psi_h[sim._h].cpu().numpy()}
+
+
+def _assert_same(a, b, what):
+    for k in a:
+        bad = numpy.flatnonzero(a[k] != b[k])
+        assert bad.size == 0, "%s: %s differs at %d of %d values, first %s: %r vs %r" % (
+            what, k, bad.size, a[k].size, bad[:4], a[k][bad[:4]], b[k][bad[:4]])
+
+
+@pytest.mark.parametrize("shape, T, n, dtype, nonuniform", [
+    ((11, 13, 9), 3, 2, numpy.float32, False),
+    ((11, 13, 9), 3, 3, numpy.float32, False),
+    ((11, 13, 9), 3, 8, numpy.float32, True),
+    ((9, 40, 73), 4, 7, numpy.float32, False),       # rows of 73: 6 rows + halo per 512-thread tile
+    ((6, 9, 200), 2, 5, numpy.float32, False),       # 1024-thread tiles (rows of 129..341 cells)
+    ((5, 37, 3), 2, 6, numpy.float32, False),        # short rows, many per tile
+    ((33, 8, 17), 3, 9, numpy.float64, True),
+    ((5, 6, 160), 2, 4, numpy.float64, False),       # FP64, 1024-thread tiles
+])
+def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
+    print("case fused step_n", shape, T, n, dtype.__name__, nonuniform, flush=True)
+    eps, pmls, init, dxes = _case(shape, T, seed=sum(shape) + n, nonuniform=nonuniform, pec=True)
+    a = _sim(eps, pmls, init, dxes, dtype, True)
+    b = _sim(eps, pmls, init, dxes, dtype, False)
+    assert a._E_out is not None and b._E_out is None
+    N = a.N
+    idx = 2 * N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 3) * shape[2] + shape[2] // 2
+    wave = numpy.sin(0.3 * numpy.arange(3 * n)).astype(dtype)
+    E0 = a._E
+    for k in range(3):                           # three calls: E must stay in its buffer
+        a.step_n(n, idx, wave[k * n:(k + 1) * n])
+        b.step_n(n, idx, wave[k * n:(k + 1) * n])
+    torch.cuda.synchronize()
+    assert a._E.data_ptr() == E0.data_ptr()
+    _assert_same(_state(a), _state(b), "step_n fused vs unfused")
+
+
+@pytest.mark.parametrize("dtype", [numpy.float32, numpy.float64])
+def test_update_he_equals_h_then_e(dtype):
+    """One fdtd_update_he = update_H (+S), update_E, E[idx] += amp."""
+    shape = (10, 12, 14)
+    eps, pmls, init, _ = _case(shape, 3, seed=5)
+    a = _sim(eps, pmls, init, None, dtype, True)
+    b = _sim(eps, pmls, init, None, dtype, False)
+    for s in (a, b):                             # non-zero psi first
+        for _ in range(3):
+            s.update_E([])
+            s.update_H([])
+    idx = 1 * a.N + 5 * shape[1] * shape[2] + 6 * shape[2] + 7
+    tdt = torch.float32 if dtype == numpy.float32 else torch.float64
+    for amp in (0.75, -1.5):
+        t_amp = torch.tensor([amp], dtype=tdt, device="cuda")
+        a.update_HE([], idx, t_amp)
+        b.update_H([])
+        b.update_E([])
+        b.E[idx] += t_amp[0]
+        torch.cuda.synchronize()
+        _assert_same(_state(a), _state(b), "update_HE amp %g" % amp)
+
+
+def test_fused_step_n_matches_oracle_1024_tiles():
+    """Rows of 150 cells (1024-thread tiles) against the oracle, with a source."""
+    shape = (14, 11, 150)
+    eps, pmls, init, _ = _case(shape, 4, seed=9)
+    sim, orc = build_pair(eps, pmls, init=init)
+    assert sim._E_out is not None
+    idx = 2 * sim.N + 7 * shape[1] * shape[2] + 5 * shape[2] + 75
+    wave = numpy.cos(0.2 * numpy.arange(21))
+    sim.step_n(21, idx, wave.astype(numpy.float32)).wait()
+    step_oracle(orc, 21, idx, wave)
+    for name in ("E", "H", "S"):
+        g, o = fetch(sim, name), getattr(orc, name)
+        bad = numpy.flatnonzero(g != o)
+        assert bad.size == 0, (name, bad.size, bad[:4])
+
+
+def test_fusion_is_off_where_unsupported():
+    from opencl_fdtd_amd import Simulation
+    shape = (6, 5, 7)
+    eps = [numpy.ones(shape)] * 3
+    assert Simulation(eps, [])._E_out is not None
+    assert Simulation(eps, [], poynting_mode="two_pass")._E_out is None
+    assert Simulation(eps, [], do_poynting=False)._E_out is None
+    bloch = [{"axis": "x", "real": 1.0, "imag": 0.0}]
+    assert Simulation(eps, [], bloch_boundaries=bloch)._E_out is None
+    s = Simulation(eps, [], poynting_mode="two_pass")
+    with pytest.raises(Exception):
+        s.update_HE([])

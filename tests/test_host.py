@@ -34,7 +34,7 @@ def test_struct_layouts_match_header():
     # fdtd_face: 4 int32; fdtd_desc: 3 int64 + 2 int32 + 12 faces + 2 double + 4 int32 + 6 double
     assert ctypes.sizeof(_lib.Face) == 16
     assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
-    assert ctypes.sizeof(_lib.Fields) == 17 * 8
+    assert ctypes.sizeof(_lib.Fields) == 18 * 8
     assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4
 
 
@@ -159,3 +159,26 @@ def test_workloads():
     assert len(b.bloch_boundaries) == 2 and [p["axis"] for p in b.pmls] == ["z", "z"]
     bytes_c2 = W.algorithmic_bytes_per_cell(s.shape, s.pmls)
     assert abs(bytes_c2 - 105.8) < 0.1
+
+
+def test_update_he_support_rules():
+    """The fused H+E pass needs fused Poynting, no Bloch boundaries and
+    rows of at most 341 cells (plan-level query, no GPU)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+
+    def supported(**kw):
+        shape = kw.pop("shape", (10, 11, 12))
+        d = _desc(shape, [(0, 0, 2)], **kw)
+        plan = ctypes.c_void_p()
+        assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+        r = lib.fdtd_update_he_supported(plan)
+        lib.fdtd_plan_destroy(plan)
+        return r
+    assert supported(poynting=1) == 1
+    assert supported(poynting=1, shape=(4, 4, 341)) == 1
+    assert supported(poynting=1, shape=(4, 4, 342)) == 0
+    assert supported(poynting=0) == 0
+    assert supported(poynting=2) == 0
+    assert supported(poynting=1, bloch_mask=1) == 0
+    assert lib.fdtd_update_he_supported(None) == 0
