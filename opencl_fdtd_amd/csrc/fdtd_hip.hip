@@ -82,6 +82,18 @@ namespace {
 #ifndef FDTD_HE_WAVES
 #define FDTD_HE_WAVES 4              // same, fused H+E kernel
 #endif
+#ifndef FDTD_HE_HB_SMALL
+#define FDTD_HE_HB_SMALL 1024        // fused H+E workgroup for rows of <= 128 cells
+#endif
+#ifndef FDTD_HE_CTAB
+#define FDTD_HE_CTAB 1               // fused H+E: CPML coefficients read from an LDS copy
+#endif
+#ifndef FDTD_HE_PF
+#define FDTD_HE_PF 0                 // fused H+E, FP32: issue the loads of plane x+1 before computing plane x
+#endif
+#ifndef FDTD_HE_TARGET
+#define FDTD_HE_TARGET 1024          // fused H+E workgroups per launch
+#endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
 constexpr int kTargetBlocks = FDTD_TARGET_BLOCKS;
@@ -199,6 +211,31 @@ __device__ __forceinline__ void sts_(T* base, uint32_t boff, T v) {
         st(base, boff, v);
 }
 
+// The three components of a vector field.  Split: one pointer per
+// component.  Packed3: one base with the components nb bytes apart (the
+// 3N layout of a field buffer), which keeps a single base pointer per field
+// in scalar registers; usable while 3N values fit 32-bit byte offsets.
+template <class T>
+struct Packed3 {
+    T* b;
+    uint32_t nb;
+};
+
+template <bool NT, class T>
+__device__ __forceinline__ T ldc(const T* const (&f)[3], int k, uint32_t off) { return lds_<NT>(f[k], off); }
+template <bool NT, class T>
+__device__ __forceinline__ T ldc(T* const (&f)[3], int k, uint32_t off) { return lds_<NT>((const T*)f[k], off); }
+template <bool NT, class T>
+__device__ __forceinline__ T ldc(const Packed3<const T>& f, int k, uint32_t off) {
+    return lds_<NT>(f.b, off + (uint32_t)k * f.nb);
+}
+template <bool NT, class T>
+__device__ __forceinline__ void stc(T* const (&f)[3], int k, uint32_t off, T v) { sts_<NT>(f[k], off, v); }
+template <bool NT, class T>
+__device__ __forceinline__ void stc(const Packed3<T>& f, int k, uint32_t off, T v) {
+    sts_<NT>(f.b, off + (uint32_t)k * f.nb, v);
+}
+
 // Workgroup -> (plane tile, x chunk).  Hardware workgroup b runs on XCD
 // b % 8; logical tiles are handed out so each XCD gets a contiguous range.
 __device__ __forceinline__ void block_coords(int pblocks, int nchunk, int* pb, int* ck) {
@@ -256,6 +293,17 @@ struct FaceCoef {
 template <class T, bool ISE>
 __device__ __forceinline__ FaceCoef<T> face_coef(const Geom<T>& g, const FaceDev<T>& f, int ir) {
     const T* c = g.coef + f.coef_off + (ISE ? 0 : 3 * f.thick);
+    FaceCoef<T> k;
+    k.p0 = c[ir];
+    k.p1 = c[f.thick + ir];
+    k.p2 = c[2 * f.thick + ir];
+    return k;
+}
+
+// the same from a copy of the coefficient block (e.g. in LDS)
+template <class T, bool ISE>
+__device__ __forceinline__ FaceCoef<T> face_coef_at(const T* tab, const FaceDev<T>& f, int ir) {
+    const T* c = tab + f.coef_off + (ISE ? 0 : 3 * f.thick);
     FaceCoef<T> k;
     k.p0 = c[ir];
     k.p1 = c[f.thick + ir];
@@ -432,8 +480,6 @@ struct PsiPair {
 template <int R, class T>
 __device__ __forceinline__ void psi_fetch_axis(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
                                                PsiHit h, PsiPair<T>& q) {
-    q.u[R] = (T)0;
-    q.v[R] = (T)0;
     if (h.on()) {
         const FaceDev<T> f = face_of(g, R, h.slot());
         const uint32_t bu = psi_byte<R>(g, f, x, c, h.ir());
@@ -451,15 +497,18 @@ __device__ __forceinline__ void psi_fetch(const Geom<T>& g, const T* psi_in, int
     psi_fetch_axis<2>(g, psi_in, x, c, hit[2], q);
 }
 
+// tab: the coefficient block copied to LDS (read here, just before use), or
+// null to use the coefficients k loaded beforehand
 template <int R, class T, bool ISE>
 __device__ __forceinline__ void cpml_axis(const Geom<T>& g, int x, const Cell& c, PsiHit h,
-                                          const FaceCoef<T>& k, const PsiPair<T>& q, const T* psi_in,
-                                          T* psi_out, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
-                                          T& dzy, T& px, T& py, T& pz) {
+                                          const FaceCoef<T>& k, const T* tab, const PsiPair<T>& q,
+                                          const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
+                                          T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
     if (!h.on()) return;
     const FaceDev<T> f = face_of(g, R, h.slot());
     const uint32_t bu = psi_byte<R>(g, f, x, c, h.ir());
-    face_step<R, T, ISE>(k, q.u[R], q.v[R], psi_out, bu, bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T),
+    const FaceCoef<T> kk = tab ? face_coef_at<T, ISE>(tab, f, h.ir()) : k;
+    face_step<R, T, ISE>(kk, q.u[R], q.v[R], psi_out, bu, bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T),
                          write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     if (h.both()) {
         const FaceDev<T>& f1 = g.face[R][1];
@@ -478,19 +527,132 @@ template <class T, bool ISE>
 __device__ __forceinline__ void cpml_coefs(const Geom<T>& g, const PsiHit* hit, FaceCoef<T>* k) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        k[r].p0 = k[r].p1 = k[r].p2 = (T)0;
         if (hit[r].on()) k[r] = face_coef<T, ISE>(g, face_of(g, r, hit[r].slot()), hit[r].ir());
     }
 }
 
 template <class T, bool ISE>
 __device__ __forceinline__ void cpml_apply(const Geom<T>& g, int x, const Cell& c, const PsiHit* hit,
-                                           const FaceCoef<T>* k, const PsiPair<T>& q, const T* psi_in,
-                                           T* psi_out, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
-                                           T& dzy, T& px, T& py, T& pz) {
-    cpml_axis<0, T, ISE>(g, x, c, hit[0], k[0], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    cpml_axis<1, T, ISE>(g, x, c, hit[1], k[1], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    cpml_axis<2, T, ISE>(g, x, c, hit[2], k[2], q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+                                           const FaceCoef<T>* k, const T* tab, const PsiPair<T>& q,
+                                           const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
+                                           T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+    cpml_axis<0, T, ISE>(g, x, c, hit[0], k[0], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                         py, pz);
+    cpml_axis<1, T, ISE>(g, x, c, hit[1], k[1], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                         py, pz);
+    cpml_axis<2, T, ISE>(g, x, c, hit[2], k[2], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                         py, pz);
+}
+
+// The same for an in-plane axis (y or z) with the slab data a thread needs
+// worked out once before the march: the psi byte offset of the cell at
+// plane x is base + x * stride (psi_index along that axis), v is vofs bytes
+// further, and the coefficients of depth ir sit at cidx, cidx + thick,
+// cidx + 2 * thick (E step; the H step's follow 3 * thick later).
+struct PsiCol {
+    PsiHit h;
+    uint32_t base, stride, vofs;
+    int cidx, thick;
+};
+
+template <int R, class T>
+__device__ __forceinline__ PsiCol psi_col(const Geom<T>& g, const Cell& c, PsiHit h) {
+    PsiCol k;
+    k.h = h;
+    k.base = k.stride = k.vofs = 0;
+    k.cidx = k.thick = 0;
+    if (h.on()) {
+        const FaceDev<T> f = face_of(g, R, h.slot());
+        const uint32_t B = sizeof(T);
+        if (R == 1) {
+            k.base = ((uint32_t)f.psi_off + (uint32_t)h.ir() * (uint32_t)g.sz + (uint32_t)c.z) * B;
+            k.stride = (uint32_t)f.thick * (uint32_t)g.sz * B;
+        } else {
+            k.base = ((uint32_t)f.psi_off + (uint32_t)c.y * (uint32_t)f.thick + (uint32_t)h.ir()) * B;
+            k.stride = (uint32_t)g.sy * (uint32_t)f.thick * B;
+        }
+        k.vofs = (uint32_t)f.face_elems * B;
+        k.cidx = f.coef_off + h.ir();
+        k.thick = f.thick;
+    }
+    return k;
+}
+
+template <int R, class T>
+__device__ __forceinline__ void psi_fetch_col(const PsiCol& k, const T* psi_in, int x, PsiPair<T>& q) {
+    if (k.h.on()) {
+        const uint32_t bu = k.base + (uint32_t)x * k.stride;
+        q.u[R] = ld(psi_in, bu);
+        q.v[R] = ld(psi_in, bu + k.vofs);
+    }
+}
+
+template <int R, class T, bool ISE>
+__device__ __forceinline__ void cpml_col(const Geom<T>& g, int x, const Cell& c, const PsiCol& k, const T* tab,
+                                         const PsiPair<T>& q, const T* psi_in, T* psi_out, bool write, T& dxy,
+                                         T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+    if (!k.h.on()) return;
+    const uint32_t bu = k.base + (uint32_t)x * k.stride;
+    const T* cc = tab + k.cidx + (ISE ? 0 : 3 * k.thick);
+    FaceCoef<T> kk;
+    kk.p0 = cc[0];
+    kk.p1 = cc[k.thick];
+    kk.p2 = cc[2 * k.thick];
+    face_step<R, T, ISE>(kk, q.u[R], q.v[R], psi_out, bu, bu + k.vofs, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                         py, pz);
+    if (k.h.both()) {
+        const FaceDev<T>& f1 = g.face[R][1];
+        const int cr = R == 1 ? c.y : c.z;
+        const int sr = R == 1 ? g.sy : g.sz;
+        int ir1 = 0;
+        (void)slab(f1, cr, sr, &ir1);
+        apply_face<R, T, ISE>(g, f1, ir1, psi_index<R>(g, f1, x, c.y, c.z, c.p, ir1), psi_in, psi_out, write,
+                              dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    }
+}
+
+// The fused pass's CPML: x from the uniform per-plane hit, y and z from the
+// per-thread columns; coefficients from the table tab.
+template <class T>
+struct CpmlCols {
+    PsiHit hx;
+    PsiCol y, z;
+};
+
+template <class T>
+__device__ __forceinline__ void psi_fetch_cols(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
+                                               const CpmlCols<T>& k, PsiPair<T>& q) {
+    psi_fetch_axis<0>(g, psi_in, x, c, k.hx, q);
+    psi_fetch_col<1>(k.y, psi_in, x, q);
+    psi_fetch_col<2>(k.z, psi_in, x, q);
+}
+
+template <class T, bool ISE>
+__device__ __forceinline__ void cpml_cols(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>& k,
+                                          const T* tab, const PsiPair<T>& q, const T* psi_in, T* psi_out,
+                                          bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px,
+                                          T& py, T& pz) {
+    const FaceCoef<T> none{};
+    cpml_axis<0, T, ISE>(g, x, c, k.hx, none, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                         py, pz);
+    cpml_col<1, T, ISE>(g, x, c, k.y, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    cpml_col<2, T, ISE>(g, x, c, k.z, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+}
+
+// CPML with prefetched psi: per-axis hits (cpml_apply) or columns (cpml_cols)
+template <class T, bool ISE>
+__device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const PsiHit* hit,
+                                         const FaceCoef<T>* kc, const T* tab, const PsiPair<T>& q,
+                                         const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
+                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+    cpml_apply<T, ISE>(g, x, c, hit, kc, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+}
+template <class T, bool ISE>
+__device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>* cols,
+                                         const FaceCoef<T>*, const T* tab, const PsiPair<T>& q,
+                                         const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
+                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+    cpml_cols<T, ISE>(g, x, c, *cols, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
 }
 
 // ---------------------------------------------------------------------------
@@ -505,17 +667,17 @@ struct EIn {
 template <class T, bool SEAM>
 __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, const Cell& c, uint32_t ob,
                                         EIn<T>& v) {
-    v.hx = lds_<FDTD_NT_E>(s.H[0], ob + c.op);
-    v.hy = lds_<FDTD_NT_E>(s.H[1], ob + c.op);
-    v.hz = lds_<FDTD_NT_E>(s.H[2], ob + c.op);
-    v.hx_ym = ld(s.H[0], ob + c.om_y);
-    v.hz_ym = ld(s.H[2], ob + c.om_y);
-    v.hx_zm = ld(s.H[0], ob + c.om_z);
-    v.hy_zm = ld(s.H[1], ob + c.om_z);
-    v.ex = lds_<FDTD_NT_E>(s.E[0], ob + c.op);
-    v.ey = lds_<FDTD_NT_E>(s.E[1], ob + c.op);
-    v.ez = lds_<FDTD_NT_E>(s.E[2], ob + c.op);
-    if (SEAM) {
+    v.hx = ldc<FDTD_NT_E>(s.H, 0, ob + c.op);
+    v.hy = ldc<FDTD_NT_E>(s.H, 1, ob + c.op);
+    v.hz = ldc<FDTD_NT_E>(s.H, 2, ob + c.op);
+    v.hx_ym = ldc<false>(s.H, 0, ob + c.om_y);
+    v.hz_ym = ldc<false>(s.H, 2, ob + c.om_y);
+    v.hx_zm = ldc<false>(s.H, 0, ob + c.om_z);
+    v.hy_zm = ldc<false>(s.H, 1, ob + c.om_z);
+    v.ex = ldc<FDTD_NT_E>(s.E, 0, ob + c.op);
+    v.ey = ldc<FDTD_NT_E>(s.E, 1, ob + c.op);
+    v.ez = ldc<FDTD_NT_E>(s.E, 2, ob + c.op);
+    if constexpr (SEAM) {
         if ((g.seam_mask & 2) && c.y == 0) {
             v.hx_ym = seam_mix(v.hx_ym, ld(s.P[0], ob + c.om_y), s.sa[1], s.sb[1]);
             v.hz_ym = seam_mix(v.hz_ym, ld(s.P[2], ob + c.om_y), s.sa[1], s.sb[1]);
@@ -527,11 +689,12 @@ __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, cons
     }
 }
 
-template <class T, bool PRE = false>
-__device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, const Cell& c, int x, uint32_t oi,
+template <class T, bool PRE = false, class S, class K = const PsiHit*>
+__device__ __forceinline__ void e_update(const Geom<T>& g, const S& s, const Cell& c, int x, uint32_t oi,
                                          const EIn<T>& v, T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy,
-                                         T iz, int src_comp, T amp, const PsiHit* hit = nullptr,
-                                         const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr) {
+                                         T iz, int src_comp, T amp, K hit = nullptr,
+                                         const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr,
+                                         const T* tab = nullptr) {
     T dxy = (v.hx - v.hx_ym) * iy;
     T dxz = (v.hx - v.hx_zm) * iz;
     T dyx = (v.hy - hy_xm) * ix;
@@ -540,7 +703,7 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, con
     T dzy = (v.hz - v.hz_ym) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
     if (PRE)
-        cpml_apply<T, true>(g, x, c, hit, kc, *q, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+        cpml_pre<T, true>(g, x, c, hit, kc, tab, *q, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     T ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
     T ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
@@ -548,9 +711,9 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const ESys<T>& s, con
     if (src_comp == 0) ex = ex + amp;
     else if (src_comp == 1) ey = ey + amp;
     else if (src_comp == 2) ez = ez + amp;
-    sts_<FDTD_NT_E>(s.E[0], oi, ex);
-    sts_<FDTD_NT_E>(s.E[1], oi, ey);
-    sts_<FDTD_NT_E>(s.E[2], oi, ez);
+    stc<FDTD_NT_E>(s.E, 0, oi, ex);
+    stc<FDTD_NT_E>(s.E, 1, oi, ey);
+    stc<FDTD_NT_E>(s.E, 2, oi, ez);
 }
 
 // The learned shell's cell order: the x faces (T x sy x sz each), then the
@@ -698,22 +861,22 @@ struct HIn {
     T exn, eyn, ezn, ex_y, ez_y, ex_z, ey_z, hx, hy, hz;
 };
 
-template <class T, bool SEAM>
-__device__ __forceinline__ void h_fetch(const Geom<T>& g, const HSys<T>& s, const Cell& c, int x, uint32_t pbytes,
+template <class T, bool SEAM, class S>
+__device__ __forceinline__ void h_fetch(const Geom<T>& g, const S& s, const Cell& c, int x, uint32_t pbytes,
                                         HIn<T>& v) {
     const uint32_t ob = (uint32_t)x * pbytes;
     const uint32_t on = (x == g.sx - 1) ? 0u : ob + pbytes;
-    v.exn = lds_<FDTD_NT_H>(s.E[0], on + c.op);
-    v.eyn = lds_<FDTD_NT_H>(s.E[1], on + c.op);
-    v.ezn = lds_<FDTD_NT_H>(s.E[2], on + c.op);
-    v.ex_y = ld(s.E[0], ob + c.op_y);
-    v.ez_y = ld(s.E[2], ob + c.op_y);
-    v.ex_z = ld(s.E[0], ob + c.op_z);
-    v.ey_z = ld(s.E[1], ob + c.op_z);
-    v.hx = lds_<FDTD_NT_H>(s.Hin[0], ob + c.op);
-    v.hy = lds_<FDTD_NT_H>(s.Hin[1], ob + c.op);
-    v.hz = lds_<FDTD_NT_H>(s.Hin[2], ob + c.op);
-    if (SEAM) {
+    v.exn = ldc<FDTD_NT_H>(s.E, 0, on + c.op);
+    v.eyn = ldc<FDTD_NT_H>(s.E, 1, on + c.op);
+    v.ezn = ldc<FDTD_NT_H>(s.E, 2, on + c.op);
+    v.ex_y = ldc<false>(s.E, 0, ob + c.op_y);
+    v.ez_y = ldc<false>(s.E, 2, ob + c.op_y);
+    v.ex_z = ldc<false>(s.E, 0, ob + c.op_z);
+    v.ey_z = ldc<false>(s.E, 1, ob + c.op_z);
+    v.hx = ldc<FDTD_NT_H>(s.Hin, 0, ob + c.op);
+    v.hy = ldc<FDTD_NT_H>(s.Hin, 1, ob + c.op);
+    v.hz = ldc<FDTD_NT_H>(s.Hin, 2, ob + c.op);
+    if constexpr (SEAM) {
         if ((g.seam_mask & 2) && c.y == g.sy - 1) {
             v.ex_y = seam_mix(v.ex_y, ld(s.P[0], ob + c.op_y), s.sa[1], s.sb[1]);
             v.ez_y = seam_mix(v.ez_y, ld(s.P[2], ob + c.op_y), s.sa[1], s.sb[1]);
@@ -727,15 +890,18 @@ __device__ __forceinline__ void h_fetch(const Geom<T>& g, const HSys<T>& s, cons
 
 // The H update of cell (x, c).  ex, ey, ez: E at the cell.  Stores H_out and
 // psi_out when write is set.
-template <class T, bool SEAM, bool PRE = false>
-__device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s, const Cell& c, int x,
+template <class T, bool SEAM, bool PRE = false, class S, class K = const PsiHit*>
+__device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const S& s, const Cell& c, int x,
                                              uint32_t pbytes, T ex, T ey, T ez, const HIn<T>& v, T ix, T iy,
-                                             T iz, bool write, const PsiHit* hit = nullptr,
-                                             const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr) {
+                                             T iz, bool write, K hit = nullptr,
+                                             const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr,
+                                             const T* tab = nullptr) {
     T ey1 = v.eyn, ez1 = v.ezn;
-    if (SEAM && (g.seam_mask & 1) && x == g.sx - 1) {
-        ey1 = seam_mix(ey1, ld(s.P[1], c.op), s.sa[0], s.sb[0]);
-        ez1 = seam_mix(ez1, ld(s.P[2], c.op), s.sa[0], s.sb[0]);
+    if constexpr (SEAM) {
+        if ((g.seam_mask & 1) && x == g.sx - 1) {
+            ey1 = seam_mix(ey1, ld(s.P[1], c.op), s.sa[0], s.sb[0]);
+            ez1 = seam_mix(ez1, ld(s.P[2], c.op), s.sa[0], s.sb[0]);
+        }
     }
     T dxy = (v.ex_y - ex) * iy;
     T dxz = (v.ex_z - ex) * iz;
@@ -745,8 +911,8 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s,
     T dzy = (v.ez_y - ez) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
     if (PRE)
-        cpml_apply<T, false>(g, x, c, hit, kc, *q, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                             py, pz);
+        cpml_pre<T, false>(g, x, c, hit, kc, tab, *q, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                           py, pz);
     else
         cpml_cell<T, false>(g, x, c.y, c.z, c.p, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
                             py, pz);
@@ -759,9 +925,9 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const HSys<T>& s,
     r.hz1 = v.hz - g.dt * (dyx - dxy - pz);
     if (write) {
         const uint32_t oi = (uint32_t)x * pbytes + c.op;
-        sts_<FDTD_NT_H>(s.Hout[0], oi, r.hx1);
-        sts_<FDTD_NT_H>(s.Hout[1], oi, r.hy1);
-        sts_<FDTD_NT_H>(s.Hout[2], oi, r.hz1);
+        stc<FDTD_NT_H>(s.Hout, 0, oi, r.hx1);
+        stc<FDTD_NT_H>(s.Hout, 1, oi, r.hy1);
+        stc<FDTD_NT_H>(s.Hout, 2, oi, r.hz1);
     }
     r.aExy = v.ex_y + ex; r.aExz = v.ex_z + ex;
     r.aEyx = ey1 + ey; r.aEyz = v.ey_z + ey;
@@ -934,21 +1100,46 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
 // e_update (as in e_march), so the results are bit-identical to the two
 // kernels run one after the other.
 // ---------------------------------------------------------------------------
+// The fused pass's systems: packed fields, no Bloch partner.
 template <class T>
-struct HEArgs {
-    HArgs<T> h;
-    const T* eps[3];
-    T* Eout[3];
-    T* psi_e;
-    const T* src_amp;        // E_out[src] += *src_amp after the E update; null: none
-    int src_x, src_p, src_comp;
+struct HSysP {
+    Packed3<const T> E, Hin;
+    Packed3<T> Hout;
+    const T* psi_in;
+    T* psi_out;
 };
 
-template <class T, int HB>
+template <class T>
+struct ESysP {
+    Packed3<T> E;            // written: E_out
+    T* psi;
+};
+
+template <class T>
+struct HEArgs {
+    Geom<T> g;
+    HSysP<T> hs;
+    ESysP<T> es;
+    Packed3<T> S;
+    Packed3<const T> eps;
+    const T* src_amp;        // E_out[src] += *src_amp after the E update; null: none
+    int src_x, src_p, src_comp;
+    int coef_elems;
+};
+
+// NU: nonuniform grid (per-index 1/dx arrays); uniform grids keep those
+// pointers out of the kernel's scalar registers.
+// CT: the coefficient block (at most kCoefTab values) is copied to LDS
+// once and read there.
+constexpr int kCoefTab = 2048;
+
+template <class T, int HB, bool NU, bool CT, bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
 __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     __shared__ T lds[2][7][HB];
-    const Geom<T>& g = a.h.g;
-    const HSys<T>& hs = a.h.s[0];
+    __shared__ T ctab[CT ? kCoefTab : 1];
+    const Geom<T>& g = a.g;
+    const HSysP<T>& hs = a.hs;
+    const ESysP<T>& es = a.es;
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
     const int t = threadIdx.x;
@@ -962,30 +1153,66 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     const int x0 = ck * g.xchunk;
     const int x1 = min(x0 + g.xchunk, g.sx);
     const T sf = g.dt * (T)0.125;
-    const T iy = g.inv_e[1] ? g.inv_e[1][c.y] : g.inv_u;     // H step
-    const T iz = g.inv_e[2] ? g.inv_e[2][c.z] : g.inv_u;
-    const T jy = g.inv_h[1] ? g.inv_h[1][c.y] : g.inv_u;     // E step
-    const T jz = g.inv_h[2] ? g.inv_h[2][c.z] : g.inv_u;
+    const T iy = NU ? g.inv_e[1][c.y] : g.inv_u;     // H step
+    const T iz = NU ? g.inv_e[2][c.z] : g.inv_u;
+    const T jy = NU ? g.inv_h[1][c.y] : g.inv_u;     // E step
+    const T jz = NU ? g.inv_h[2][c.z] : g.inv_u;
     const int xs = (x0 == 0) ? g.sx - 1 : x0 - 1;
     const int n_it = (x1 - x0) + 1;
-    // the E half-step writes E_out and psi_E through an ESys (its H and
-    // partner fields are not used: H comes from this kernel's registers/LDS)
-    ESys<T> es;
-    for (int r = 0; r < 3; ++r) { es.E[r] = a.Eout[r]; es.H[r] = nullptr; es.P[r] = nullptr; es.sa[r] = es.sb[r] = (T)0; }
-    es.psi = a.psi_e;
     const int src_comp = (own && a.src_amp != nullptr && p == a.src_p) ? a.src_comp : -1;
+    if (CT) {
+        for (int i = t; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
+        lds_barrier();
+    }
+    const T* tab = CT ? ctab : nullptr;
     PsiHit hit[3];
     hit[1] = slab_hit(g, 1, c.y, g.sy);
     hit[2] = slab_hit(g, 2, c.z, g.sz);
+    // with the table in LDS: the y and z slab data per thread, worked out once
+    CpmlCols<T> cols;
+    if (CT) {
+        cols.y = psi_col<1>(g, c, hit[1]);
+        cols.z = psi_col<2>(g, c, hit[2]);
+    }
 
     T qx, qy, qz;                 // E^n at the current plane
-    HIn<T> cur;
     {
         const uint32_t ob = (uint32_t)xs * pbytes;
-        qx = ld(hs.E[0], ob + c.op);
-        qy = ld(hs.E[1], ob + c.op);
-        qz = ld(hs.E[2], ob + c.op);
+        qx = ldc<false>(hs.E, 0, ob + c.op);
+        qy = ldc<false>(hs.E, 1, ob + c.op);
+        qz = ldc<false>(hs.E, 2, ob + c.op);
     }
+    // Everything a plane iteration loads from memory: the H-step inputs,
+    // the psi pairs of both half-steps and eps.  With PF the loads of plane
+    // x+1 are issued before plane x is computed.
+    struct PlaneIn {
+        HIn<T> h;
+        PsiPair<T> qh, qe;    // (left unset where the cell is in no slab: only read where it is)
+        T e0, e1, e2;
+        PsiHit hx;
+    };
+    const auto fetch = [&](int xf, bool dryf, PlaneIn& in) {
+        const uint32_t of = (uint32_t)xf * pbytes + c.op;
+        PsiHit ht[3] = {slab_hit(g, 0, xf, g.sx), hit[1], hit[2]};
+        in.hx = ht[0];
+        in.e0 = in.e1 = in.e2 = (T)1;
+        CpmlCols<T> kc = cols;
+        kc.hx = ht[0];
+        if (valid) {
+            if (CT) psi_fetch_cols(g, hs.psi_in, xf, c, kc, in.qh);
+            else psi_fetch(g, hs.psi_in, xf, c, ht, in.qh);
+        }
+        if (own && !dryf) {
+            if (CT) psi_fetch_cols(g, (const T*)es.psi, xf, c, kc, in.qe);
+            else psi_fetch(g, (const T*)es.psi, xf, c, ht, in.qe);
+            in.e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
+            in.e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
+            in.e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
+        }
+        h_fetch<T, false>(g, hs, c, xf, pbytes, in.h);
+    };
+    PlaneIn cur;
+    if (PF) fetch(xs, true, cur);
     T oSyx_m = (T)0, oSzx_m = (T)0;   // oS products of the -x neighbour
     T hy_m = (T)0, hz_m = (T)0;       // H^{n+1/2} of the -x neighbour
     int x = xs;
@@ -993,31 +1220,32 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
         const int xn = (x == g.sx - 1) ? 0 : x + 1;
         const bool dry = it == 0;   // uniform
         const uint32_t oi = (uint32_t)x * pbytes + c.op;
-        // every load of the plane is issued here, before any of them is used:
-        // psi pairs, eps, then the H-step inputs
-        hit[0] = slab_hit(g, 0, x, g.sx);
-        PsiPair<T> qh{}, qe{};
+        PlaneIn nxt;
+        if (PF) {
+            if (it + 1 < n_it) fetch(xn, false, nxt);
+        } else {
+            fetch(x, dry, cur);
+        }
+        hit[0] = cur.hx;
+        cols.hx = cur.hx;
         FaceCoef<T> kh[3], ke[3];
-        T e0 = (T)1, e1 = (T)1, e2 = (T)1;
-        const T jx = g.inv_h[0] ? g.inv_h[0][x] : g.inv_u;
+        const T jx = NU ? g.inv_h[0][x] : g.inv_u;
         const bool src = src_comp >= 0 && x == a.src_x;
         const T amp = src ? *a.src_amp : (T)0;
-        cpml_coefs<T, false>(g, hit, kh);
-        cpml_coefs<T, true>(g, hit, ke);
-        if (valid) psi_fetch(g, hs.psi_in, x, c, hit, qh);
-        if (own && !dry) {
-            psi_fetch(g, (const T*)a.psi_e, x, c, hit, qe);
-            e0 = lds_<FDTD_NT_E>(a.eps[0], oi);
-            e1 = lds_<FDTD_NT_E>(a.eps[1], oi);
-            e2 = lds_<FDTD_NT_E>(a.eps[2], oi);
+        if (!CT) {
+            cpml_coefs<T, false>(g, hit, kh);
+            cpml_coefs<T, true>(g, hit, ke);
         }
-        h_fetch<T, false>(g, hs, c, x, pbytes, cur);
-        const T ix = g.inv_e[0] ? g.inv_e[0][x] : g.inv_u;
+        const auto kcpml = [&]() {
+            if constexpr (CT) return (const CpmlCols<T>*)&cols;
+            else return (const PsiHit*)hit;
+        }();
+        const T ix = NU ? g.inv_e[0][x] : g.inv_u;
         T oSxy = (T)0, oSxz = (T)0, oSyz = (T)0, oSyx = (T)0, oSzx = (T)0, oSzy = (T)0;
         T hx1 = (T)0, hy1 = (T)0, hz1 = (T)0;
         if (valid) {
-            const HCell<T> h = h_update<T, false, true>(g, hs, c, x, pbytes, qx, qy, qz, cur, ix, iy, iz,
-                                                        own && !dry, hit, kh, &qh);
+            const HCell<T> h = h_update<T, false, true>(g, hs, c, x, pbytes, qx, qy, qz, cur.h, ix, iy, iz,
+                                                        own && !dry, kcpml, kh, &cur.qh, tab);
             const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
             oSxy = h.aEyx * aHz;
             oSxz = -h.aEzx * aHy;
@@ -1041,9 +1269,9 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
             const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
             const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
             const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
-            sts_<FDTD_NT_H>(a.h.S[0], oi, (oSxy + oSxz + nxy + nxz) * sf);
-            sts_<FDTD_NT_H>(a.h.S[1], oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
-            sts_<FDTD_NT_H>(a.h.S[2], oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            stc<FDTD_NT_H>(a.S, 0, oi, (oSxy + oSxz + nxy + nxz) * sf);
+            stc<FDTD_NT_H>(a.S, 1, oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+            stc<FDTD_NT_H>(a.S, 2, oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
             EIn<T> v;
             v.hx = hx1; v.hy = hy1; v.hz = hz1;
             v.hx_ym = lds[b][4][ty];
@@ -1051,16 +1279,17 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
             v.hx_zm = lds[b][4][tz];
             v.hy_zm = lds[b][5][tz];
             v.ex = qx; v.ey = qy; v.ez = qz;
-            e_update<T, true>(g, es, c, x, oi, v, hy_m, hz_m, e0, e1, e2, jx, jy, jz, src ? src_comp : -1, amp,
-                              hit, ke, &qe);
+            e_update<T, true>(g, es, c, x, oi, v, hy_m, hz_m, cur.e0, cur.e1, cur.e2, jx, jy, jz,
+                              src ? src_comp : -1, amp, kcpml, ke, &cur.qe, tab);
         }
         oSyx_m = oSyx;
         oSzx_m = oSzx;
         hy_m = hy1;
         hz_m = hz1;
-        qx = cur.exn;
-        qy = cur.eyn;
-        qz = cur.ezn;
+        qx = cur.h.exn;
+        qy = cur.h.eyn;
+        qz = cur.h.ezn;
+        if (PF) cur = nxt;
         x = xn;
     }
 }
@@ -1123,6 +1352,7 @@ struct Plan {
     int sx, sy, sz, plane;
     int64_t N;
     Launch le, lh;
+    Launch lhe;          // fused H+E pass (mode 2 when available, else mode -1)
     int64_t psi_elems, coef_elems;
     struct F { int lo, hi, psi_off, face_elems, coef_off; } face[FDTD_MAX_FACES];
 };
@@ -1304,18 +1534,26 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
 }
 
 bool he_supported(const Plan& P) {
-    return P.d.poynting == FDTD_POYNTING_FUSED && P.d.bloch_mask == 0 && P.lh.mode == 2;
+    return P.d.poynting == FDTD_POYNTING_FUSED && P.d.bloch_mask == 0 && P.lhe.mode == 2;
 }
 
 template <class T>
 int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index) {
     HEArgs<T> a;
     memset(&a, 0, sizeof(a));
-    fill_h<T>(P, P.lh, f, a.h);
+    fill_geom<T>(P, P.lhe, f, a.g);
+    const uint32_t nb = (uint32_t)(P.N * (int64_t)sizeof(T));
+    a.hs.E = {(const T*)f->E, nb};
+    a.hs.Hin = {(const T*)f->H, nb};
+    a.hs.Hout = {(T*)f->H_out, nb};
+    a.hs.psi_in = (const T*)f->psi_h;
+    a.hs.psi_out = (T*)f->psi_h_out;
+    a.es.E = {(T*)f->E_out, nb};
+    a.es.psi = (T*)f->psi_e;
+    a.S = {(T*)f->S, nb};
+    a.eps = {(const T*)f->eps, nb};
     const int64_t N = P.N;
-    comp3c<T>(f->eps, N, a.eps);
-    comp3<T>(f->E_out, N, a.Eout);
-    a.psi_e = (T*)f->psi_e;
+    a.coef_elems = (int)P.coef_elems;
     a.src_amp = src_amp;
     if (src_amp) {
         const int64_t comp = src_index / N;
@@ -1324,11 +1562,20 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
         a.src_x = (int)(i / P.plane);
         a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
     }
-    const Launch& L = P.lh;
+    const Launch& L = P.lhe;
     const dim3 grid(L.pblocks * L.nchunk);
-    if (L.block == FDTD_HB_SMALL && FDTD_HB_SMALL != 1024)
-        hipLaunchKernelGGL((he_march<T, FDTD_HB_SMALL>), grid, dim3(FDTD_HB_SMALL), 0, st, a);
-    else hipLaunchKernelGGL((he_march<T, 1024>), grid, dim3(1024), 0, st, a);
+    const bool nu = P.d.nonuniform != 0;
+    const bool ct = FDTD_HE_CTAB && P.coef_elems <= kCoefTab;
+#define FDTD_LAUNCH_HE(HB_)                                                                        \
+    do {                                                                                           \
+        if (nu && ct) hipLaunchKernelGGL((he_march<T, HB_, true, true>), grid, dim3(HB_), 0, st, a);    \
+        else if (nu) hipLaunchKernelGGL((he_march<T, HB_, true, false>), grid, dim3(HB_), 0, st, a);    \
+        else if (ct) hipLaunchKernelGGL((he_march<T, HB_, false, true>), grid, dim3(HB_), 0, st, a);    \
+        else hipLaunchKernelGGL((he_march<T, HB_, false, false>), grid, dim3(HB_), 0, st, a);           \
+    } while (0)
+    if (L.block == FDTD_HE_HB_SMALL && FDTD_HE_HB_SMALL != 1024) FDTD_LAUNCH_HE(FDTD_HE_HB_SMALL);
+    else FDTD_LAUNCH_HE(1024);
+#undef FDTD_LAUNCH_HE
     return check_launch("fdtd_update_he");
 }
 
@@ -1438,6 +1685,12 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     } else {
         P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, d.poynting == FDTD_POYNTING_TWO_PASS ? 1 : 0,
                             kTargetBlocks);
+    }
+    // fused H+E pass: the same tile shape (whole rows plus one halo row)
+    P->lhe.mode = -1;
+    if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= 341 && 3 * N * esz <= (int64_t)UINT32_MAX) {
+        const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
+        P->lhe = make_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, 2, FDTD_HE_TARGET);
     }
     *plan_out = P;
     return 0;
