@@ -1,5 +1,6 @@
-// HBM probe: plain copy kernels of known size and an E-step-shaped stream
-// (9 arrays read, 3 written, no neighbours).  Used to measure the practical
+// HBM probe: plain copy kernels of known size, an E-step-shaped stream
+// (9 arrays read, 3 written, no neighbours) and a stream shaped like the
+// fused H+E pass (9 read, 9 written: E, H, eps in; E, H, S out).  Used to measure the practical
 // bandwidth ceiling of the GPU and to calibrate the rocprofv3 FETCH_SIZE /
 // WRITE_SIZE counters (bytes per KiB reported) against a known byte count.
 //   copy_probe [floats per buffer, default 2^28] [repetitions, default 20]
@@ -26,6 +27,22 @@ __global__ void e_like(const float* __restrict__ h, const float* __restrict__ ep
         float a0 = h[i], a1 = h[i + n], a2 = h[i + 2 * n];
         float q0 = eps[i], q1 = eps[i + n], q2 = eps[i + 2 * n];
         e[i] += a0 / q0; e[i + n] += a1 / q1; e[i + 2 * n] += a2 / q2;
+    }
+}
+
+// 9 streams read, 9 written (the fused H+E pass's byte pattern without
+// neighbours or psi)
+__global__ void he_like(const float* __restrict__ e, const float* __restrict__ h, const float* __restrict__ eps,
+                        float* __restrict__ eo, float* __restrict__ ho, float* __restrict__ so, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t s = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) {
+        float e0 = e[i], e1 = e[i + n], e2 = e[i + 2 * n];
+        float h0 = h[i], h1 = h[i + n], h2 = h[i + 2 * n];
+        float q0 = eps[i], q1 = eps[i + n], q2 = eps[i + 2 * n];
+        ho[i] = h0 - e1; ho[i + n] = h1 - e2; ho[i + 2 * n] = h2 - e0;
+        so[i] = e1 * h2; so[i + n] = e2 * h0; so[i + 2 * n] = e0 * h1;
+        eo[i] = e0 + h0 / q0; eo[i + n] = e1 + h1 / q1; eo[i + 2 * n] = e2 + h2 / q2;
     }
 }
 
@@ -58,6 +75,23 @@ int main(int argc, char** argv) {
         for (int r = 0; r < reps; r++) hipLaunchKernelGGL(e_like, dim3(g), dim3(256), 0, 0, a, b, c, m);
         CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1)); CK(hipEventElapsedTime(&ms, e0, e1));
         printf("e_like grid %6d: %.1f GB/s\n", g, 48.0 * m * reps / ms / 1e6);
+    }
+    // he_like on six buffers of 3m floats (m = 16 Mi cells: 1.2 GB moved per launch)
+    {
+        const size_t m = (size_t)16 << 20;
+        float* f[6];
+        for (int k = 0; k < 6; ++k) { CK(hipMalloc(&f[k], m * 12)); CK(hipMemset(f[k], 0, m * 12)); }
+        CK(hipMemset(f[2], 0x3f, m * 12));   // eps ~ 0.75, nonzero
+        for (int g : grids) {
+            float ms;
+            hipLaunchKernelGGL(he_like, dim3(g), dim3(256), 0, 0, f[0], f[1], f[2], f[3], f[4], f[5], m);
+            CK(hipEventRecord(e0));
+            for (int r = 0; r < reps; r++)
+                hipLaunchKernelGGL(he_like, dim3(g), dim3(256), 0, 0, f[0], f[1], f[2], f[3], f[4], f[5], m);
+            CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1)); CK(hipEventElapsedTime(&ms, e0, e1));
+            printf("he_like grid %6d: %.1f GB/s\n", g, 72.0 * m * reps / ms / 1e6);
+        }
+        for (int k = 0; k < 6; ++k) CK(hipFree(f[k]));
     }
     CK(hipDeviceSynchronize());
     printf("done\n");

@@ -92,7 +92,7 @@ namespace {
 #define FDTD_HE_PF 0                 // fused H+E, FP32: issue the loads of plane x+1 before computing plane x
 #endif
 #ifndef FDTD_HE_TARGET
-#define FDTD_HE_TARGET 1024          // fused H+E workgroups per launch
+#define FDTD_HE_TARGET 256           // fused H+E workgroups per launch (at most; 256 CUs)
 #endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
@@ -1689,8 +1689,14 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
     if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= 341 && 3 * N * esz <= (int64_t)UINT32_MAX) {
+        // one workgroup per CU at a time (1024 threads, <= 128 VGPRs): as many
+        // chunks per tile column as fit FDTD_HE_TARGET workgroups without
+        // starting a partial extra round, so fewer dry planes
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
-        P->lhe = make_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, 2, FDTD_HE_TARGET);
+        const int tile = ((hb - P->sz) / P->sz) * P->sz;
+        const int pblocks = (P->plane + tile - 1) / tile;
+        const int want = FDTD_HE_TARGET / pblocks;
+        P->lhe = make_launch(P->plane, P->sx, hb, tile, 2, (want > 0 ? want : 1) * pblocks);
     }
     *plan_out = P;
     return 0;
