@@ -85,6 +85,9 @@ namespace {
 #ifndef FDTD_HE_HB_SMALL
 #define FDTD_HE_HB_SMALL 1024        // fused H+E workgroup for rows of <= 128 cells
 #endif
+#ifndef FDTD_EH_CTAB
+#define FDTD_EH_CTAB 1               // E and H kernels: CPML psi loaded with the fields, coefficients from LDS
+#endif
 #ifndef FDTD_HE_CTAB
 #define FDTD_HE_CTAB 1               // fused H+E: CPML coefficients read from an LDS copy
 #endif
@@ -96,6 +99,7 @@ namespace {
 #endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
+constexpr int kCoefTab = 2048;       // largest coefficient block the kernels copy to LDS
 constexpr int kTargetBlocks = FDTD_TARGET_BLOCKS;
 
 thread_local std::string g_last_error;
@@ -172,6 +176,7 @@ struct EArgs {
     // order (shell_index); null: none
     const float* ycorr[3];
     int ycT;
+    int coef_elems;
 };
 
 template <class T>
@@ -180,6 +185,7 @@ struct HArgs {
     HSys<T> s[2];
     T* S[3];
     T* oS[6];            // two-pass mode: oSxy, oSyz, oSzx, oSxz, oSyx, oSzy
+    int coef_elems;
 };
 
 // ---------------------------------------------------------------------------
@@ -745,9 +751,16 @@ __device__ __forceinline__ ShellCol shell_col(int y, int z, int T, int sx, int s
 
 // SHELL: add the pending learned-shell correction (EArgs::ycorr) to E
 // before the update (real fields only).
-template <class T, bool PAIR, bool SEAM, bool SHELL = false>
+// CT: the CPML coefficient block is copied to LDS and each plane's psi
+// pairs are loaded with its fields (per-thread y/z slab columns).
+template <class T, bool PAIR, bool SEAM, bool SHELL = false, bool CT = false>
 __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     const Geom<T>& g = a.g;
+    __shared__ T ctab[CT ? kCoefTab : 1];
+    if (CT) {
+        for (int i = threadIdx.x; i < a.coef_elems; i += kBlock) ctab[i] = g.coef[i];
+        lds_barrier();
+    }
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
     const int p = pb * kBlock + threadIdx.x;
@@ -774,6 +787,11 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     }
     const T iy = g.inv_h[1] ? g.inv_h[1][c.y] : g.inv_u;
     const T iz = g.inv_h[2] ? g.inv_h[2][c.z] : g.inv_u;
+    CpmlCols<T> cols;
+    if (CT) {
+        cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
+        cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
+    }
     ShellCol sc{};
     if (SHELL) sc = shell_col(c.y, c.z, a.ycT, g.sx, g.sy, g.sz);
 
@@ -789,6 +807,12 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
     }
     for (int x = x0; x < x1; ++x) {
         const uint32_t ob = (uint32_t)x * pbytes;
+        PsiPair<T> pq[NS];
+        if (CT) {
+            cols.hx = slab_hit(g, 0, x, g.sx);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) psi_fetch_cols(g, (const T*)a.s[k].psi, x, c, cols, pq[k]);
+        }
         // pending shell correction, loaded ahead of the plane's fields so
         // the loads overlap
         bool in_shell = false;
@@ -834,8 +858,13 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            e_update<T>(g, a.s[k], c, x, ob + c.op, cur[k], hy_m[k], hz_m[k], e0, e1, e2, ix, iy, iz,
-                        k == 0 ? src_comp : -1, amp);
+            if constexpr (CT)
+                e_update<T, true>(g, a.s[k], c, x, ob + c.op, cur[k], hy_m[k], hz_m[k], e0, e1, e2, ix, iy, iz,
+                                  k == 0 ? src_comp : -1, amp, (const CpmlCols<T>*)&cols, nullptr, &pq[k],
+                                  (const T*)ctab);
+            else
+                e_update<T>(g, a.s[k], c, x, ob + c.op, cur[k], hy_m[k], hz_m[k], e0, e1, e2, ix, iy, iz,
+                            k == 0 ? src_comp : -1, amp);
             hy_m[k] = cur[k].hy;
             hz_m[k] = cur[k].hz;
             if (FDTD_PREFETCH) cur[k] = nxt[k];
@@ -961,12 +990,19 @@ __device__ __forceinline__ HCell<T> h_eval(const Geom<T>& g, const HSys<T>& s, i
 //
 // Fused modes start the march one plane early: that first iteration only
 // evaluates the oS products of plane x0-1 (the -x neighbours of plane x0).
-template <class T, bool PAIR, bool SEAM, int MODE, int HB>
+// CT (modes 0-2): the CPML coefficient block is copied to LDS and each
+// plane's psi pairs are loaded with its fields (per-thread y/z columns).
+template <class T, bool PAIR, bool SEAM, int MODE, int HB, bool CT = false>
 __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
     constexpr bool LDSX = (MODE == 2);
     constexpr bool FUSED = (MODE == 2 || MODE == 3);
     __shared__ T lds[LDSX ? 2 : 1][4][LDSX ? HB : 1];
+    __shared__ T ctab[CT ? kCoefTab : 1];
     const Geom<T>& g = a.g;
+    if (CT) {
+        for (int i = threadIdx.x; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
+        lds_barrier();
+    }
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
     const int t = threadIdx.x;
@@ -986,6 +1022,11 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
     constexpr int NS = PAIR ? 2 : 1;
     const int xs = FUSED ? (x0 == 0 ? g.sx - 1 : x0 - 1) : x0;
     const int n_it = (x1 - x0) + (FUSED ? 1 : 0);
+    CpmlCols<T> cols;
+    if (CT) {
+        cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
+        cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
+    }
 
     // register queue: E (and F) at the current plane
     T qx[NS], qy[NS], qz[NS];
@@ -1010,14 +1051,28 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) h_fetch<T, SEAM>(g, a.s[k], c, xn, pbytes, nxt[k]);
         }
+        PsiPair<T> pq[NS];
+        if (CT) {
+            cols.hx = slab_hit(g, 0, x, g.sx);
+            if (valid) {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) psi_fetch_cols(g, a.s[k].psi_in, x, c, cols, pq[k]);
+            }
+        }
         const bool dry = FUSED && it == 0;   // uniform
         const T ix = g.inv_e[0] ? g.inv_e[0][x] : g.inv_u;
         const uint32_t oi = (uint32_t)x * pbytes + c.op;
         // ---- system 0 (E -> H), with Poynting
         T oSxy = (T)0, oSxz = (T)0, oSyz = (T)0, oSyx = (T)0, oSzx = (T)0, oSzy = (T)0;
         if (valid) {
-            const HCell<T> h = h_update<T, SEAM>(g, a.s[0], c, x, pbytes, qx[0], qy[0], qz[0], cur[0], ix, iy,
-                                                 iz, own && !dry);
+            HCell<T> h;
+            if constexpr (CT)
+                h = h_update<T, SEAM, true>(g, a.s[0], c, x, pbytes, qx[0], qy[0], qz[0], cur[0], ix, iy, iz,
+                                            own && !dry, (const CpmlCols<T>*)&cols, nullptr, &pq[0],
+                                            (const T*)ctab);
+            else
+                h = h_update<T, SEAM>(g, a.s[0], c, x, pbytes, qx[0], qy[0], qz[0], cur[0], ix, iy, iz,
+                                      own && !dry);
             if (MODE != 0) {
                 const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
                 oSxy = h.aEyx * aHz;
@@ -1069,7 +1124,12 @@ __global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
         oSzx_m = oSzx;
         // ---- system 1 (F -> G), no Poynting
         if (PAIR && own && !dry) {
-            (void)h_update<T, SEAM>(g, a.s[1], c, x, pbytes, qx[1], qy[1], qz[1], cur[1], ix, iy, iz, true);
+            if constexpr (CT)
+                (void)h_update<T, SEAM, true>(g, a.s[1], c, x, pbytes, qx[1], qy[1], qz[1], cur[1], ix, iy, iz,
+                                              true, (const CpmlCols<T>*)&cols, nullptr, &pq[NS - 1],
+                                              (const T*)ctab);
+            else
+                (void)h_update<T, SEAM>(g, a.s[1], c, x, pbytes, qx[1], qy[1], qz[1], cur[1], ix, iy, iz, true);
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
@@ -1131,7 +1191,6 @@ struct HEArgs {
 // pointers out of the kernel's scalar registers.
 // CT: the coefficient block (at most kCoefTab values) is copied to LDS
 // once and read there.
-constexpr int kCoefTab = 2048;
 
 template <class T, int HB, bool NU, bool CT, bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
 __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
@@ -1469,23 +1528,32 @@ int launch_e(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_a
     const dim3 grid(P.le.pblocks * P.le.nchunk);
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
     if (ycorr && pair) return fail(3, "the shell correction is for real fields (no Bloch boundaries)");
-    if (ycorr) hipLaunchKernelGGL((e_march<T, false, false, true>), grid, dim3(kBlock), 0, st, a);
-    else if (!pair) hipLaunchKernelGGL((e_march<T, false, false>), grid, dim3(kBlock), 0, st, a);
-    else if (!seam) hipLaunchKernelGGL((e_march<T, true, false>), grid, dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL((e_march<T, true, true>), grid, dim3(kBlock), 0, st, a);
+    a.coef_elems = (int)P.coef_elems;
+    const bool ct = FDTD_EH_CTAB && P.coef_elems <= kCoefTab;
+#define FDTD_LAUNCH_E(PA, SE, SH)                                                                            \
+    do {                                                                                                     \
+        if (ct) hipLaunchKernelGGL((e_march<T, PA, SE, SH, true>), grid, dim3(kBlock), 0, st, a);            \
+        else hipLaunchKernelGGL((e_march<T, PA, SE, SH, false>), grid, dim3(kBlock), 0, st, a);             \
+    } while (0)
+    if (ycorr) FDTD_LAUNCH_E(false, false, true);
+    else if (!pair) FDTD_LAUNCH_E(false, false, false);
+    else if (!seam) FDTD_LAUNCH_E(true, false, false);
+    else FDTD_LAUNCH_E(true, true, false);
+#undef FDTD_LAUNCH_E
     return check_launch("fdtd_update_e");
 }
 
-template <class T, bool PAIR, bool SEAM>
+template <class T, bool PAIR, bool SEAM, bool CT>
 void launch_h_mode(const Launch& L, hipStream_t st, const HArgs<T>& a) {
     const dim3 grid(L.pblocks * L.nchunk);
     switch (L.mode) {
-        case 0: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 0, kBlock>), grid, dim3(kBlock), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 1, kBlock>), grid, dim3(kBlock), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 0, kBlock, CT>), grid, dim3(kBlock), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 1, kBlock, CT>), grid, dim3(kBlock), 0, st, a); break;
         case 2:
             if (L.block == FDTD_HB_SMALL && FDTD_HB_SMALL != 1024)
-                hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, FDTD_HB_SMALL>), grid, dim3(FDTD_HB_SMALL), 0, st, a);
-            else hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 1024>), grid, dim3(1024), 0, st, a);
+                hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, FDTD_HB_SMALL, CT>), grid, dim3(FDTD_HB_SMALL), 0, st,
+                                   a);
+            else hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 2, 1024, CT>), grid, dim3(1024), 0, st, a);
             break;
         default: hipLaunchKernelGGL((h_march<T, PAIR, SEAM, 3, kBlock>), grid, dim3(kBlock), 0, st, a); break;
     }
@@ -1527,9 +1595,13 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     fill_h<T>(P, P.lh, f, a);
     const bool pair = P.d.bloch_mask != 0;
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
-    if (!pair) launch_h_mode<T, false, false>(P.lh, st, a);
-    else if (!seam) launch_h_mode<T, true, false>(P.lh, st, a);
-    else launch_h_mode<T, true, true>(P.lh, st, a);
+    a.coef_elems = (int)P.coef_elems;
+    // (the paired Bloch systems stay on the plain CPML path: their register
+    // budget has no room for the prefetched psi pairs)
+    const bool ct = FDTD_EH_CTAB && P.coef_elems <= kCoefTab && P.lh.mode != 3 && !pair;
+    if (!pair) ct ? launch_h_mode<T, false, false, true>(P.lh, st, a) : launch_h_mode<T, false, false, false>(P.lh, st, a);
+    else if (!seam) launch_h_mode<T, true, false, false>(P.lh, st, a);
+    else launch_h_mode<T, true, true, false>(P.lh, st, a);
     return check_launch("fdtd_update_h");
 }
 

@@ -19,6 +19,13 @@
 // live per box on the box grown by one cell, in a caller-provided FP32
 // scratch [cell][16].
 
+#ifndef FDTD_SHELL_TILE_A1
+#define FDTD_SHELL_TILE_A1 8         // shell_mfma tile: output rows along a1 (even; 8 or 10)
+#endif
+#ifndef FDTD_SHELL_WAVES
+#define FDTD_SHELL_WAVES 4           // shell_mfma: minimum waves per SIMD (4 workgroups of 4 waves per CU)
+#endif
+
 namespace {
 
 constexpr int kHid = 16;
@@ -393,7 +400,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxT = 4;
-constexpr int kTileA1 = 8, kTileA2 = 14;                 // output cells along a1, a2
+constexpr int kTileA1 = FDTD_SHELL_TILE_A1, kTileA2 = 14;   // output cells along a1, a2
 constexpr int kRow = 16;                                  // MFMA columns = hidden a2 extent
 constexpr int kInA0 = (kTileA1 + 4) * (kTileA2 + 4) + 1;  // input a0 stride (padded: no bank conflicts)
 constexpr int kInMax = (kMaxT + 4) * kInA0;
@@ -476,7 +483,7 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const uint4& b, const f32
 constexpr int kInIter = (kInMax + 255) / 256;           // input cells per thread (upper bound)
 
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FDTD_SHELL_WAVES))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf) {
     // the input tile [cell][8 x BF16], then (after layer 1) the hidden tile
     // [2 planes][cell][8 x BF16] in the same words
     __shared__ uint4 s_buf[kTileLds];
@@ -537,10 +544,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kDepth * kFrag2) * 64 + lane;
     __syncthreads();
     // 2. layer 1: wave w takes a0 rows h0 = w/2, w/2 + 2, w/2 + 4 (< T+2) and
-    // along a1 the five rows h1 = 5 (w % 2) + [0, 5); B fragment (h0 + d0,
-    // y) feeds rows y - d1.  Results stay in registers (BF16, channels
-    // 4g .. 4g+3 of cell n) until every wave is done with the input tile.
-    constexpr int kR1 = 5, kH0 = 3;
+    // along a1 the kR1 = (kTileA1 + 2) / 2 rows h1 = kR1 (w % 2) + [0, kR1);
+    // B fragment (h0 + d0, y) feeds rows y - d1.  Results stay in registers
+    // (BF16, channels 4g .. 4g+3 of cell n) until every wave is done with
+    // the input tile.
+    constexpr int kR1 = (kTileA1 + 2) / 2, kH0 = 3;
+    static_assert(2 * kR1 == kTileA1 + 2, "two waves split the hidden rows along a1");
     uint32_t hreg[kH0][kR1][2];
     {
         bf16x8 W1[kFrag1];
