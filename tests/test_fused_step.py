@@ -56,6 +56,10 @@ def _assert_same(a, b, what):
     ((5, 37, 3), 2, 6, numpy.float32, False),        # short rows, many per tile
     ((33, 8, 17), 3, 9, numpy.float64, True),
     ((5, 6, 160), 2, 4, numpy.float64, False),       # FP64, 1024-thread tiles
+    ((6, 7, 5), 4, 5, numpy.float32, False),         # slabs of both faces overlap on every axis
+    ((2, 9, 10), 1, 4, numpy.float32, False),        # two planes: every chunk is one plane
+    ((3, 4, 341), 2, 3, numpy.float32, False),       # longest rows the fused pass takes
+    ((5, 6, 1), 1, 6, numpy.float32, True),          # one-cell rows
 ])
 def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
     print("case fused step_n", shape, T, n, dtype.__name__, nonuniform, flush=True)
@@ -98,13 +102,18 @@ def test_update_he_equals_h_then_e(dtype):
         _assert_same(_state(a), _state(b), "update_HE amp %g" % amp)
 
 
-def test_fused_step_n_matches_oracle_1024_tiles():
-    """Rows of 150 cells (1024-thread tiles) against the oracle, with a source."""
-    shape = (14, 11, 150)
-    eps, pmls, init, _ = _case(shape, 4, seed=9)
+@pytest.mark.parametrize("shape, T", [
+    ((14, 11, 150), 4),      # rows of 150 cells (1024-thread tiles)
+    ((6, 7, 5), 4),          # overlapping slabs on every axis
+    ((23, 30, 73), 8),       # README row length, CPML-8
+])
+def test_fused_step_n_matches_oracle(shape, T):
+    """step_n through the fused pass against the oracle, with a source."""
+    print("case fused vs oracle", shape, T, flush=True)
+    eps, pmls, init, _ = _case(shape, T, seed=9, pec=True)
     sim, orc = build_pair(eps, pmls, init=init)
     assert sim._E_out is not None
-    idx = 2 * sim.N + 7 * shape[1] * shape[2] + 5 * shape[2] + 75
+    idx = 2 * sim.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
     wave = numpy.cos(0.2 * numpy.arange(21))
     sim.step_n(21, idx, wave.astype(numpy.float32)).wait()
     step_oracle(orc, 21, idx, wave)
