@@ -65,7 +65,7 @@ namespace {
 #define FDTD_XCD 1                   // XCD-contiguous workgroup order
 #endif
 #ifndef FDTD_HB_SMALL
-#define FDTD_HB_SMALL 512            // fused-S H workgroup for rows of <= 128 cells
+#define FDTD_HB_SMALL 1024           // fused-S H workgroup for rows of <= 128 cells
 #endif
 #ifndef FDTD_E_WAVES
 #define FDTD_E_WAVES 1               // __launch_bounds__ min waves per SIMD, E kernel
@@ -74,7 +74,8 @@ namespace {
 #define FDTD_PREFETCH_H FDTD_PREFETCH   // same for the H kernel
 #endif
 #ifndef FDTD_H_TARGET
-#define FDTD_H_TARGET 2048           // fused-S H workgroups per launch (measured: 2048 beats 1024-8192)
+#define FDTD_H_TARGET 256            // fused-S H workgroups per launch, at most (one round of 1024-thread
+                                     // workgroups; measured 135 vs 145 us for 512 threads x 2048)
 #endif
 #ifndef FDTD_H_WAVES
 #define FDTD_H_WAVES 6               // same, H kernel
@@ -993,7 +994,7 @@ __device__ __forceinline__ HCell<T> h_eval(const Geom<T>& g, const HSys<T>& s, i
 // CT (modes 0-2): the CPML coefficient block is copied to LDS and each
 // plane's psi pairs are loaded with its fields (per-thread y/z columns).
 template <class T, bool PAIR, bool SEAM, int MODE, int HB, bool CT = false>
-__global__ __launch_bounds__(HB, FDTD_H_WAVES) void h_march(HArgs<T> a) {
+__global__ __launch_bounds__(HB, HB >= 1024 ? 4 : FDTD_H_WAVES) void h_march(HArgs<T> a) {
     constexpr bool LDSX = (MODE == 2);
     constexpr bool FUSED = (MODE == 2 || MODE == 3);
     __shared__ T lds[LDSX ? 2 : 1][4][LDSX ? HB : 1];
@@ -1743,17 +1744,19 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     P->coef_elems = coef_off;
     P->le = make_launch(P->plane, P->sx, kBlock, kBlock, 0, kTargetBlocks);
     if (d.poynting == FDTD_POYNTING_FUSED) {
-        // LDS exchange: 512 threads for rows up to 128 cells, 1024 up to
-        // 341, else per-cell re-evaluation of the in-plane neighbours.
-        // whole rows per workgroup plus one halo row
-        if (P->sz <= 128)
-            P->lh = make_launch(P->plane, P->sx, FDTD_HB_SMALL, ((FDTD_HB_SMALL - P->sz) / P->sz) * P->sz, 2,
-                                FDTD_H_TARGET ? FDTD_H_TARGET : kTargetBlocks * kBlock / FDTD_HB_SMALL);
-        else if (P->sz <= 341)
-            P->lh = make_launch(P->plane, P->sx, 1024, ((1024 - P->sz) / P->sz) * P->sz, 2,
-                                FDTD_H_TARGET ? FDTD_H_TARGET : kTargetBlocks / 4);
-        else
+        // LDS exchange for rows of up to 341 cells: whole rows per workgroup
+        // plus one halo row, at most FDTD_H_TARGET workgroups (as many x
+        // chunks per tile column as fit); longer rows: per-cell
+        // re-evaluation of the in-plane neighbours
+        if (P->sz <= 341) {
+            const int hb = P->sz <= 128 ? FDTD_HB_SMALL : 1024;
+            const int tile = ((hb - P->sz) / P->sz) * P->sz;
+            const int pblocks = (P->plane + tile - 1) / tile;
+            const int want = FDTD_H_TARGET / pblocks;
+            P->lh = make_launch(P->plane, P->sx, hb, tile, 2, (want > 0 ? want : 1) * pblocks);
+        } else {
             P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, 3, kTargetBlocks);
+        }
     } else {
         P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, d.poynting == FDTD_POYNTING_TWO_PASS ? 1 : 0,
                             kTargetBlocks);
