@@ -1186,6 +1186,10 @@ struct HEArgs {
     const T* src_amp;        // E_out[src] += *src_amp after the E update; null: none
     int src_x, src_p, src_comp;
     int coef_elems;
+    // learned shell (SHELL): the correction added to E^n before its E update
+    // on the cells within ycT of a face, in the shell's box order
+    Packed3<const float> ycorr;
+    int ycT;
 };
 
 // NU: nonuniform grid (per-index 1/dx arrays); uniform grids keep those
@@ -1193,7 +1197,9 @@ struct HEArgs {
 // CT: the coefficient block (at most kCoefTab values) is copied to LDS
 // once and read there.
 
-template <class T, int HB, bool NU, bool CT, bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
+// SHELL: add the pending learned-shell correction to E^n before the E
+// update (the H update and S use E^n as it is).
+template <class T, int HB, bool NU, bool CT, bool SHELL, bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
 __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     __shared__ T lds[2][7][HB];
     __shared__ T ctab[CT ? kCoefTab : 1];
@@ -1245,11 +1251,15 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     // Everything a plane iteration loads from memory: the H-step inputs,
     // the psi pairs of both half-steps and eps.  With PF the loads of plane
     // x+1 are issued before plane x is computed.
+    ShellCol sc{};
+    if (SHELL) sc = shell_col(c.y, c.z, a.ycT, g.sx, g.sy, g.sz);
     struct PlaneIn {
         HIn<T> h;
         PsiPair<T> qh, qe;    // (left unset where the cell is in no slab: only read where it is)
         T e0, e1, e2;
         PsiHit hx;
+        float y0, y1, y2;     // pending shell correction (SHELL)
+        bool ins;
     };
     const auto fetch = [&](int xf, bool dryf, PlaneIn& in) {
         const uint32_t of = (uint32_t)xf * pbytes + c.op;
@@ -1268,6 +1278,19 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
             in.e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
             in.e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
             in.e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
+        }
+        in.ins = false;
+        if (SHELL && own && !dryf) {
+            const bool xlo = xf < a.ycT, xhi = xf >= g.sx - a.ycT;
+            in.ins = xlo || xhi || sc.in_yz;
+            if (in.ins) {
+                const uint32_t i = xlo   ? sc.xf + (uint32_t)xf * g.plane
+                                   : xhi ? (uint32_t)a.ycT * g.plane + sc.xf + (uint32_t)(xf - (g.sx - a.ycT)) * g.plane
+                                         : sc.base + (uint32_t)(xf - a.ycT) * sc.step;
+                in.y0 = ldc<false>(a.ycorr, 0, i * 4u);
+                in.y1 = ldc<false>(a.ycorr, 1, i * 4u);
+                in.y2 = ldc<false>(a.ycorr, 2, i * 4u);
+            }
         }
         h_fetch<T, false>(g, hs, c, xf, pbytes, in.h);
     };
@@ -1339,6 +1362,11 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
             v.hx_zm = lds[b][4][tz];
             v.hy_zm = lds[b][5][tz];
             v.ex = qx; v.ey = qy; v.ez = qz;
+            if (SHELL && cur.ins) {
+                v.ex = v.ex + (T)cur.y0;
+                v.ey = v.ey + (T)cur.y1;
+                v.ez = v.ez + (T)cur.y2;
+            }
             e_update<T, true>(g, es, c, x, oi, v, hy_m, hz_m, cur.e0, cur.e1, cur.e2, jx, jy, jz,
                               src ? src_comp : -1, amp, kcpml, ke, &cur.qe, tab);
         }
@@ -1611,9 +1639,14 @@ bool he_supported(const Plan& P) {
 }
 
 template <class T>
-int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index) {
+int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index,
+              const float* ycorr = nullptr, int ycT = 0, int64_t ycells = 0) {
     HEArgs<T> a;
     memset(&a, 0, sizeof(a));
+    if (ycorr) {
+        a.ycorr = {ycorr, (uint32_t)(ycells * 4)};
+        a.ycT = ycT;
+    }
     fill_geom<T>(P, P.lhe, f, a.g);
     const uint32_t nb = (uint32_t)(P.N * (int64_t)sizeof(T));
     a.hs.E = {(const T*)f->E, nb};
@@ -1639,16 +1672,22 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
     const dim3 grid(L.pblocks * L.nchunk);
     const bool nu = P.d.nonuniform != 0;
     const bool ct = FDTD_HE_CTAB && P.coef_elems <= kCoefTab;
-#define FDTD_LAUNCH_HE(HB_)                                                                        \
-    do {                                                                                           \
-        if (nu && ct) hipLaunchKernelGGL((he_march<T, HB_, true, true>), grid, dim3(HB_), 0, st, a);    \
-        else if (nu) hipLaunchKernelGGL((he_march<T, HB_, true, false>), grid, dim3(HB_), 0, st, a);    \
-        else if (ct) hipLaunchKernelGGL((he_march<T, HB_, false, true>), grid, dim3(HB_), 0, st, a);    \
-        else hipLaunchKernelGGL((he_march<T, HB_, false, false>), grid, dim3(HB_), 0, st, a);           \
+#define FDTD_LAUNCH_HE2(HB_, SH_)                                                                     \
+    do {                                                                                              \
+        if (nu && ct) hipLaunchKernelGGL((he_march<T, HB_, true, true, SH_>), grid, dim3(HB_), 0, st, a);    \
+        else if (nu) hipLaunchKernelGGL((he_march<T, HB_, true, false, SH_>), grid, dim3(HB_), 0, st, a);    \
+        else if (ct) hipLaunchKernelGGL((he_march<T, HB_, false, true, SH_>), grid, dim3(HB_), 0, st, a);    \
+        else hipLaunchKernelGGL((he_march<T, HB_, false, false, SH_>), grid, dim3(HB_), 0, st, a);           \
+    } while (0)
+#define FDTD_LAUNCH_HE(HB_)                      \
+    do {                                         \
+        if (ycorr) FDTD_LAUNCH_HE2(HB_, true);   \
+        else FDTD_LAUNCH_HE2(HB_, false);        \
     } while (0)
     if (L.block == FDTD_HE_HB_SMALL && FDTD_HE_HB_SMALL != 1024) FDTD_LAUNCH_HE(FDTD_HE_HB_SMALL);
     else FDTD_LAUNCH_HE(1024);
 #undef FDTD_LAUNCH_HE
+#undef FDTD_LAUNCH_HE2
     return check_launch("fdtd_update_he");
 }
 

@@ -856,6 +856,55 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     // step k: E update (+ the correction of step k-1, added before the
     // update) and source; correction y(E, H) into the scratch; H update from
     // the uncorrected E.  The last step's correction is added at the end.
+    const auto amp_of = [&](int k) -> const void* {
+        return src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
+    };
+    const auto half_e = [&](int k) {
+        const float* pending = k > 0 ? hidden : nullptr;
+        return (P.d.dtype == FDTD_F32)
+                   ? launch_e<float>(P, &cur, st, (const float*)amp_of(k), src_index, pending, thickness, S.out_cells)
+                   : launch_e<double>(P, &cur, st, (const double*)amp_of(k), src_index, pending, thickness,
+                                      S.out_cells);
+    };
+    const auto half_h = [&]() {
+        const int rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
+        if (swap) {
+            std::swap(cur.H, cur.H_out);
+            std::swap(cur.psi_h, cur.psi_h_out);
+        }
+        return rc;
+    };
+    // With a second E buffer and a plan that has the fused pass: E_0 and its
+    // correction, then per step one pass for H_{k-1} + S and E_k (+ the
+    // correction of step k-1) and the correction of step k, then H_{n-1}.
+    // Same order of operations per cell as below; the count of passes is
+    // kept even so that E ends in f->E.
+    if (n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P)) {
+        if (int rc = half_e(0)) return rc;
+        launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+        int k = 1;
+        if ((n - 1) % 2 == 1) {
+            if (int rc = half_h()) return rc;
+            if (int rc = half_e(1)) return rc;
+            launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+            k = 2;
+        }
+        for (; k < n; ++k) {
+            const int rc = (P.d.dtype == FDTD_F32)
+                               ? launch_he<float>(P, &cur, st, (const float*)amp_of(k), src_index, hidden,
+                                                  thickness, S.out_cells)
+                               : launch_he<double>(P, &cur, st, (const double*)amp_of(k), src_index, hidden,
+                                                   thickness, S.out_cells);
+            if (rc) return rc;
+            std::swap(cur.H, cur.H_out);
+            std::swap(cur.psi_h, cur.psi_h_out);
+            std::swap(cur.E, cur.E_out);
+            launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+        }
+        if (int rc = half_h()) return rc;
+        launch_shell_add(P, S, cur.E, hidden, st);
+        return check_launch("fdtd_shell_step_n");
+    }
     for (int k = 0; k < n; ++k) {
         const void* amp = src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
         const float* pending = k > 0 ? hidden : nullptr;
