@@ -182,10 +182,12 @@ def precision_report():
     return rows
 
 
-def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), profile_only: bool = False):
+def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), profile_only: bool = False,
+                   poynting: bool = False):
     """Interior 256x256x128: interior Gcell-updates/s of CPML-16, CPML-8, the
-    learned 4-cell shell and its CPML-4 base (Poynting off), and reflection
-    (dB vs a 48-cell padded run) of each on unseen scenarios."""
+    learned 4-cell shell and its CPML-4 base (Poynting off by default; with
+    poynting=True every run steps through the fused H+E pass), and
+    reflection (dB vs a 48-cell padded run) of each on unseen scenarios."""
     import torch
     from opencl_fdtd_amd import Simulation, learned as L
     net = L.ShellNet.load(L.WEIGHTS)
@@ -196,7 +198,7 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
 
     def timed(name, pad, faces, runner_net=None):
         shape, eps, idx, sl = L.embed(sc, pad)
-        sim = Simulation(eps, faces, dt=sc.dt, do_poynting=False)
+        sim = Simulation(eps, faces, dt=sc.dt, do_poynting=poynting)
         wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device="cuda")
         runner = L.ShellRunner(sim, runner_net) if runner_net is not None else None
         st = torch.cuda.current_stream()
@@ -215,6 +217,7 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
         torch.cuda.synchronize()
         ms = a.elapsed_time(b)
         rows[name] = {"shape": list(shape), "cells": int(numpy.prod(shape)), "ms_per_step": ms / steps,
+                      "poynting": poynting, "fused_pass": bool(sim.fused_steps),
                       "interior_gcell_s": n_int * steps / ms / 1e6,
                       "total_gcell_s": int(numpy.prod(shape)) * steps / ms / 1e6}
         print(json.dumps({name: rows[name]}), file=sys.stderr, flush=True)
@@ -249,6 +252,8 @@ def main():
                     help="learned 4-cell shell vs CPML-8/16: interior speed and reflection")
     ap.add_argument("--learned-profile", action="store_true",
                     help="with --learned: only the timed learned-4 loop (for rocprofv3)")
+    ap.add_argument("--poynting", action="store_true",
+                    help="with --learned: Poynting on, so every run uses the fused H+E pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
@@ -259,7 +264,7 @@ def main():
         precision_report()
         return
     if args.learned:
-        learned_report(profile_only=args.learned_profile)
+        learned_report(profile_only=args.learned_profile, poynting=args.poynting)
         return
 
     import torch
