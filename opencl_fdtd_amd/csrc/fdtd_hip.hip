@@ -74,8 +74,8 @@ namespace {
 #define FDTD_PREFETCH_H FDTD_PREFETCH   // same for the H kernel
 #endif
 #ifndef FDTD_H_TARGET
-#define FDTD_H_TARGET 256            // fused-S H workgroups per launch, at most (one round of 1024-thread
-                                     // workgroups; measured 135 vs 145 us for 512 threads x 2048)
+#define FDTD_H_TARGET 256            // fused-S H: workgroups resident at once (one 1024-thread workgroup
+                                     // per CU); the chunking minimises rounds x chunk length
 #endif
 #ifndef FDTD_H_WAVES
 #define FDTD_H_WAVES 6               // same, H kernel
@@ -96,7 +96,7 @@ namespace {
 #define FDTD_HE_PF 0                 // fused H+E, FP32: issue the loads of plane x+1 before computing plane x
 #endif
 #ifndef FDTD_HE_TARGET
-#define FDTD_HE_TARGET 256           // fused H+E workgroups per launch (at most; 256 CUs)
+#define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
 #endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
@@ -1465,6 +1465,30 @@ Launch make_launch(int plane, int sx, int block, int tile, int mode, int target)
     return L;
 }
 
+// Marching launch with one workgroup per CU at a time (1024-thread tiles of
+// whole rows plus a halo row, one plane of extra "dry" work per x chunk):
+// the number of x chunks per tile column that minimises rounds x (planes
+// per chunk + 1), the time model of such a launch (each round runs its
+// workgroups side by side; a chunk takes a time proportional to its planes).
+Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks) {
+    const int pblocks = (plane + tile - 1) / tile;
+    int best = 1;
+    int64_t best_cost = -1;
+    for (int n = 1; n <= sx; ++n) {
+        const int xchunk = (sx + n - 1) / n;
+        const int nchunk = (sx + xchunk - 1) / xchunk;
+        if (nchunk != n) continue;                       // same chunking as a smaller n
+        const int64_t blocks = (int64_t)pblocks * nchunk;
+        const int64_t rounds = (blocks + max_blocks - 1) / max_blocks;
+        const int64_t cost = rounds * (xchunk + 1);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = n;
+        }
+    }
+    return make_launch(plane, sx, block, tile, 2, best * pblocks);
+}
+
 template <class T>
 void fill_geom(const Plan& P, const Launch& L, const fdtd_fields* f, Geom<T>& g) {
     g.sx = P.sx; g.sy = P.sy; g.sz = P.sz; g.plane = P.plane;
@@ -1789,10 +1813,7 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         // re-evaluation of the in-plane neighbours
         if (P->sz <= 341) {
             const int hb = P->sz <= 128 ? FDTD_HB_SMALL : 1024;
-            const int tile = ((hb - P->sz) / P->sz) * P->sz;
-            const int pblocks = (P->plane + tile - 1) / tile;
-            const int want = FDTD_H_TARGET / pblocks;
-            P->lh = make_launch(P->plane, P->sx, hb, tile, 2, (want > 0 ? want : 1) * pblocks);
+            P->lh = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_H_TARGET);
         } else {
             P->lh = make_launch(P->plane, P->sx, kBlock, kBlock, 3, kTargetBlocks);
         }
@@ -1803,14 +1824,9 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
     if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= 341 && 3 * N * esz <= (int64_t)UINT32_MAX) {
-        // one workgroup per CU at a time (1024 threads, <= 128 VGPRs): as many
-        // chunks per tile column as fit FDTD_HE_TARGET workgroups without
-        // starting a partial extra round, so fewer dry planes
+        // one workgroup per CU at a time (1024 threads, <= 128 VGPRs)
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
-        const int tile = ((hb - P->sz) / P->sz) * P->sz;
-        const int pblocks = (P->plane + tile - 1) / tile;
-        const int want = FDTD_HE_TARGET / pblocks;
-        P->lhe = make_launch(P->plane, P->sx, hb, tile, 2, (want > 0 ? want : 1) * pblocks);
+        P->lhe = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_HE_TARGET);
     }
     *plan_out = P;
     return 0;
