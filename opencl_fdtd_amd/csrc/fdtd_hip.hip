@@ -1160,6 +1160,18 @@ __global__ __launch_bounds__(HB, HB >= 1024 ? 4 : FDTD_H_WAVES) void h_march(HAr
 // Per cell the arithmetic is h_update + S (as in h_march MODE 2) followed by
 // e_update (as in e_march), so the results are bit-identical to the two
 // kernels run one after the other.
+//
+// What keeps the march short per plane (measured on MI355X):
+//   * every load of a plane is issued before any of them is used: the field
+//     loads, eps, and the psi pairs of both half-steps (one memory round trip
+//     per plane; loading a slab face's psi inside its branch costs one more
+//     round trip per face);
+//   * y/z CPML slab offsets per thread are worked out once (psi_col), the
+//     coefficient block is read from an LDS copy, and each field is one
+//     base pointer plus 32-bit offsets (Packed3): few scalar registers, no
+//     spills;
+//   * the barrier orders LDS only (lds_barrier), so stores stay in flight;
+//   * one 1024-thread workgroup per CU, chunks chosen by make_round_launch.
 // ---------------------------------------------------------------------------
 // The fused pass's systems: packed fields, no Bloch partner.
 template <class T>
