@@ -182,3 +182,31 @@ def test_update_he_support_rules():
     assert supported(poynting=2) == 0
     assert supported(poynting=1, bloch_mask=1) == 0
     assert lib.fdtd_update_he_supported(None) == 0
+
+
+def test_update_he_rejects_bad_calls():
+    """fdtd_update_he validates before launching anything (fake device
+    pointers: no kernel runs on these paths)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+
+    def call(poynting, e_out_same=False, h_out_same=False):
+        d = _desc((10, 11, 12), [(0, 0, 2)], poynting=poynting)
+        plan = ctypes.c_void_p()
+        assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+        f = _lib.Fields()
+        f.E, f.H, f.eps, f.S, f.oS = 0x1000, 0x2000, 0x3000, 0x4000, 0x5000
+        f.H_out = f.H if h_out_same else 0x6000
+        f.psi_e, f.psi_h, f.psi_h_out, f.coef = 0x7000, 0x8000, 0x9000, 0xa000
+        f.E_out = f.E if e_out_same else 0xb000
+        rc = lib.fdtd_update_he(plan, ctypes.byref(f), -1, None, None)
+        msg = lib.fdtd_last_error().decode()
+        lib.fdtd_plan_destroy(plan)
+        return rc, msg
+    rc, msg = call(poynting=2)
+    assert rc != 0 and "Poynting" in msg
+    rc, msg = call(poynting=1, e_out_same=True)
+    assert rc != 0 and "E_out" in msg
+    rc, msg = call(poynting=1, h_out_same=True)
+    assert rc != 0 and "H_out" in msg
+    assert lib.fdtd_update_he(None, None, -1, None, None) != 0
