@@ -236,6 +236,52 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
     return out
 
 
+def configs_report(seconds: float = 1.0):
+    """Throughput of every workload (C1 small 64^3, C2 README slab, C3 Bloch
+    supercell, C4 the adjoint's 128^3 forward grid): step_n timed with HIP
+    events, Gcell-updates/s and the fraction of 8 TB/s that the step's
+    algorithmic bytes per cell-update represent.  One JSON line per config."""
+    import torch
+    from opencl_fdtd_amd import Simulation
+    from opencl_fdtd_amd import workloads
+    out = []
+    for name in ("small", "phc_slab", "bloch", "adjoint"):
+        w = workloads.BUILDERS[name](steps=10)
+        sim = Simulation(w.epsilon, [dict(p) for p in w.pmls], dt=w.dt, bloch_boundaries=w.bloch_boundaries,
+                         do_poynting=w.do_poynting)
+        st = torch.cuda.current_stream()
+        sim.step_n(20)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        sim.step_n(20)
+        b.record(st)
+        torch.cuda.synchronize()
+        n = max(20, int(seconds * 1e3 / (a.elapsed_time(b) / 20)))
+        a.record(st)
+        sim.step_n(n)
+        b.record(st)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        gcell = w.cells * n / (ms / 1e3) / 1e9
+        fused = bool(sim.fused_steps)
+        if fused:
+            # one he_march per step: 18 words per cell, 8 per slab cell and face
+            nc = float(w.cells)
+            slab = sum(min(int(p.get("thickness", 8)), w.shape["xyz".find(p["axis"])]) * nc /
+                       w.shape["xyz".find(p["axis"])] for p in w.pmls)
+            bpc = 4.0 * (18 + 8 * slab / nc)
+        else:
+            bpc = workloads.algorithmic_bytes_per_cell(w.shape, w.pmls, poynting=w.do_poynting,
+                                                       bloch=bool(w.bloch_boundaries))
+        row = {"workload": w.name, "shape": list(w.shape), "steps": n, "ms_per_step": ms / n,
+               "gcell_updates_s": gcell, "fused_pass": fused, "bytes_per_cell_update": bpc,
+               "hbm_frac": gcell * bpc / HBM_PEAK_GBS}
+        print(json.dumps(row), flush=True)
+        out.append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +300,8 @@ def main():
                     help="with --learned: only the timed learned-4 loop (for rocprofv3)")
     ap.add_argument("--poynting", action="store_true",
                     help="with --learned: Poynting on, so every run uses the fused H+E pass")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="throughput of every workload (one JSON line each) instead of the benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
@@ -265,6 +313,9 @@ def main():
         return
     if args.learned:
         learned_report(profile_only=args.learned_profile, poynting=args.poynting)
+        return
+    if args.all_configs:
+        configs_report()
         return
 
     import torch
