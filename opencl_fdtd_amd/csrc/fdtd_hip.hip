@@ -1661,12 +1661,12 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     const bool pair = P.d.bloch_mask != 0;
     const bool seam = pair && P.d.bloch_mode == FDTD_BLOCH_PHASE;
     a.coef_elems = (int)P.coef_elems;
-    // (the paired Bloch systems stay on the plain CPML path: their register
-    // budget has no room for the prefetched psi pairs)
-    const bool ct = FDTD_EH_CTAB && P.coef_elems <= kCoefTab && P.lh.mode != 3 && !pair;
+    // (the paired Bloch systems use it in mode 2 only, whose 1024-thread
+    // tiles allow 128 VGPRs; in modes 0/1 the prefetched psi pairs spill)
+    const bool ct = FDTD_EH_CTAB && P.coef_elems <= kCoefTab && (P.lh.mode == 2 || (P.lh.mode != 3 && !pair));
     if (!pair) ct ? launch_h_mode<T, false, false, true>(P.lh, st, a) : launch_h_mode<T, false, false, false>(P.lh, st, a);
-    else if (!seam) launch_h_mode<T, true, false, false>(P.lh, st, a);
-    else launch_h_mode<T, true, true, false>(P.lh, st, a);
+    else if (!seam) ct ? launch_h_mode<T, true, false, true>(P.lh, st, a) : launch_h_mode<T, true, false, false>(P.lh, st, a);
+    else ct ? launch_h_mode<T, true, true, true>(P.lh, st, a) : launch_h_mode<T, true, true, false>(P.lh, st, a);
     return check_launch("fdtd_update_h");
 }
 
