@@ -79,6 +79,31 @@ def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
     _assert_same(_state(a), _state(b), "step_n fused vs unfused")
 
 
+@pytest.mark.parametrize("shape, T, n", [
+    ((160, 300, 250), 8, 6),     # 12 M cells, 1024-thread tiles of 3 rows, several rounds of workgroups
+    ((40, 700, 100), 6, 5),      # 78 tile columns of 9 rows, short x
+])
+def test_fused_large_grids_equal_unfused(shape, T, n):
+    """Launch shapes the cost model gives larger grids (multiple rounds of
+    workgroups), fused against unfused, bit for bit."""
+    print("case fused large", shape, T, n, flush=True)
+    rng = numpy.random.default_rng(sum(shape))
+    eps = [rng.uniform(1, 4, shape).astype(numpy.float32) for _ in range(3)]
+    pmls = [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
+    a = _sim(eps, pmls, None, None, numpy.float32, True)
+    b = _sim(eps, pmls, None, None, numpy.float32, False)
+    idx = 1 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
+    wave = numpy.sin(0.25 * numpy.arange(n)).astype(numpy.float32)
+    a.step_n(n, idx, wave)
+    b.step_n(n, idx, wave)
+    torch.cuda.synchronize()
+    assert float(torch.abs(a.E).max()) > 0
+    for name in ("E", "H", "S"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    del a, b
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("dtype", [numpy.float32, numpy.float64])
 def test_update_he_equals_h_then_e(dtype):
     """One fdtd_update_he = update_H (+S), update_E, E[idx] += amp."""
