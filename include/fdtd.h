@@ -54,7 +54,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 2
+#define FDTD_ABI_VERSION 3
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -119,6 +119,8 @@ typedef struct {
     void* E_out;      /* optional second E buffer (3N, != E): written by fdtd_update_he;
                          lets fdtd_step_n fuse each H half-step with the next E half-step.
                          NULL: no fusion */
+    void* psi_e_out;  /* optional second psi_E block (!= psi_e): written by fdtd_update_he2;
+                         with E_out it lets fdtd_step_n run two steps per pass.  NULL: none */
 } fdtd_fields;
 
 int fdtd_abi_version(void);
@@ -166,6 +168,24 @@ int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, co
                    void* stream);
 /* 1 when fdtd_update_he can run with this plan, else 0. */
 int fdtd_update_he_supported(const void* plan);
+
+/* Two iterations of the reference caller loop (fdtd.py:162-175) from the
+ * middle of a step, in one pass over the grid: H-step (with S) on E, H;
+ * E-step; E[src_index] += src_amp[0]; H-step (with S); E-step;
+ * E[src_index] += src_amp[1] (src_amp: NULL, or a device pointer to two
+ * values).  Reads E, H, psi_e, psi_h and writes E_out, H_out, psi_e_out,
+ * psi_h_out, S (S of the second H-step).  Bit-identical to fdtd_update_he
+ * twice (or to update_H/update_S (update_h.cl, update_s.cl), update_E
+ * (update_e.cl) and the source line of fdtd.py:169, twice).  Needs
+ * fdtd_update_he2_supported(plan) and distinct E/E_out, H/H_out,
+ * psi_h/psi_h_out, psi_e/psi_e_out.  With f->E_out and f->psi_e_out set,
+ * fdtd_step_n runs most of its steps two per pass; psi_e ends in f->psi_e. */
+int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,
+                    void* stream);
+/* 1 when fdtd_update_he2 can run with this plan (FP32, fused Poynting, no
+   Bloch boundaries, rows of at most 256 cells, CPML slabs of one axis
+   apart), else 0. */
+int fdtd_update_he2_supported(const void* plan);
 
 /* ------------------------------------------------------------------------
  * Discrete adjoint of the step and FP64 reductions.  No counterpart in the

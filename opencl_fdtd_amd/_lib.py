@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_FACES = 12
 
 F32, F64 = 0, 1
@@ -23,7 +23,7 @@ EXPORTS = (
     "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy",
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
-    "fdtd_update_he", "fdtd_update_he_supported",
+    "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
     "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems",
     "fdtd_shell_apply", "fdtd_shell_step_n", "fdtd_shell_backward",
@@ -50,7 +50,7 @@ class Desc(ctypes.Structure):
 class Fields(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "E", "H", "H_out", "eps", "S", "oS", "F", "G", "G_out",
-        "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d", "E_out")]
+        "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d", "E_out", "psi_e_out")]
 
 
 class AdjointBufs(ctypes.Structure):
@@ -99,6 +99,8 @@ def load() -> ctypes.CDLL:
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_update_he.argtypes = [P, F, ctypes.c_int64, P, P]
     lib.fdtd_update_he_supported.argtypes = [P]
+    lib.fdtd_update_he2.argtypes = [P, F, ctypes.c_int64, P, P]
+    lib.fdtd_update_he2_supported.argtypes = [P]
     lib.fdtd_adjoint_step.argtypes = [P, ctypes.POINTER(AdjointBufs), P]
     lib.fdtd_plane_sum.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_reduce_partials.argtypes = [P]

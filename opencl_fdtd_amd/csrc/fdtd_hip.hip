@@ -95,6 +95,12 @@ namespace {
 #ifndef FDTD_HE_PF
 #define FDTD_HE_PF 0                 // fused H+E, FP32: issue the loads of plane x+1 before computing plane x
 #endif
+#ifndef FDTD_HE2
+#define FDTD_HE2 1                   // two-step pass available (FP32)
+#endif
+#ifndef FDTD_HE2_S2
+#define FDTD_HE2_S2 1                // two-step pass: also store S of its first H half-step
+#endif
 #ifndef FDTD_HE_TARGET
 #define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
 #endif
@@ -297,6 +303,13 @@ struct FaceCoef {
     T p0, p1, p2;
 };
 
+// A cell's psi pair per axis (u, v); only the axes whose slab holds the cell
+// are set.
+template <class T>
+struct PsiPair {
+    T u[3], v[3];
+};
+
 template <class T, bool ISE>
 __device__ __forceinline__ FaceCoef<T> face_coef(const Geom<T>& g, const FaceDev<T>& f, int ir) {
     const T* c = g.coef + f.coef_off + (ISE ? 0 : 3 * f.thick);
@@ -321,7 +334,7 @@ __device__ __forceinline__ FaceCoef<T> face_coef_at(const T* tab, const FaceDev<
 template <int R, class T, bool ISE>
 __device__ __forceinline__ void face_step(const FaceCoef<T>& k, T ou, T ov, T* psi_out, uint32_t bu,
                                           uint32_t bv, bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx,
-                                          T& dzy, T& px, T& py, T& pz) {
+                                          T& dzy, T& px, T& py, T& pz, PsiPair<T>* keep = nullptr) {
     const T p0 = k.p0, p1 = k.p1, p2 = k.p2;
     T su, sv;
     if (R == 0) {                      // r = x, (u, v) = (y, z)
@@ -343,6 +356,10 @@ __device__ __forceinline__ void face_step(const FaceCoef<T>& k, T ou, T ov, T* p
     if (write) {
         st(psi_out, bu, su);
         st(psi_out, bv, sv);
+    }
+    if (keep) {
+        keep->u[R] = su;
+        keep->v[R] = sv;
     }
 }
 
@@ -479,11 +496,6 @@ __device__ __forceinline__ uint32_t psi_byte(const Geom<T>& g, const FaceDev<T>&
     return ((uint32_t)f.psi_off + psi_index<R>(g, f, x, c.y, c.z, c.p, ir)) * (uint32_t)sizeof(T);
 }
 
-template <class T>
-struct PsiPair {
-    T u[3], v[3];
-};
-
 template <int R, class T>
 __device__ __forceinline__ void psi_fetch_axis(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
                                                PsiHit h, PsiPair<T>& q) {
@@ -510,13 +522,14 @@ template <int R, class T, bool ISE>
 __device__ __forceinline__ void cpml_axis(const Geom<T>& g, int x, const Cell& c, PsiHit h,
                                           const FaceCoef<T>& k, const T* tab, const PsiPair<T>& q,
                                           const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
-                                          T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+                                          T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz,
+                                          PsiPair<T>* keep = nullptr) {
     if (!h.on()) return;
     const FaceDev<T> f = face_of(g, R, h.slot());
     const uint32_t bu = psi_byte<R>(g, f, x, c, h.ir());
     const FaceCoef<T> kk = tab ? face_coef_at<T, ISE>(tab, f, h.ir()) : k;
     face_step<R, T, ISE>(kk, q.u[R], q.v[R], psi_out, bu, bu + (uint32_t)f.face_elems * (uint32_t)sizeof(T),
-                         write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+                         write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz, keep);
     if (h.both()) {
         const FaceDev<T>& f1 = g.face[R][1];
         const int cr = R == 0 ? x : R == 1 ? c.y : c.z;
@@ -542,13 +555,14 @@ template <class T, bool ISE>
 __device__ __forceinline__ void cpml_apply(const Geom<T>& g, int x, const Cell& c, const PsiHit* hit,
                                            const FaceCoef<T>* k, const T* tab, const PsiPair<T>& q,
                                            const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
-                                           T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+                                           T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz,
+                                           PsiPair<T>* keep = nullptr) {
     cpml_axis<0, T, ISE>(g, x, c, hit[0], k[0], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz);
+                         py, pz, keep);
     cpml_axis<1, T, ISE>(g, x, c, hit[1], k[1], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz);
+                         py, pz, keep);
     cpml_axis<2, T, ISE>(g, x, c, hit[2], k[2], tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz);
+                         py, pz, keep);
 }
 
 // The same for an in-plane axis (y or z) with the slab data a thread needs
@@ -597,7 +611,8 @@ __device__ __forceinline__ void psi_fetch_col(const PsiCol& k, const T* psi_in, 
 template <int R, class T, bool ISE>
 __device__ __forceinline__ void cpml_col(const Geom<T>& g, int x, const Cell& c, const PsiCol& k, const T* tab,
                                          const PsiPair<T>& q, const T* psi_in, T* psi_out, bool write, T& dxy,
-                                         T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
+                                         T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz,
+                                         PsiPair<T>* keep = nullptr) {
     if (!k.h.on()) return;
     const uint32_t bu = k.base + (uint32_t)x * k.stride;
     const T* cc = tab + k.cidx + (ISE ? 0 : 3 * k.thick);
@@ -606,7 +621,7 @@ __device__ __forceinline__ void cpml_col(const Geom<T>& g, int x, const Cell& c,
     kk.p1 = cc[k.thick];
     kk.p2 = cc[2 * k.thick];
     face_step<R, T, ISE>(kk, q.u[R], q.v[R], psi_out, bu, bu + k.vofs, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz);
+                         py, pz, keep);
     if (k.h.both()) {
         const FaceDev<T>& f1 = g.face[R][1];
         const int cr = R == 1 ? c.y : c.z;
@@ -638,12 +653,14 @@ template <class T, bool ISE>
 __device__ __forceinline__ void cpml_cols(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>& k,
                                           const T* tab, const PsiPair<T>& q, const T* psi_in, T* psi_out,
                                           bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px,
-                                          T& py, T& pz) {
+                                          T& py, T& pz, PsiPair<T>* keep = nullptr) {
     const FaceCoef<T> none{};
     cpml_axis<0, T, ISE>(g, x, c, k.hx, none, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz);
-    cpml_col<1, T, ISE>(g, x, c, k.y, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    cpml_col<2, T, ISE>(g, x, c, k.z, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+                         py, pz, keep);
+    cpml_col<1, T, ISE>(g, x, c, k.y, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                        keep);
+    cpml_col<2, T, ISE>(g, x, c, k.z, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                        keep);
 }
 
 // CPML with prefetched psi: per-axis hits (cpml_apply) or columns (cpml_cols)
@@ -651,15 +668,17 @@ template <class T, bool ISE>
 __device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const PsiHit* hit,
                                          const FaceCoef<T>* kc, const T* tab, const PsiPair<T>& q,
                                          const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
-                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
-    cpml_apply<T, ISE>(g, x, c, hit, kc, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz, PsiPair<T>* keep = nullptr) {
+    cpml_apply<T, ISE>(g, x, c, hit, kc, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                       keep);
 }
 template <class T, bool ISE>
 __device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>* cols,
                                          const FaceCoef<T>*, const T* tab, const PsiPair<T>& q,
                                          const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
-                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz) {
-    cpml_cols<T, ISE>(g, x, c, *cols, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+                                         T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz, PsiPair<T>* keep = nullptr) {
+    cpml_cols<T, ISE>(g, x, c, *cols, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                      keep);
 }
 
 // ---------------------------------------------------------------------------
@@ -696,12 +715,14 @@ __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, cons
     }
 }
 
-template <class T, bool PRE = false, class S, class K = const PsiHit*>
-__device__ __forceinline__ void e_update(const Geom<T>& g, const S& s, const Cell& c, int x, uint32_t oi,
-                                         const EIn<T>& v, T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy,
-                                         T iz, int src_comp, T amp, K hit = nullptr,
-                                         const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr,
-                                         const T* tab = nullptr) {
+// The E update of cell (x, c) without storing E: the new E (with the source
+// added) in ex, ey, ez.  psi is written in place when write is set and the
+// new pairs are kept in *keep when keep is set.
+template <class T, bool PRE, class S, class K>
+__device__ __forceinline__ void e_value(const Geom<T>& g, const S& s, const Cell& c, int x, const EIn<T>& v,
+                                        T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy, T iz, int src_comp,
+                                        T amp, K hit, const FaceCoef<T>* kc, const PsiPair<T>* q, const T* tab,
+                                        bool write, PsiPair<T>* keep, T& ex, T& ey, T& ez) {
     T dxy = (v.hx - v.hx_ym) * iy;
     T dxz = (v.hx - v.hx_zm) * iz;
     T dyx = (v.hy - hy_xm) * ix;
@@ -710,14 +731,26 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const S& s, const Cel
     T dzy = (v.hz - v.hz_ym) * iy;
     T px = (T)0, py = (T)0, pz = (T)0;
     if (PRE)
-        cpml_pre<T, true>(g, x, c, hit, kc, tab, *q, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, true, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    T ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
-    T ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
-    T ez = v.ez + g.dt / eps2 * (dyx - dxy + pz);
+        cpml_pre<T, true>(g, x, c, hit, kc, tab, *q, s.psi, s.psi, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                          keep);
+    else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
+    ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
+    ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
+    ez = v.ez + g.dt / eps2 * (dyx - dxy + pz);
     if (src_comp == 0) ex = ex + amp;
     else if (src_comp == 1) ey = ey + amp;
     else if (src_comp == 2) ez = ez + amp;
+}
+
+template <class T, bool PRE = false, class S, class K = const PsiHit*>
+__device__ __forceinline__ void e_update(const Geom<T>& g, const S& s, const Cell& c, int x, uint32_t oi,
+                                         const EIn<T>& v, T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy,
+                                         T iz, int src_comp, T amp, K hit = nullptr,
+                                         const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr,
+                                         const T* tab = nullptr) {
+    T ex, ey, ez;
+    e_value<T, PRE>(g, s, c, x, v, hy_xm, hz_xm, eps0, eps1, eps2, ix, iy, iz, src_comp, amp, hit, kc, q, tab, true,
+                    (PsiPair<T>*)nullptr, ex, ey, ez);
     stc<FDTD_NT_E>(s.E, 0, oi, ex);
     stc<FDTD_NT_E>(s.E, 1, oi, ey);
     stc<FDTD_NT_E>(s.E, 2, oi, ez);
@@ -925,7 +958,7 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const S& s, const
                                              uint32_t pbytes, T ex, T ey, T ez, const HIn<T>& v, T ix, T iy,
                                              T iz, bool write, K hit = nullptr,
                                              const FaceCoef<T>* kc = nullptr, const PsiPair<T>* q = nullptr,
-                                             const T* tab = nullptr) {
+                                             const T* tab = nullptr, PsiPair<T>* keep = nullptr) {
     T ey1 = v.eyn, ez1 = v.ezn;
     if constexpr (SEAM) {
         if ((g.seam_mask & 1) && x == g.sx - 1) {
@@ -942,7 +975,7 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const S& s, const
     T px = (T)0, py = (T)0, pz = (T)0;
     if (PRE)
         cpml_pre<T, false>(g, x, c, hit, kc, tab, *q, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                           py, pz);
+                           py, pz, keep);
     else
         cpml_cell<T, false>(g, x, c.y, c.z, c.p, s.psi_in, s.psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
                             py, pz);
@@ -1395,6 +1428,246 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Two full steps in one march (he2_march): H^{n+1/2}, E^{n+1}, H^{n+3/2}
+// with S, E^{n+2}.  E^n, H^{n-1/2}, eps and the psi blocks are read once and
+// E^{n+2}, H^{n+3/2}, S and the psi blocks written once for the two steps;
+// the fields of step n+1 never leave the chip.
+//
+// The march runs two stages one plane apart.  At plane x the first stage
+// evaluates H^{n+1/2}(x) and E^{n+1}(x) exactly as he_march does (H through
+// LDS, E^{n+1} at y-1, z-1 from LDS, x-1 from registers); the second stage
+// then evaluates H^{n+3/2}(x-1), which needs E^{n+1} at x-1 (registers), at x
+// (just computed) and at y+1, z+1 of plane x-1 (LDS, written one iteration
+// before), and E^{n+2}(x-1) and S(x-1) from H^{n+3/2} through LDS.  Each
+// stage widens the work in y by one row, so a tile computes rows y0-2 .. y1
+// for the owned rows y0 .. y1-1 (whole rows, z periodic inside a row), and
+// an x chunk x0 .. x1-1 marches planes x0-2 .. x1.  The psi pairs of step
+// n+1 stay in registers from the first stage to the second.  Everything the
+// pass reads is read from buffers it does not write (E -> E_out, H -> H_out,
+// psi_h -> psi_h_out, psi_e -> psi_e_out), because the halo rows and planes
+// of one workgroup are owned cells of another.  Per cell the arithmetic is
+// that of h_update + S and e_update, in the same order, so the results are
+// bit-identical to two he_march passes (or to update_H, update_E, ...).
+// S2 also stores S of the first H half-step (which the second overwrites).
+// ---------------------------------------------------------------------------
+template <class T>
+struct HE2Args {
+    Geom<T> g;
+    Packed3<const T> E, H, eps;     // E^n, H^{n-1/2}
+    Packed3<T> E_out, H_out, S;     // E^{n+2}, H^{n+3/2}, S
+    const T* psi_h_in;
+    T* psi_h_out;
+    const T* psi_e_in;
+    T* psi_e_out;
+    const T* src_amp;               // two values: added to E^{n+1} and E^{n+2} at the source; null: none
+    int src_x, src_p, src_comp;
+    int coef_elems;
+};
+
+template <class T, bool NU, bool S2>
+__global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
+    constexpr int HB = 1024;
+    __shared__ T la[S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
+    __shared__ T le[2][3][HB];         // E^{n+1} by plane parity
+    __shared__ T lb[7][HB];            // oS products of H^{n+3/2}, then H^{n+3/2} (x, y, z)
+    __shared__ T ctab[kCoefTab];
+    const Geom<T>& g = a.g;
+    int pb, ck;
+    block_coords(g.pblocks, g.nchunk, &pb, &ck);
+    const int t = threadIdx.x;
+    const int sz = g.sz;
+    const int tbeg = pb * g.tile;
+    const int tend = min(tbeg + g.tile, g.plane);
+    const int ncell = (tend - tbeg) + 3 * sz;          // rows y0-2 .. y1
+    const bool valid = t < ncell;                       // H^{n+1/2}
+    const bool rowe = valid && t >= sz;                 // E^{n+1}: rows y0-1 .. y1
+    const bool rowh = rowe && t < ncell - sz;           // H^{n+3/2}: rows y0-1 .. y1-1
+    const bool own = rowh && t >= 2 * sz;               // rows y0 .. y1-1
+    int p = (tbeg - 2 * sz + t) % g.plane;
+    if (p < 0) p += g.plane;
+    const Cell c = make_cell<T>(g, valid ? p : 0);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const int x0 = ck * g.xchunk;
+    const int x1 = min(x0 + g.xchunk, g.sx);
+    const int n_it = (x1 - x0) + 3;
+    const T sf = g.dt * (T)0.125;
+    const T iy = NU ? g.inv_e[1][c.y] : g.inv_u;     // H steps
+    const T iz = NU ? g.inv_e[2][c.z] : g.inv_u;
+    const T jy = NU ? g.inv_h[1][c.y] : g.inv_u;     // E steps
+    const T jz = NU ? g.inv_h[2][c.z] : g.inv_u;
+    const int tym = t - sz, typ = t + sz;
+    const int tzm = (c.z > 0) ? t - 1 : t + sz - 1;
+    const int tzp = (c.z < sz - 1) ? t + 1 : t - (sz - 1);
+    const int src_comp = (valid && a.src_amp != nullptr && p == a.src_p) ? a.src_comp : -1;
+    for (int i = t; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
+    lds_barrier();
+    const T* tab = ctab;
+    CpmlCols<T> cols;
+    cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
+    cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
+    HSysP<T> hA{a.E, a.H, {nullptr, 0u}, a.psi_h_in, nullptr};
+    HSysP<T> hB{a.E, a.H, a.H_out, a.psi_h_in, a.psi_h_out};
+    ESysP<T> eA{{nullptr, 0u}, (T*)a.psi_e_in};
+    ESysP<T> eB{a.E_out, a.psi_e_out};
+
+    int xa = (x0 - 2) % g.sx;
+    if (xa < 0) xa += g.sx;
+    T qx = (T)0, qy = (T)0, qz = (T)0;                  // E^n at plane xa
+    if (valid) {
+        const uint32_t ob = (uint32_t)xa * pbytes;
+        qx = ldc<false>(a.E, 0, ob + c.op);
+        qy = ldc<false>(a.E, 1, ob + c.op);
+        qz = ldc<false>(a.E, 2, ob + c.op);
+    }
+    // carried from plane xa-1: H^{n+1/2}, E^{n+1}, the psi pairs after the
+    // first half-steps, eps, (S2) oS of the first H half-step; from xa-2:
+    // H^{n+3/2} and its oS products
+    T h1x = (T)0, h1y = (T)0, h1z = (T)0;
+    T f1x = (T)0, f1y = (T)0, f1z = (T)0;
+    PsiPair<T> kh{}, ke{};
+    T ep0 = (T)1, ep1 = (T)1, ep2 = (T)1;
+    T oAyx_m = (T)0, oAzx_m = (T)0;
+    T h2y_m = (T)0, h2z_m = (T)0, oByx_m = (T)0, oBzx_m = (T)0;
+    for (int it = 0; it < n_it; ++it) {
+        const int xb = (xa == 0) ? g.sx - 1 : xa - 1;
+        const int xn = (xa == g.sx - 1) ? 0 : xa + 1;
+        const bool ea = it >= 1;                 // E^{n+1} at xa
+        const bool hb = it >= 2;                 // H^{n+3/2} at xb
+        const bool wb = it >= 3;                 // xb is in the chunk: stores
+        // ---- loads of plane xa
+        CpmlCols<T> ka = cols, kb = cols;
+        ka.hx = slab_hit(g, 0, xa, g.sx);
+        kb.hx = slab_hit(g, 0, xb, g.sx);
+        HIn<T> in;
+        PsiPair<T> qh, qe;
+        T e0 = (T)1, e1 = (T)1, e2 = (T)1;
+        if (valid) psi_fetch_cols(g, a.psi_h_in, xa, c, ka, qh);
+        if (rowe && ea) {
+            const uint32_t of = (uint32_t)xa * pbytes + c.op;
+            psi_fetch_cols(g, a.psi_e_in, xa, c, ka, qe);
+            e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
+            e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
+            e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
+        }
+        h_fetch<T, false>(g, hA, c, xa, pbytes, in);
+        const int b = it & 1;
+        // ---- first stage: H^{n+1/2}(xa) on every row
+        const T ixa = NU ? g.inv_e[0][xa] : g.inv_u;
+        PsiPair<T> nkh, nke;
+        T g1x = (T)0, g1y = (T)0, g1z = (T)0;
+        T oAxy = (T)0, oAxz = (T)0, oAyz = (T)0, oAyx = (T)0, oAzx = (T)0, oAzy = (T)0;
+        if (valid) {
+            const HCell<T> h = h_update<T, false, true>(g, hA, c, xa, pbytes, qx, qy, qz, in, ixa, iy, iz, false,
+                                                        (const CpmlCols<T>*)&ka, nullptr, &qh, tab, &nkh);
+            g1x = h.hx1; g1y = h.hy1; g1z = h.hz1;
+            if (S2) {
+                const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+                oAxy = h.aEyx * aHz;
+                oAxz = -h.aEzx * aHy;
+                oAyz = h.aEzy * aHx;
+                oAyx = -h.aExy * aHz;
+                oAzx = h.aExz * aHy;
+                oAzy = -h.aEyz * aHx;
+            }
+        }
+        la[0][t] = g1x;
+        la[1][t] = g1y;
+        la[2][t] = g1z;
+        if (S2) {
+            la[3][t] = oAxy;
+            la[4][t] = oAzy;
+            la[5][t] = oAxz;
+            la[6][t] = oAyz;
+        }
+        lds_barrier();
+        // ---- first stage: S(xa) (S2) and E^{n+1}(xa) on rows y0-1 .. y1
+        T n1x = (T)0, n1y = (T)0, n1z = (T)0;
+        if (rowe && ea) {
+            if (S2 && own && hb && it < n_it - 1) {
+                const uint32_t oi = (uint32_t)xa * pbytes + c.op;
+                stc<FDTD_NT_H>(a.S, 0, oi, (oAxy + oAxz + la[3][tym] + la[5][tzm]) * sf);
+                stc<FDTD_NT_H>(a.S, 1, oi, (oAyz + oAyx + la[6][tzm] + oAyx_m) * sf);
+                stc<FDTD_NT_H>(a.S, 2, oi, (oAzx + oAzy + oAzx_m + la[4][tym]) * sf);
+            }
+            EIn<T> v;
+            v.hx = g1x; v.hy = g1y; v.hz = g1z;
+            v.hx_ym = la[0][tym];
+            v.hz_ym = la[2][tym];
+            v.hx_zm = la[0][tzm];
+            v.hy_zm = la[1][tzm];
+            v.ex = qx; v.ey = qy; v.ez = qz;
+            const T jxa = NU ? g.inv_h[0][xa] : g.inv_u;
+            const bool src = src_comp >= 0 && xa == a.src_x;
+            e_value<T, true>(g, eA, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1,
+                             src ? a.src_amp[0] : (T)0, (const CpmlCols<T>*)&ka, nullptr, &qe, tab, false, &nke, n1x,
+                             n1y, n1z);
+        }
+        le[b][0][t] = n1x;
+        le[b][1][t] = n1y;
+        le[b][2][t] = n1z;
+        // ---- second stage: H^{n+3/2}(xb) on rows y0-1 .. y1-1
+        T g2x = (T)0, g2y = (T)0, g2z = (T)0;
+        T oBxy = (T)0, oBxz = (T)0, oByz = (T)0, oByx = (T)0, oBzx = (T)0, oBzy = (T)0;
+        if (rowh && hb) {
+            HIn<T> v;
+            v.exn = n1x; v.eyn = n1y; v.ezn = n1z;
+            v.ex_y = le[b ^ 1][0][typ];
+            v.ez_y = le[b ^ 1][2][typ];
+            v.ex_z = le[b ^ 1][0][tzp];
+            v.ey_z = le[b ^ 1][1][tzp];
+            v.hx = h1x; v.hy = h1y; v.hz = h1z;
+            const T ixb = NU ? g.inv_e[0][xb] : g.inv_u;
+            const HCell<T> h = h_update<T, false, true>(g, hB, c, xb, pbytes, f1x, f1y, f1z, v, ixb, iy, iz,
+                                                        own && wb, (const CpmlCols<T>*)&kb, nullptr, &kh, tab);
+            const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+            oBxy = h.aEyx * aHz;
+            oBxz = -h.aEzx * aHy;
+            oByz = h.aEzy * aHx;
+            oByx = -h.aExy * aHz;
+            oBzx = h.aExz * aHy;
+            oBzy = -h.aEyz * aHx;
+            g2x = h.hx1; g2y = h.hy1; g2z = h.hz1;
+        }
+        lb[0][t] = oBxy;
+        lb[1][t] = oBzy;
+        lb[2][t] = oBxz;
+        lb[3][t] = oByz;
+        lb[4][t] = g2x;
+        lb[5][t] = g2y;
+        lb[6][t] = g2z;
+        lds_barrier();
+        // ---- second stage: S(xb) and E^{n+2}(xb) on the owned rows
+        if (own && wb) {
+            const uint32_t oi = (uint32_t)xb * pbytes + c.op;
+            stc<FDTD_NT_H>(a.S, 0, oi, (oBxy + oBxz + lb[0][tym] + lb[2][tzm]) * sf);
+            stc<FDTD_NT_H>(a.S, 1, oi, (oByz + oByx + lb[3][tzm] + oByx_m) * sf);
+            stc<FDTD_NT_H>(a.S, 2, oi, (oBzx + oBzy + oBzx_m + lb[1][tym]) * sf);
+            EIn<T> v;
+            v.hx = g2x; v.hy = g2y; v.hz = g2z;
+            v.hx_ym = lb[4][tym];
+            v.hz_ym = lb[6][tym];
+            v.hx_zm = lb[4][tzm];
+            v.hy_zm = lb[5][tzm];
+            v.ex = f1x; v.ey = f1y; v.ez = f1z;
+            const T jxb = NU ? g.inv_h[0][xb] : g.inv_u;
+            const bool src = src_comp >= 0 && xb == a.src_x;
+            e_update<T, true>(g, eB, c, xb, oi, v, h2y_m, h2z_m, ep0, ep1, ep2, jxb, jy, jz, src ? src_comp : -1,
+                              src ? a.src_amp[1] : (T)0, (const CpmlCols<T>*)&kb, nullptr, &ke, tab);
+        }
+        // ---- carry to the next plane
+        h2y_m = g2y; h2z_m = g2z;
+        oByx_m = oByx; oBzx_m = oBzx;
+        h1x = g1x; h1y = g1y; h1z = g1z;
+        f1x = n1x; f1y = n1y; f1z = n1z;
+        kh = nkh; ke = nke;
+        ep0 = e0; ep1 = e1; ep2 = e2;
+        if (S2) { oAyx_m = oAyx; oAzx_m = oAzx; }
+        qx = in.exn; qy = in.eyn; qz = in.ezn;
+        xa = xn;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // S from oS (two-pass reference dataflow), update_s.cl:33-36.
 // ---------------------------------------------------------------------------
 template <class T>
@@ -1453,6 +1726,7 @@ struct Plan {
     int64_t N;
     Launch le, lh;
     Launch lhe;          // fused H+E pass (mode 2 when available, else mode -1)
+    Launch lhe2;         // two-step pass (mode 2 when available, else mode -1)
     int64_t psi_elems, coef_elems;
     struct F { int lo, hi, psi_off, face_elems, coef_off; } face[FDTD_MAX_FACES];
 };
@@ -1482,7 +1756,7 @@ Launch make_launch(int plane, int sx, int block, int tile, int mode, int target)
 // the number of x chunks per tile column that minimises rounds x (planes
 // per chunk + 1), the time model of such a launch (each round runs its
 // workgroups side by side; a chunk takes a time proportional to its planes).
-Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks) {
+Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks, int extra = 1) {
     const int pblocks = (plane + tile - 1) / tile;
     int best = 1;
     int64_t best_cost = -1;
@@ -1492,7 +1766,7 @@ Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks)
         if (nchunk != n) continue;                       // same chunking as a smaller n
         const int64_t blocks = (int64_t)pblocks * nchunk;
         const int64_t rounds = (blocks + max_blocks - 1) / max_blocks;
-        const int64_t cost = rounds * (xchunk + 1);
+        const int64_t cost = rounds * (xchunk + extra);
         if (best_cost < 0 || cost < best_cost) {
             best_cost = cost;
             best = n;
@@ -1734,6 +2008,51 @@ int check_he_fields(const Plan& P, const fdtd_fields* f) {
     return 0;
 }
 
+bool he2_supported(const Plan& P) { return P.lhe2.mode == 2; }
+
+int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index) {
+    using T = float;
+    HE2Args<T> a;
+    memset(&a, 0, sizeof(a));
+    fill_geom<T>(P, P.lhe2, f, a.g);
+    const uint32_t nb = (uint32_t)(P.N * (int64_t)sizeof(T));
+    a.E = {(const T*)f->E, nb};
+    a.H = {(const T*)f->H, nb};
+    a.eps = {(const T*)f->eps, nb};
+    a.E_out = {(T*)f->E_out, nb};
+    a.H_out = {(T*)f->H_out, nb};
+    a.S = {(T*)f->S, nb};
+    a.psi_h_in = (const T*)f->psi_h;
+    a.psi_h_out = (T*)f->psi_h_out;
+    a.psi_e_in = (const T*)f->psi_e;
+    a.psi_e_out = (T*)f->psi_e_out;
+    a.coef_elems = (int)P.coef_elems;
+    a.src_amp = src_amp;
+    if (src_amp) {
+        const int64_t N = P.N;
+        const int64_t comp = src_index / N;
+        const int64_t i = src_index - comp * N;
+        a.src_comp = (int)comp;
+        a.src_x = (int)(i / P.plane);
+        a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
+    }
+    const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
+    const bool nu = P.d.nonuniform != 0;
+    if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0>), grid, dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0>), grid, dim3(1024), 0, st, a);
+    return check_launch("fdtd_update_he2");
+}
+
+int check_he2_fields(const Plan& P, const fdtd_fields* f) {
+    if (!he2_supported(P))
+        return fail(3, "fdtd_update_he2 needs FP32, fused Poynting, no Bloch boundaries, rows of at most 256 cells, "
+                       "at most %d CPML coefficients and no two overlapping slabs on one axis", kCoefTab);
+    if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he2 needs E_out != E");
+    if (P.d.n_face > 0 && (!f->psi_e_out || f->psi_e_out == f->psi_e))
+        return fail(2, "fdtd_update_he2 needs psi_e_out != psi_e");
+    return 0;
+}
+
 int check_fields(const Plan& P, const fdtd_fields* f, bool for_h) {
     if (!f) return fail(2, "fields is NULL");
     if (!f->E || !f->H || !f->eps) return fail(2, "E, H and eps are required");
@@ -1840,6 +2159,17 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
         P->lhe = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_HE_TARGET);
     }
+    // two-step pass: FP32, 1024-thread tiles of whole rows plus three halo
+    // rows, the coefficient block in LDS, slabs of one axis apart (their
+    // psi pairs are carried between the half-steps one slab per axis)
+    P->lhe2.mode = -1;
+    bool apart = true;
+    for (int k = 0; k < d.n_face; ++k)
+        for (int j = k + 1; j < d.n_face; ++j)
+            if (d.face[k].axis == d.face[j].axis && P->face[k].lo < P->face[j].hi && P->face[j].lo < P->face[k].hi)
+                apart = false;
+    if (FDTD_HE2 && P->lhe.mode == 2 && d.dtype == FDTD_F32 && P->sz <= 256 && P->coef_elems <= kCoefTab && apart)
+        P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
     *plan_out = P;
     return 0;
 }
@@ -1962,27 +2292,41 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
         }
         return rc;
     };
-    // Half-steps E_0 H_0 E_1 H_1 ... E_{n-1} H_{n-1}.  Fused: E_0, then
-    // (H_{k-1}, E_k) in one he_march each, then H_{n-1}.  Every he_march
-    // moves E to the other buffer, so their count is kept even (H_0, E_1
-    // unfused when n-1 is odd) and E ends in f->E.
+    // Half-steps E_0 H_0 E_1 H_1 ... E_{n-1} H_{n-1}.  Fused: E_0, then the
+    // m = n-1 pairs (H_{k-1}, E_k), then H_{n-1}.  A pair runs unfused, in
+    // one he_march, or two pairs in one he2_march.  Each he_march moves E,
+    // and each he2_march E and psi_e, to the other buffer and H once, so
+    // both counts are kept even (E and psi_e end in f->E and f->psi_e, and
+    // H swaps n times in all, as unfused): 2 * (m / 4) he2_march passes,
+    // then of the rest an even number of he_march and at most one unfused.
     const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P);
+    const bool fuse2 = fuse && he2_supported(P) && (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
     if (fuse) {
         if (int rc = half_e(0)) return rc;
         k = 1;
-        if ((n - 1) % 2 == 1) {
+        const int m = n - 1;
+        const int n2 = fuse2 ? 2 * (m / 4) : 0;
+        const int r = m - 2 * n2;
+        if (r % 2 == 1) {
             if (int rc = half_h()) return rc;
             if (int rc = half_e(1)) return rc;
             k = 2;
         }
-        for (; k < n; ++k) {
+        for (int j = 0; j < r / 2 * 2; ++j, ++k) {
             int rc = (P.d.dtype == FDTD_F32) ? launch_he<float>(P, &cur, st, (const float*)amp_of(k), src_index)
                                              : launch_he<double>(P, &cur, st, (const double*)amp_of(k), src_index);
             if (rc) return rc;
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
             std::swap(cur.E, cur.E_out);
+        }
+        for (int j = 0; j < n2; ++j, k += 2) {
+            if (int rc = launch_he2(P, &cur, st, (const float*)amp_of(k), src_index)) return rc;
+            std::swap(cur.H, cur.H_out);
+            std::swap(cur.psi_h, cur.psi_h_out);
+            std::swap(cur.E, cur.E_out);
+            std::swap(cur.psi_e, cur.psi_e_out);
         }
         return half_h();
     }
@@ -2007,6 +2351,20 @@ int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, co
 }
 
 int fdtd_update_he_supported(const void* plan) { return plan && he_supported(*(const Plan*)plan) ? 1 : 0; }
+
+int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,
+                    void* stream) {
+    if (!plan) return fail(1, "plan is NULL");
+    const Plan& P = *(const Plan*)plan;
+    if (int rc = check_fields(P, f, true)) return rc;
+    if (int rc = check_he_fields(P, f)) return rc;
+    if (int rc = check_he2_fields(P, f)) return rc;
+    if (src_amp && (src_index < 0 || src_index >= 3 * P.N))
+        return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
+    return launch_he2(P, f, (hipStream_t)stream, (const float*)src_amp, src_index);
+}
+
+int fdtd_update_he2_supported(const void* plan) { return plan && he2_supported(*(const Plan*)plan) ? 1 : 0; }
 
 }  // extern "C"
 

@@ -255,6 +255,11 @@ class Simulation:
         self._coef = _vec([numpy.concatenate(coefs)] if coefs else [numpy.zeros(1)], self._tdtype, self.device)
         zeros = lambda: torch.zeros(max(n_psi, 1), dtype=self._tdtype, device=self.device)
         self._psi_e = zeros()
+        # second psi_E block: step_n then runs two steps per pass where the
+        # plan allows it (fdtd_update_he2)
+        self._psi_e_out = None
+        if self._E_out is not None and self._lib.fdtd_update_he2_supported(plan):
+            self._psi_e_out = zeros()
         self._psi_h = [zeros()] + ([zeros()] if pingpong else [])
         self._psi_f = self._psi_g = None
         if self.bloch_boundaries:
@@ -335,6 +340,7 @@ class Simulation:
         f.coef = ptr(self._coef)
         f.inv_d = ptr(self._inv_d)
         f.E_out = ptr(self._E_out)
+        f.psi_e_out = ptr(self._psi_e_out)
         return f
 
     def _launch(self, fn, events, what, *extra):
@@ -394,6 +400,30 @@ class Simulation:
         self._E, self._E_out = self._E_out, self._E
         return ev
 
+    def update_HE2(self, events: Sequence = (), src_index: Optional[int] = None, src_amp=None) -> Event:
+        """``update_HE`` twice in one pass over the grid: H (with S), E,
+        ``E[src_index] += src_amp[0]``, H (with S), E, ``E[src_index] +=
+        src_amp[1]`` (``src_amp``: a two-value device tensor).  Needs a plan
+        that supports it (``two_step_passes``).  The new E is in a second
+        buffer afterwards: read ``sim.E`` again."""
+        if self._psi_e_out is None:
+            raise _lib.FdtdError("update_HE2 is not available for this simulation (fdtd_update_he2_supported)")
+        amp = None
+        if src_amp is not None:
+            if src_index is None:
+                raise Exception("src_index is required with src_amp")
+            amp = src_amp.to(device=self.device, dtype=self._tdtype).contiguous()
+            if amp.numel() < 2:
+                raise Exception("src_amp needs two values")
+            self._amp_keepalive = amp
+        ev = self._launch(self._lib.fdtd_update_he2, events, "fdtd_update_he2",
+                          ctypes.c_int64(int(src_index) if amp is not None else -1),
+                          ctypes.c_void_p(amp.data_ptr() if amp is not None else None))
+        self._h = 1 - self._h
+        self._E, self._E_out = self._E_out, self._E
+        self._psi_e, self._psi_e_out = self._psi_e_out, self._psi_e
+        return ev
+
     def step_n(self, n: int, src_index: Optional[int] = None, src_wave=None, events: Sequence = ()) -> Event:
         """``n`` iterations of the reference caller loop (fdtd.py:162-175)
         enqueued without host synchronisation: E (+F), ``E[src_index] +=
@@ -447,6 +477,11 @@ class Simulation:
     def fused_steps(self) -> bool:
         """True when step_n runs the fused H+E pass (see update_HE)."""
         return self._E_out is not None
+
+    @property
+    def two_step_passes(self) -> bool:
+        """True when step_n runs two steps per pass (see update_HE2)."""
+        return self._psi_e_out is not None
 
     @property
     def plan(self) -> ctypes.c_void_p:

@@ -1,5 +1,6 @@
-"""The fused H+E pass (fdtd_update_he, used by step_n) against the unfused
-kernels and the CPU oracle.
+"""The fused H+E pass (fdtd_update_he) and the two-step pass
+(fdtd_update_he2), both used by step_n, against the unfused kernels and the
+CPU oracle.
 
 The fused pass does, per cell, the same arithmetic in the same order as the
 H kernel followed by the E kernel, so every comparison here is bit-exact:
@@ -127,6 +128,41 @@ def test_update_he_equals_h_then_e(dtype):
         _assert_same(_state(a), _state(b), "update_HE amp %g" % amp)
 
 
+@pytest.mark.parametrize("shape, T, nonuniform", [
+    ((10, 12, 14), 3, False),
+    ((10, 12, 14), 3, True),
+    ((7, 3, 256), 1, False),     # longest rows the two-step pass takes: one owned row per tile
+    ((4, 2, 9), 1, False),       # two rows: the halo rows are owned rows of the same tile
+    ((4, 25, 40), 2, False),     # four planes, several tiles per column
+])
+def test_update_he2_equals_he_twice(shape, T, nonuniform):
+    """One fdtd_update_he2 = fdtd_update_he twice (sources of both steps),
+    bit for bit: fields, S and both psi blocks."""
+    print("case update_HE2", shape, T, nonuniform, flush=True)
+    eps, pmls, init, dxes = _case(shape, T, seed=11, nonuniform=nonuniform, pec=True)
+    a = _sim(eps, pmls, init, dxes, numpy.float32, True)
+    b = _sim(eps, pmls, init, dxes, numpy.float32, True)
+    assert a.two_step_passes
+    for s in (a, b):                             # non-zero psi first
+        for _ in range(3):
+            s.update_E([])
+            s.update_H([])
+        s.update_E([])
+    idx = 1 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 3
+    for amps in ((0.75, -1.5), (2.0, 0.25)):
+        t_amp = torch.tensor(amps, dtype=torch.float32, device="cuda")
+        a.update_HE2([], idx, t_amp)
+        b.update_HE([], idx, t_amp[0:1])
+        b.update_HE([], idx, t_amp[1:2])
+        torch.cuda.synchronize()
+        _assert_same(_state(a), _state(b), "update_HE2 amps %s" % (amps,))
+    a.update_HE2([])
+    b.update_HE([])
+    b.update_HE([])
+    torch.cuda.synchronize()
+    _assert_same(_state(a), _state(b), "update_HE2 without a source")
+
+
 @pytest.mark.parametrize("shape, T", [
     ((14, 11, 150), 4),      # rows of 150 cells (1024-thread tiles)
     ((6, 7, 5), 4),          # overlapping slabs on every axis
@@ -160,3 +196,8 @@ def test_fusion_is_off_where_unsupported():
     s = Simulation(eps, [], poynting_mode="two_pass")
     with pytest.raises(Exception):
         s.update_HE([])
+    assert Simulation(eps, []).two_step_passes
+    assert not Simulation(eps, [], float_type=numpy.float64).two_step_passes
+    assert not Simulation([numpy.ones((4, 4, 300))] * 3, []).two_step_passes
+    with pytest.raises(Exception):
+        Simulation(eps, [], float_type=numpy.float64).update_HE2([])

@@ -34,7 +34,7 @@ def test_struct_layouts_match_header():
     # fdtd_face: 4 int32; fdtd_desc: 3 int64 + 2 int32 + 12 faces + 2 double + 4 int32 + 6 double
     assert ctypes.sizeof(_lib.Face) == 16
     assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
-    assert ctypes.sizeof(_lib.Fields) == 18 * 8
+    assert ctypes.sizeof(_lib.Fields) == 19 * 8
     assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4
 
 
@@ -210,3 +210,62 @@ def test_update_he_rejects_bad_calls():
     rc, msg = call(poynting=1, h_out_same=True)
     assert rc != 0 and "H_out" in msg
     assert lib.fdtd_update_he(None, None, -1, None, None) != 0
+
+
+def test_update_he2_support_rules():
+    """The two-step pass needs FP32, fused Poynting, no Bloch boundaries,
+    rows of at most 256 cells, a coefficient block that fits its LDS copy
+    and the CPML slabs of one axis apart (plan-level query, no GPU)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+
+    def supported(shape=(10, 11, 12), faces=((0, 0, 2),), **kw):
+        d = _desc(shape, list(faces), **kw)
+        plan = ctypes.c_void_p()
+        assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+        r = lib.fdtd_update_he2_supported(plan)
+        lib.fdtd_plan_destroy(plan)
+        return r
+    six = [(a, p, 3) for a in range(3) for p in range(2)]
+    assert supported() == 1
+    assert supported(faces=six) == 1
+    assert supported(faces=()) == 1
+    assert supported(shape=(4, 4, 256)) == 1
+    assert supported(shape=(4, 4, 257)) == 0
+    assert supported(dtype=1) == 0
+    assert supported(poynting=0) == 0
+    assert supported(poynting=2) == 0
+    assert supported(bloch_mask=2) == 0
+    assert supported(shape=(6, 7, 5), faces=six) == 0          # both z slabs (3 of 5 cells) overlap
+    assert supported(shape=(6, 7, 6), faces=six) == 1          # 3 + 3 of 6 cells: apart
+    assert supported(shape=(8, 9, 200), faces=[(a, p, 120) for a in range(2, 3) for p in range(2)]) == 0
+    assert lib.fdtd_update_he2_supported(None) == 0
+
+
+def test_update_he2_rejects_bad_calls():
+    """fdtd_update_he2 validates before launching anything (fake device
+    pointers: no kernel runs on these paths)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+
+    def call(dtype=0, psi_e_out_same=False, e_out_same=False):
+        d = _desc((10, 11, 12), [(0, 0, 2)], dtype=dtype)
+        plan = ctypes.c_void_p()
+        assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+        f = _lib.Fields()
+        f.E, f.H, f.eps, f.S, f.oS = 0x1000, 0x2000, 0x3000, 0x4000, 0x5000
+        f.H_out = 0x6000
+        f.psi_e, f.psi_h, f.psi_h_out, f.coef = 0x7000, 0x8000, 0x9000, 0xa000
+        f.E_out = f.E if e_out_same else 0xb000
+        f.psi_e_out = f.psi_e if psi_e_out_same else 0xc000
+        rc = lib.fdtd_update_he2(plan, ctypes.byref(f), -1, None, None)
+        msg = lib.fdtd_last_error().decode()
+        lib.fdtd_plan_destroy(plan)
+        return rc, msg
+    rc, msg = call(dtype=1)
+    assert rc != 0 and "FP32" in msg
+    rc, msg = call(psi_e_out_same=True)
+    assert rc != 0 and "psi_e_out" in msg
+    rc, msg = call(e_out_same=True)
+    assert rc != 0 and "E_out" in msg
+    assert lib.fdtd_update_he2(None, None, -1, None, None) != 0
