@@ -34,9 +34,10 @@ def kernel_bytes(shape, pmls, kind, dtype_bytes=4, poynting=True):
     words/cell); H kernel = E r, H r, H w, S w (12 words/cell with Poynting,
     9 without); plus psi r+w (4 words) per CPML slab cell and face.  The
     fused H+E pass ("HE") = E r, H r, eps r, H w, S w, E w (18 words/cell)
-    plus both psi blocks (8 words per slab cell and face)."""
+    plus both psi blocks (8 words per slab cell and face).  The two-step
+    pass ("HE2") moves the same per launch, for two steps."""
     n = float(numpy.prod(shape))
-    if kind == "HE":
+    if kind in ("HE", "HE2"):
         words, psi_words = 18, 8
     else:
         words, psi_words = (12 if (kind == "E" or poynting) else 9), 4
@@ -265,18 +266,22 @@ def configs_report(seconds: float = 1.0):
         ms = a.elapsed_time(b)
         gcell = w.cells * n / (ms / 1e3) / 1e9
         fused = bool(sim.fused_steps)
+        two = bool(sim.two_step_passes)
         if fused:
-            # one he_march per step: 18 words per cell, 8 per slab cell and face
+            # one he_march per step (he2_march per two steps): 18 words per
+            # cell, 8 per slab cell and face, per pass
             nc = float(w.cells)
             slab = sum(min(int(p.get("thickness", 8)), w.shape["xyz".find(p["axis"])]) * nc /
                        w.shape["xyz".find(p["axis"])] for p in w.pmls)
-            bpc = 4.0 * (18 + 8 * slab / nc)
+            one = 4.0 * (18 + 8 * slab / nc)
+            bpc = one / 2 if two else one
         else:
-            bpc = workloads.algorithmic_bytes_per_cell(w.shape, w.pmls, poynting=w.do_poynting,
-                                                       bloch=bool(w.bloch_boundaries))
+            bpc = one = workloads.algorithmic_bytes_per_cell(w.shape, w.pmls, poynting=w.do_poynting,
+                                                             bloch=bool(w.bloch_boundaries))
         row = {"workload": w.name, "shape": list(w.shape), "steps": n, "ms_per_step": ms / n,
-               "gcell_updates_s": gcell, "fused_pass": fused, "bytes_per_cell_update": bpc,
-               "hbm_frac": gcell * bpc / HBM_PEAK_GBS}
+               "gcell_updates_s": gcell, "fused_pass": fused, "steps_per_pass": 2 if two else 1,
+               "bytes_per_cell_update": bpc, "hbm_frac": gcell * bpc / HBM_PEAK_GBS,
+               "hbm_frac_one_step_pass_equivalent": gcell * one / HBM_PEAK_GBS}
         print(json.dumps(row), flush=True)
         out.append(row)
     return out
@@ -356,7 +361,7 @@ def main():
         a.record(stream)
         b.record(stream)
         empty.append((a, b))
-    dur = {"E": [], "H": [], "HE": []}
+    dur = {"E": [], "H": [], "HE": [], "HE2": []}
     evs = []
     for _ in range(args.kernel_steps):
         a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -378,6 +383,17 @@ def main():
             sim.update_HE([], w.src_index, amp)
             b.record(stream)
             hevs.append((a, b))
+    # the two-step pass step_n runs for most steps (fdtd_update_he2)
+    two = sim.two_step_passes
+    h2evs = []
+    if two:
+        amp2 = wave[:2]
+        for _ in range(args.kernel_steps):
+            a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+            a.record(stream)
+            sim.update_HE2([], w.src_index, amp2)
+            b.record(stream)
+            h2evs.append((a, b))
     torch.cuda.synchronize()
     marker_ms = float(numpy.median([a.elapsed_time(b) for a, b in empty]))
     for a, b, c in evs:
@@ -385,23 +401,31 @@ def main():
         dur["H"].append(b.elapsed_time(c) - marker_ms)
     for a, b in hevs:
         dur["HE"].append(a.elapsed_time(b) - marker_ms)
+    for a, b in h2evs:
+        dur["HE2"].append(a.elapsed_time(b) - marker_ms)
     dbytes = 4 if dtype == numpy.float32 else 8
     kern = {}
-    for k in ("E", "H", "HE"):
+    for k in ("E", "H", "HE", "HE2"):
         if not dur[k]:
             continue
         avg_ms = float(numpy.mean(dur[k]))
         byts = kernel_bytes(w.shape, w.pmls, k, dbytes, w.do_poynting)
         kern[k] = {"avg_us": avg_ms * 1e3, "bytes": byts, "GBs": byts / (avg_ms / 1e3) / 1e9,
                    "marker_us_subtracted": marker_ms * 1e3}
-    # the dominant kernel: the fused pass when step_n uses it (one launch
-    # per step), else the slower of the two half-steps
-    dom = "HE" if fused else max(kern, key=lambda k: kern[k]["avg_us"])
+    # the dominant kernel: the two-step pass when step_n uses it (one launch
+    # per two steps), else the fused pass (one launch per step), else the
+    # slower of the two half-steps
+    dom = "HE2" if two else "HE" if fused else max(kern, key=lambda k: kern[k]["avg_us"])
     print("bench: kernels %s" % json.dumps(kern), file=sys.stderr, flush=True)
 
-    step_bytes = kern["HE"]["bytes"] if fused else kern["E"]["bytes"] + kern["H"]["bytes"]
+    # algorithmic bytes per cell update of the pass step_n runs, and of the
+    # one-step fused pass (what a pass per step would have to move)
+    step_bytes = (kern["HE2"]["bytes"] / 2 if two else kern["HE"]["bytes"] if fused
+                  else kern["E"]["bytes"] + kern["H"]["bytes"])
+    one_pass_bytes = kern["HE"]["bytes"] if fused else kern["E"]["bytes"] + kern["H"]["bytes"]
     names = {"E": ("e_march (E)", "e_march"), "H": ("h_march (H+S)", "h_march"),
-             "HE": ("he_march (H+S, then E)", "he_march")}
+             "HE": ("he_march (H+S, then E)", "he_march"),
+             "HE2": ("he2_march (two steps: H+S, E, H+S, E)", "he2_march")}
     result = {
         "metric": "Gcell-updates/s + %HBM roofline; learned vs 16-cell CPML at -40 dB reflection",
         "value": gcell,
@@ -417,13 +441,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "shape": list(w.shape), "pml": "CPML-8 on 6 faces",
                    "poynting": w.do_poynting, "cells": w.cells,
+                   "steps_per_pass": 2 if two else 1,
                    "bytes_per_cell_update": step_bytes / w.cells,
-                   "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS},
+                   "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS,
+                   "hbm_frac_one_step_pass_equivalent": gcell * one_pass_bytes / w.cells / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "kernel": names[dom][0],
                      "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, names[dom][1]),
                      "avg_us": kern[dom]["avg_us"], "bytes_per_launch": kern[dom]["bytes"],
+                     "steps_per_launch": 2 if dom == "HE2" else 1,
                      "other_kernel": {k: v for k, v in kern.items() if k != dom}},
     }
     if not args.no_cpu:

@@ -98,8 +98,11 @@ namespace {
 #ifndef FDTD_HE2
 #define FDTD_HE2 1                   // two-step pass available (FP32)
 #endif
+#ifndef FDTD_HE2_LDSCARRY
+#define FDTD_HE2_LDSCARRY 1          // two-step pass: H, E, eps of the previous plane read back from LDS
+#endif
 #ifndef FDTD_HE2_S2
-#define FDTD_HE2_S2 1                // two-step pass: also store S of its first H half-step
+#define FDTD_HE2_S2 0                // two-step pass: also store S of its first H half-step
 #endif
 #ifndef FDTD_HE_TARGET
 #define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
@@ -734,12 +737,17 @@ __device__ __forceinline__ void e_value(const Geom<T>& g, const S& s, const Cell
         cpml_pre<T, true>(g, x, c, hit, kc, tab, *q, s.psi, s.psi, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
                           keep);
     else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
-    ex = v.ex + g.dt / eps0 * (dzy - dyz + px);
-    ey = v.ey + g.dt / eps1 * (dxz - dzx + py);
-    ez = v.ez + g.dt / eps2 * (dyx - dxy + pz);
-    if (src_comp == 0) ex = ex + amp;
-    else if (src_comp == 1) ey = ey + amp;
-    else if (src_comp == 2) ez = ez + amp;
+    // (locals, assigned once at the end: a conditional store through one of
+    // the three references would put them in a scratch array)
+    T nx = v.ex + g.dt / eps0 * (dzy - dyz + px);
+    T ny = v.ey + g.dt / eps1 * (dxz - dzx + py);
+    T nz = v.ez + g.dt / eps2 * (dyx - dxy + pz);
+    if (src_comp == 0) nx = nx + amp;
+    else if (src_comp == 1) ny = ny + amp;
+    else if (src_comp == 2) nz = nz + amp;
+    ex = nx;
+    ey = ny;
+    ez = nz;
 }
 
 template <class T, bool PRE = false, class S, class K = const PsiHit*>
@@ -1448,7 +1456,11 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
 // of one workgroup are owned cells of another.  Per cell the arithmetic is
 // that of h_update + S and e_update, in the same order, so the results are
 // bit-identical to two he_march passes (or to update_H, update_E, ...).
-// S2 also stores S of the first H half-step (which the second overwrites).
+// S is stored for the second H half-step: the S of the first is overwritten
+// by the same pass, at the same address and by the same thread, before
+// anything can read it, so the pass leaves every buffer as two he_march
+// passes do.  S2 stores it anyway (one more 12-byte store per cell; 6 %
+// slower on a 395 x 345 x 73 grid).
 // ---------------------------------------------------------------------------
 template <class T>
 struct HE2Args {
@@ -1464,10 +1476,13 @@ struct HE2Args {
     int coef_elems;
 };
 
-template <class T, bool NU, bool S2>
+// LC: H^{n+1/2}, E^{n+1} and eps of plane x-1 are read back from LDS (by
+// plane parity) instead of being carried in registers (fewer spills).
+template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0)>
 __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
     constexpr int HB = 1024;
-    __shared__ T la[S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
+    __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
+    __shared__ T lp[LC ? 2 : 1][3][LC ? HB : 1];   // eps
     __shared__ T le[2][3][HB];         // E^{n+1} by plane parity
     __shared__ T lb[7][HB];            // oS products of H^{n+3/2}, then H^{n+3/2} (x, y, z)
     __shared__ T ctab[kCoefTab];
@@ -1551,6 +1566,7 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         }
         h_fetch<T, false>(g, hA, c, xa, pbytes, in);
         const int b = it & 1;
+        const int ba = LC ? b : 0;
         // ---- first stage: H^{n+1/2}(xa) on every row
         const T ixa = NU ? g.inv_e[0][xa] : g.inv_u;
         PsiPair<T> nkh, nke;
@@ -1570,14 +1586,14 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
                 oAzy = -h.aEyz * aHx;
             }
         }
-        la[0][t] = g1x;
-        la[1][t] = g1y;
-        la[2][t] = g1z;
+        la[ba][0][t] = g1x;
+        la[ba][1][t] = g1y;
+        la[ba][2][t] = g1z;
         if (S2) {
-            la[3][t] = oAxy;
-            la[4][t] = oAzy;
-            la[5][t] = oAxz;
-            la[6][t] = oAyz;
+            la[ba][3][t] = oAxy;
+            la[ba][4][t] = oAzy;
+            la[ba][5][t] = oAxz;
+            la[ba][6][t] = oAyz;
         }
         lds_barrier();
         // ---- first stage: S(xa) (S2) and E^{n+1}(xa) on rows y0-1 .. y1
@@ -1585,19 +1601,23 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         if (rowe && ea) {
             if (S2 && own && hb && it < n_it - 1) {
                 const uint32_t oi = (uint32_t)xa * pbytes + c.op;
-                stc<FDTD_NT_H>(a.S, 0, oi, (oAxy + oAxz + la[3][tym] + la[5][tzm]) * sf);
-                stc<FDTD_NT_H>(a.S, 1, oi, (oAyz + oAyx + la[6][tzm] + oAyx_m) * sf);
-                stc<FDTD_NT_H>(a.S, 2, oi, (oAzx + oAzy + oAzx_m + la[4][tym]) * sf);
+                stc<FDTD_NT_H>(a.S, 0, oi, (oAxy + oAxz + la[ba][3][tym] + la[ba][5][tzm]) * sf);
+                stc<FDTD_NT_H>(a.S, 1, oi, (oAyz + oAyx + la[ba][6][tzm] + oAyx_m) * sf);
+                stc<FDTD_NT_H>(a.S, 2, oi, (oAzx + oAzy + oAzx_m + la[ba][4][tym]) * sf);
             }
             EIn<T> v;
             v.hx = g1x; v.hy = g1y; v.hz = g1z;
-            v.hx_ym = la[0][tym];
-            v.hz_ym = la[2][tym];
-            v.hx_zm = la[0][tzm];
-            v.hy_zm = la[1][tzm];
+            v.hx_ym = la[ba][0][tym];
+            v.hz_ym = la[ba][2][tym];
+            v.hx_zm = la[ba][0][tzm];
+            v.hy_zm = la[ba][1][tzm];
             v.ex = qx; v.ey = qy; v.ez = qz;
             const T jxa = NU ? g.inv_h[0][xa] : g.inv_u;
             const bool src = src_comp >= 0 && xa == a.src_x;
+            if constexpr (LC) {
+                h1y = la[b ^ 1][1][t];
+                h1z = la[b ^ 1][2][t];
+            }
             e_value<T, true>(g, eA, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1,
                              src ? a.src_amp[0] : (T)0, (const CpmlCols<T>*)&ka, nullptr, &qe, tab, false, &nke, n1x,
                              n1y, n1z);
@@ -1605,6 +1625,11 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         le[b][0][t] = n1x;
         le[b][1][t] = n1y;
         le[b][2][t] = n1z;
+        if constexpr (LC) {
+            lp[b][0][t] = e0;
+            lp[b][1][t] = e1;
+            lp[b][2][t] = e2;
+        }
         // ---- second stage: H^{n+3/2}(xb) on rows y0-1 .. y1-1
         T g2x = (T)0, g2y = (T)0, g2z = (T)0;
         T oBxy = (T)0, oBxz = (T)0, oByz = (T)0, oByx = (T)0, oBzx = (T)0, oBzy = (T)0;
@@ -1615,6 +1640,14 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
             v.ez_y = le[b ^ 1][2][typ];
             v.ex_z = le[b ^ 1][0][tzp];
             v.ey_z = le[b ^ 1][1][tzp];
+            if constexpr (LC) {
+                h1x = la[b ^ 1][0][t];
+                h1y = la[b ^ 1][1][t];
+                h1z = la[b ^ 1][2][t];
+                f1x = le[b ^ 1][0][t];
+                f1y = le[b ^ 1][1][t];
+                f1z = le[b ^ 1][2][t];
+            }
             v.hx = h1x; v.hy = h1y; v.hz = h1z;
             const T ixb = NU ? g.inv_e[0][xb] : g.inv_u;
             const HCell<T> h = h_update<T, false, true>(g, hB, c, xb, pbytes, f1x, f1y, f1z, v, ixb, iy, iz,
@@ -1648,7 +1681,16 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
             v.hz_ym = lb[6][tym];
             v.hx_zm = lb[4][tzm];
             v.hy_zm = lb[5][tzm];
-            v.ex = f1x; v.ey = f1y; v.ez = f1z;
+            if constexpr (LC) {
+                v.ex = le[b ^ 1][0][t];
+                v.ey = le[b ^ 1][1][t];
+                v.ez = le[b ^ 1][2][t];
+                ep0 = lp[b ^ 1][0][t];
+                ep1 = lp[b ^ 1][1][t];
+                ep2 = lp[b ^ 1][2][t];
+            } else {
+                v.ex = f1x; v.ey = f1y; v.ez = f1z;
+            }
             const T jxb = NU ? g.inv_h[0][xb] : g.inv_u;
             const bool src = src_comp >= 0 && xb == a.src_x;
             e_update<T, true>(g, eB, c, xb, oi, v, h2y_m, h2z_m, ep0, ep1, ep2, jxb, jy, jz, src ? src_comp : -1,
@@ -1657,10 +1699,13 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         // ---- carry to the next plane
         h2y_m = g2y; h2z_m = g2z;
         oByx_m = oByx; oBzx_m = oBzx;
-        h1x = g1x; h1y = g1y; h1z = g1z;
-        f1x = n1x; f1y = n1y; f1z = n1z;
-        kh = nkh; ke = nke;
-        ep0 = e0; ep1 = e1; ep2 = e2;
+        if constexpr (!LC) {
+            h1x = g1x; h1y = g1y; h1z = g1z;
+            f1x = n1x; f1y = n1y; f1z = n1z;
+            ep0 = e0; ep1 = e1; ep2 = e2;
+        }
+        kh = nkh;
+        ke = nke;
         if (S2) { oAyx_m = oAyx; oAzx_m = oAzx; }
         qx = in.exn; qy = in.eyn; qz = in.ezn;
         xa = xn;
