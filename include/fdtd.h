@@ -186,6 +186,12 @@ int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, c
    Bloch boundaries, rows of at most 256 cells, CPML slabs of one axis
    apart), else 0. */
 int fdtd_update_he2_supported(const void* plan);
+/* 1 when fdtd_step_n runs two steps per pass with this plan (given E_out and
+   psi_e_out): fdtd_update_he2 is supported and its time model (rounds of
+   workgroups x planes per chunk) beats the one-step fused pass's, else 0.
+   Long rows leave the two-step tiles few owned rows; the one-step pass is
+   then kept. */
+int fdtd_update_he2_preferred(const void* plan);
 
 /* ------------------------------------------------------------------------
  * Discrete adjoint of the step and FP64 reductions.  No counterpart in the

@@ -24,6 +24,7 @@ EXPORTS = (
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n",
     "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
+    "fdtd_update_he2_preferred",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
     "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems",
     "fdtd_shell_apply", "fdtd_shell_step_n", "fdtd_shell_backward",
@@ -101,6 +102,7 @@ def load() -> ctypes.CDLL:
     lib.fdtd_update_he_supported.argtypes = [P]
     lib.fdtd_update_he2.argtypes = [P, F, ctypes.c_int64, P, P]
     lib.fdtd_update_he2_supported.argtypes = [P]
+    lib.fdtd_update_he2_preferred.argtypes = [P]
     lib.fdtd_adjoint_step.argtypes = [P, ctypes.POINTER(AdjointBufs), P]
     lib.fdtd_plane_sum.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_reduce_partials.argtypes = [P]

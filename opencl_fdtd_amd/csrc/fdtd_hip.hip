@@ -101,6 +101,9 @@ namespace {
 #ifndef FDTD_HE2_LDSCARRY
 #define FDTD_HE2_LDSCARRY 1          // two-step pass: H, E, eps of the previous plane read back from LDS
 #endif
+#ifndef FDTD_HE2_COST
+#define FDTD_HE2_COST 12             // two-step pass: plane iteration cost / fused pass's, x10
+#endif
 #ifndef FDTD_HE2_S2
 #define FDTD_HE2_S2 0                // two-step pass: also store S of its first H half-step
 #endif
@@ -1772,6 +1775,7 @@ struct Plan {
     Launch le, lh;
     Launch lhe;          // fused H+E pass (mode 2 when available, else mode -1)
     Launch lhe2;         // two-step pass (mode 2 when available, else mode -1)
+    bool he2_pref;       // step_n runs the two-step pass (its cost model beats the fused pass's)
     int64_t psi_elems, coef_elems;
     struct F { int lo, hi, psi_off, face_elems, coef_off; } face[FDTD_MAX_FACES];
 };
@@ -2215,6 +2219,19 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
                 apart = false;
     if (FDTD_HE2 && P->lhe.mode == 2 && d.dtype == FDTD_F32 && P->sz <= 256 && P->coef_elems <= kCoefTab && apart)
         P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
+    // Time per step of each pass: rounds of workgroups x planes a chunk
+    // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
+    // of the fused pass's (1.2 measured on MI355X with rows of 73 cells) and
+    // covering two steps.  Long rows leave the two-step tiles few owned rows
+    // of their 1024 threads; the cost model then keeps the fused pass.
+    P->he2_pref = false;
+    if (P->lhe2.mode == 2) {
+        const auto cost = [&](const Launch& L, int extra) {
+            const int64_t blocks = (int64_t)L.pblocks * L.nchunk;
+            return (double)((blocks + FDTD_HE_TARGET - 1) / FDTD_HE_TARGET) * (L.xchunk + extra);
+        };
+        P->he2_pref = cost(P->lhe2, 3) * (FDTD_HE2_COST / 10.0) / 2.0 < cost(P->lhe, 1);
+    }
     *plan_out = P;
     return 0;
 }
@@ -2345,7 +2362,8 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     // H swaps n times in all, as unfused): 2 * (m / 4) he2_march passes,
     // then of the rest an even number of he_march and at most one unfused.
     const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P);
-    const bool fuse2 = fuse && he2_supported(P) && (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
+    const bool fuse2 = fuse && he2_supported(P) && P.he2_pref &&
+                       (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
     if (fuse) {
         if (int rc = half_e(0)) return rc;
@@ -2410,6 +2428,10 @@ int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, c
 }
 
 int fdtd_update_he2_supported(const void* plan) { return plan && he2_supported(*(const Plan*)plan) ? 1 : 0; }
+
+int fdtd_update_he2_preferred(const void* plan) {
+    return plan && he2_supported(*(const Plan*)plan) && ((const Plan*)plan)->he2_pref ? 1 : 0;
+}
 
 }  // extern "C"
 

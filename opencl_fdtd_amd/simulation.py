@@ -404,8 +404,9 @@ class Simulation:
         """``update_HE`` twice in one pass over the grid: H (with S), E,
         ``E[src_index] += src_amp[0]``, H (with S), E, ``E[src_index] +=
         src_amp[1]`` (``src_amp``: a two-value device tensor).  Needs a plan
-        that supports it (``two_step_passes``).  The new E is in a second
-        buffer afterwards: read ``sim.E`` again."""
+        that supports it (FP32, fused Poynting, no Bloch boundaries, sz <=
+        256, CPML slabs of one axis apart).  The new E is in a second buffer
+        afterwards: read ``sim.E`` again."""
         if self._psi_e_out is None:
             raise _lib.FdtdError("update_HE2 is not available for this simulation (fdtd_update_he2_supported)")
         amp = None
@@ -480,8 +481,9 @@ class Simulation:
 
     @property
     def two_step_passes(self) -> bool:
-        """True when step_n runs two steps per pass (see update_HE2)."""
-        return self._psi_e_out is not None
+        """True when step_n runs two steps per pass (see update_HE2): the
+        plan supports it and its time model beats the one-step pass."""
+        return self._psi_e_out is not None and bool(self._lib.fdtd_update_he2_preferred(self._plan))
 
     @property
     def plan(self) -> ctypes.c_void_p:

@@ -142,7 +142,7 @@ def test_update_he2_equals_he_twice(shape, T, nonuniform):
     eps, pmls, init, dxes = _case(shape, T, seed=11, nonuniform=nonuniform, pec=True)
     a = _sim(eps, pmls, init, dxes, numpy.float32, True)
     b = _sim(eps, pmls, init, dxes, numpy.float32, True)
-    assert a.two_step_passes
+    assert a._psi_e_out is not None
     for s in (a, b):                             # non-zero psi first
         for _ in range(3):
             s.update_E([])
@@ -163,17 +163,20 @@ def test_update_he2_equals_he_twice(shape, T, nonuniform):
     _assert_same(_state(a), _state(b), "update_HE2 without a source")
 
 
-@pytest.mark.parametrize("shape, T", [
-    ((14, 11, 150), 4),      # rows of 150 cells (1024-thread tiles)
-    ((6, 7, 5), 4),          # overlapping slabs on every axis
-    ((23, 30, 73), 8),       # README row length, CPML-8
+@pytest.mark.parametrize("shape, T, two", [
+    ((14, 11, 150), 4, False),       # rows of 150 cells (1024-thread tiles)
+    ((6, 7, 5), 4, False),           # overlapping slabs on every axis
+    ((23, 30, 73), 8, False),        # README row length, CPML-8
+    ((120, 120, 73), 8, True),       # the same rows, a grid the two-step pass is chosen for
 ])
-def test_fused_step_n_matches_oracle(shape, T):
-    """step_n through the fused pass against the oracle, with a source."""
-    print("case fused vs oracle", shape, T, flush=True)
+def test_fused_step_n_matches_oracle(shape, T, two):
+    """step_n through the fused pass (and the two-step pass) against the
+    oracle, with a source."""
+    print("case fused vs oracle", shape, T, two, flush=True)
     eps, pmls, init, _ = _case(shape, T, seed=9, pec=True)
     sim, orc = build_pair(eps, pmls, init=init)
     assert sim._E_out is not None
+    assert sim.two_step_passes == two
     idx = 2 * sim.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
     wave = numpy.cos(0.2 * numpy.arange(21))
     sim.step_n(21, idx, wave.astype(numpy.float32)).wait()
@@ -196,8 +199,9 @@ def test_fusion_is_off_where_unsupported():
     s = Simulation(eps, [], poynting_mode="two_pass")
     with pytest.raises(Exception):
         s.update_HE([])
-    assert Simulation(eps, []).two_step_passes
+    assert Simulation(eps, [])._psi_e_out is not None
+    assert Simulation(eps, [], float_type=numpy.float64)._psi_e_out is None
+    assert Simulation([numpy.ones((4, 4, 300))] * 3, [])._psi_e_out is None
     assert not Simulation(eps, [], float_type=numpy.float64).two_step_passes
-    assert not Simulation([numpy.ones((4, 4, 300))] * 3, []).two_step_passes
     with pytest.raises(Exception):
         Simulation(eps, [], float_type=numpy.float64).update_HE2([])

@@ -242,6 +242,26 @@ def test_update_he2_support_rules():
     assert lib.fdtd_update_he2_supported(None) == 0
 
 
+def test_update_he2_preference():
+    """step_n takes the two-step pass where its time model beats the
+    one-step pass: rows of 73 cells (11 of a tile's 14 rows owned), not
+    rows of 250 (1 of 4)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+
+    def pref(shape, t):
+        d = _desc(shape, [(a, p, t) for a in range(3) for p in range(2)])
+        plan = ctypes.c_void_p()
+        assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+        r = (lib.fdtd_update_he2_supported(plan), lib.fdtd_update_he2_preferred(plan))
+        lib.fdtd_plan_destroy(plan)
+        return r
+    assert pref((395, 345, 73), 8) == (1, 1)
+    assert pref((64, 64, 250), 8) == (1, 0)
+    assert pref((64, 64, 300), 8) == (0, 0)
+    assert lib.fdtd_update_he2_preferred(None) == 0
+
+
 def test_update_he2_rejects_bad_calls():
     """fdtd_update_he2 validates before launching anything (fake device
     pointers: no kernel runs on these paths)."""
