@@ -101,8 +101,11 @@ namespace {
 #ifndef FDTD_HE2_LDSCARRY
 #define FDTD_HE2_LDSCARRY 1          // two-step pass: H, E, eps of the previous plane read back from LDS
 #endif
+#ifndef FDTD_HE2_EARLY
+#define FDTD_HE2_EARLY 0             // two-step pass: next plane's loads issued before the last stage
+#endif
 #ifndef FDTD_HE2_COST
-#define FDTD_HE2_COST 12             // two-step pass: plane iteration cost / fused pass's, x10
+#define FDTD_HE2_COST 13             // two-step pass: plane iteration cost / fused pass's, x10
 #endif
 #ifndef FDTD_HE2_S2
 #define FDTD_HE2_S2 0                // two-step pass: also store S of its first H half-step
@@ -1481,7 +1484,9 @@ struct HE2Args {
 
 // LC: H^{n+1/2}, E^{n+1} and eps of plane x-1 are read back from LDS (by
 // plane parity) instead of being carried in registers (fewer spills).
-template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0)>
+// EARLY: the loads of plane x+1 are issued after the second barrier, so
+// they are in flight while the second stage finishes plane x-1.
+template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0), bool EARLY = (FDTD_HE2_EARLY != 0)>
 __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
     constexpr int HB = 1024;
     __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
@@ -1546,28 +1551,43 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
     T ep0 = (T)1, ep1 = (T)1, ep2 = (T)1;
     T oAyx_m = (T)0, oAzx_m = (T)0;
     T h2y_m = (T)0, h2z_m = (T)0, oByx_m = (T)0, oBzx_m = (T)0;
+    // everything the first stage loads for plane xf (the E-step part when ef)
+    struct Loads {
+        HIn<T> h;
+        PsiPair<T> qh, qe;    // (left unset where the cell is in no slab: only read where it is)
+        T e0, e1, e2;
+    };
+    const auto fetch = [&](int xf, bool ef, Loads& L) {
+        CpmlCols<T> kf = cols;
+        kf.hx = slab_hit(g, 0, xf, g.sx);
+        L.e0 = L.e1 = L.e2 = (T)1;
+        if (valid) psi_fetch_cols(g, a.psi_h_in, xf, c, kf, L.qh);
+        if (rowe && ef) {
+            const uint32_t of = (uint32_t)xf * pbytes + c.op;
+            psi_fetch_cols(g, a.psi_e_in, xf, c, kf, L.qe);
+            L.e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
+            L.e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
+            L.e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
+        }
+        h_fetch<T, false>(g, hA, c, xf, pbytes, L.h);
+    };
+    Loads cur;
+    if (EARLY) fetch(xa, false, cur);
     for (int it = 0; it < n_it; ++it) {
         const int xb = (xa == 0) ? g.sx - 1 : xa - 1;
         const int xn = (xa == g.sx - 1) ? 0 : xa + 1;
         const bool ea = it >= 1;                 // E^{n+1} at xa
         const bool hb = it >= 2;                 // H^{n+3/2} at xb
         const bool wb = it >= 3;                 // xb is in the chunk: stores
-        // ---- loads of plane xa
+        // ---- loads of plane xa (issued one iteration earlier with EARLY)
         CpmlCols<T> ka = cols, kb = cols;
         ka.hx = slab_hit(g, 0, xa, g.sx);
         kb.hx = slab_hit(g, 0, xb, g.sx);
-        HIn<T> in;
-        PsiPair<T> qh, qe;
-        T e0 = (T)1, e1 = (T)1, e2 = (T)1;
-        if (valid) psi_fetch_cols(g, a.psi_h_in, xa, c, ka, qh);
-        if (rowe && ea) {
-            const uint32_t of = (uint32_t)xa * pbytes + c.op;
-            psi_fetch_cols(g, a.psi_e_in, xa, c, ka, qe);
-            e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
-            e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
-            e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
-        }
-        h_fetch<T, false>(g, hA, c, xa, pbytes, in);
+        if (!EARLY) fetch(xa, ea, cur);
+        const HIn<T>& in = cur.h;
+        const PsiPair<T>& qh = cur.qh;
+        const PsiPair<T>& qe = cur.qe;
+        const T e0 = cur.e0, e1 = cur.e1, e2 = cur.e2;
         const int b = it & 1;
         const int ba = LC ? b : 0;
         // ---- first stage: H^{n+1/2}(xa) on every row
@@ -1672,6 +1692,8 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         lb[5][t] = g2y;
         lb[6][t] = g2z;
         lds_barrier();
+        Loads nxt;
+        if (EARLY && it + 1 < n_it) fetch(xn, true, nxt);
         // ---- second stage: S(xb) and E^{n+2}(xb) on the owned rows
         if (own && wb) {
             const uint32_t oi = (uint32_t)xb * pbytes + c.op;
@@ -1711,6 +1733,7 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         ke = nke;
         if (S2) { oAyx_m = oAyx; oAzx_m = oAzx; }
         qx = in.exn; qy = in.eyn; qz = in.ezn;
+        if (EARLY) cur = nxt;
         xa = xn;
     }
 }
@@ -2221,8 +2244,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
     // Time per step of each pass: rounds of workgroups x planes a chunk
     // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
-    // of the fused pass's (1.2 measured on MI355X with rows of 73 cells) and
-    // covering two steps.  Long rows leave the two-step tiles few owned rows
+    // of the fused pass's (measured on MI355X: 1.20 with rows of 73 cells,
+    // 1.29 with rows of 128) and covering two steps.  Long rows leave the two-step tiles few owned rows
     // of their 1024 threads; the cost model then keeps the fused pass.
     P->he2_pref = false;
     if (P->lhe2.mode == 2) {
