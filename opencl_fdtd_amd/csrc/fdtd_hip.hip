@@ -726,8 +726,9 @@ __device__ __forceinline__ void e_fetch(const Geom<T>& g, const ESys<T>& s, cons
 
 // The E update of cell (x, c) without storing E: the new E (with the source
 // added) in ex, ey, ez.  psi is written in place when write is set and the
-// new pairs are kept in *keep when keep is set.
-template <class T, bool PRE, class S, class K>
+// new pairs are kept in *keep when keep is set.  QUOT: eps0..2 already hold
+// the quotients dt / eps (the same IEEE division, done by the caller).
+template <class T, bool PRE, class S, class K, bool QUOT = false>
 __device__ __forceinline__ void e_value(const Geom<T>& g, const S& s, const Cell& c, int x, const EIn<T>& v,
                                         T hy_xm, T hz_xm, T eps0, T eps1, T eps2, T ix, T iy, T iz, int src_comp,
                                         T amp, K hit, const FaceCoef<T>* kc, const PsiPair<T>* q, const T* tab,
@@ -745,9 +746,12 @@ __device__ __forceinline__ void e_value(const Geom<T>& g, const S& s, const Cell
     else cpml_cell<T, true>(g, x, c.y, c.z, c.p, s.psi, s.psi, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz);
     // (locals, assigned once at the end: a conditional store through one of
     // the three references would put them in a scratch array)
-    T nx = v.ex + g.dt / eps0 * (dzy - dyz + px);
-    T ny = v.ey + g.dt / eps1 * (dxz - dzx + py);
-    T nz = v.ez + g.dt / eps2 * (dyx - dxy + pz);
+    const T k0 = QUOT ? eps0 : g.dt / eps0;
+    const T k1 = QUOT ? eps1 : g.dt / eps1;
+    const T k2 = QUOT ? eps2 : g.dt / eps2;
+    T nx = v.ex + k0 * (dzy - dyz + px);
+    T ny = v.ey + k1 * (dxz - dzx + py);
+    T nz = v.ez + k2 * (dyx - dxy + pz);
     if (src_comp == 0) nx = nx + amp;
     else if (src_comp == 1) ny = ny + amp;
     else if (src_comp == 2) nz = nz + amp;
@@ -1490,7 +1494,7 @@ template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0), bool EA
 __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
     constexpr int HB = 1024;
     __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
-    __shared__ T lp[LC ? 2 : 1][3][LC ? HB : 1];   // eps
+    __shared__ T lp[LC ? 2 : 1][3][LC ? HB : 1];   // dt / eps
     __shared__ T le[2][3][HB];         // E^{n+1} by plane parity
     __shared__ T lb[7][HB];            // oS products of H^{n+3/2}, then H^{n+3/2} (x, y, z)
     __shared__ T ctab[kCoefTab];
@@ -1587,7 +1591,9 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
         const HIn<T>& in = cur.h;
         const PsiPair<T>& qh = cur.qh;
         const PsiPair<T>& qe = cur.qe;
-        const T e0 = cur.e0, e1 = cur.e1, e2 = cur.e2;
+        // dt / eps of plane xa, divided once for both E half-steps (the
+        // same IEEE quotient the E update forms)
+        const T e0 = g.dt / cur.e0, e1 = g.dt / cur.e1, e2 = g.dt / cur.e2;
         const int b = it & 1;
         const int ba = LC ? b : 0;
         // ---- first stage: H^{n+1/2}(xa) on every row
@@ -1641,9 +1647,9 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
                 h1y = la[b ^ 1][1][t];
                 h1z = la[b ^ 1][2][t];
             }
-            e_value<T, true>(g, eA, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1,
-                             src ? a.src_amp[0] : (T)0, (const CpmlCols<T>*)&ka, nullptr, &qe, tab, false, &nke, n1x,
-                             n1y, n1z);
+            e_value<T, true, ESysP<T>, const CpmlCols<T>*, true>(
+                g, eA, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1, src ? a.src_amp[0] : (T)0,
+                (const CpmlCols<T>*)&ka, nullptr, &qe, tab, false, &nke, n1x, n1y, n1z);
         }
         le[b][0][t] = n1x;
         le[b][1][t] = n1y;
@@ -1718,8 +1724,14 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
             }
             const T jxb = NU ? g.inv_h[0][xb] : g.inv_u;
             const bool src = src_comp >= 0 && xb == a.src_x;
-            e_update<T, true>(g, eB, c, xb, oi, v, h2y_m, h2z_m, ep0, ep1, ep2, jxb, jy, jz, src ? src_comp : -1,
-                              src ? a.src_amp[1] : (T)0, (const CpmlCols<T>*)&kb, nullptr, &ke, tab);
+            T ox, oy, oz;
+            e_value<T, true, ESysP<T>, const CpmlCols<T>*, true>(
+                g, eB, c, xb, v, h2y_m, h2z_m, ep0, ep1, ep2, jxb, jy, jz, src ? src_comp : -1,
+                src ? a.src_amp[1] : (T)0, (const CpmlCols<T>*)&kb, nullptr, &ke, tab, true, (PsiPair<T>*)nullptr, ox,
+                oy, oz);
+            stc<FDTD_NT_E>(eB.E, 0, oi, ox);
+            stc<FDTD_NT_E>(eB.E, 1, oi, oy);
+            stc<FDTD_NT_E>(eB.E, 2, oi, oz);
         }
         // ---- carry to the next plane
         h2y_m = g2y; h2z_m = g2z;

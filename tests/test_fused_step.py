@@ -134,6 +134,7 @@ def test_update_he_equals_h_then_e(dtype):
     ((7, 3, 256), 1, False),     # longest rows the two-step pass takes: one owned row per tile
     ((4, 2, 9), 1, False),       # two rows: the halo rows are owned rows of the same tile
     ((4, 25, 40), 2, False),     # four planes, several tiles per column
+    ((2, 5, 9), 1, False),       # two planes: the chunk's extra planes wrap onto its own
 ])
 def test_update_he2_equals_he_twice(shape, T, nonuniform):
     """One fdtd_update_he2 = fdtd_update_he twice (sources of both steps),
