@@ -411,7 +411,8 @@ def main():
         avg_ms = float(numpy.mean(dur[k]))
         byts = kernel_bytes(w.shape, w.pmls, k, dbytes, w.do_poynting)
         kern[k] = {"avg_us": avg_ms * 1e3, "bytes": byts, "GBs": byts / (avg_ms / 1e3) / 1e9,
-                   "marker_us_subtracted": marker_ms * 1e3}
+                   "marker_us_subtracted": marker_ms * 1e3,
+                   "us_percentiles_5_50_95": [float(numpy.percentile(dur[k], q)) * 1e3 for q in (5, 50, 95)]}
     # the dominant kernel: the two-step pass when step_n uses it (one launch
     # per two steps), else the fused pass (one launch per step), else the
     # slower of the two half-steps
