@@ -164,6 +164,33 @@ def test_update_he2_equals_he_twice(shape, T, nonuniform):
     _assert_same(_state(a), _state(b), "update_HE2 without a source")
 
 
+def test_update_he2_sources_on_tile_and_chunk_edges():
+    """The source added inside the two-step pass on cells that other
+    workgroups also evaluate: rows y0-1 and y1 of a tile (its halo rows),
+    the periodic wraps, and, on this grid, x chunks of one plane, so every
+    source plane is also an extra plane of the neighbouring chunks.  40 x 50
+    x 30 cells: tiles of 31 rows (rows 0-30 and 31-49), 40 chunks."""
+    shape = (40, 50, 30)
+    eps, pmls, init, _ = _case(shape, 3, seed=21, pec=True)
+    a = _sim(eps, pmls, init, None, numpy.float32, True)
+    b = _sim(eps, pmls, init, None, numpy.float32, True)
+    for s in (a, b):
+        for _ in range(2):
+            s.update_E([])
+            s.update_H([])
+        s.update_E([])
+    N = a.N
+    cells = [(0, 5, 31, 0), (1, 0, 30, 29), (2, 39, 0, 15), (0, 20, 49, 7), (2, 17, 32, 29), (1, 38, 1, 1)]
+    for k, (comp, x, y, z) in enumerate(cells):
+        idx = comp * N + (x * shape[1] + y) * shape[2] + z
+        t_amp = torch.tensor([0.5 + k, -1.25 * (k + 1)], dtype=torch.float32, device="cuda")
+        a.update_HE2([], idx, t_amp)
+        b.update_HE([], idx, t_amp[0:1])
+        b.update_HE([], idx, t_amp[1:2])
+        torch.cuda.synchronize()
+        _assert_same(_state(a), _state(b), "update_HE2 source at %s" % ((comp, x, y, z),))
+
+
 @pytest.mark.parametrize("shape, T, two", [
     ((14, 11, 150), 4, False),       # rows of 150 cells (1024-thread tiles)
     ((6, 7, 5), 4, False),           # overlapping slabs on every axis
