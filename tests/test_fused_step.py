@@ -21,7 +21,7 @@ def _case(shape, T, seed, nonuniform=False, pec=False):
     if pec:
         for e in eps:
             e[shape[0] // 2, :, :shape[2] // 3] = numpy.inf
-    pmls = [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
+    pmls = [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"] if T else []
     init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
     dxes = None
     if nonuniform:
@@ -135,6 +135,7 @@ def test_update_he_equals_h_then_e(dtype):
     ((4, 2, 9), 1, False),       # two rows: the halo rows are owned rows of the same tile
     ((4, 25, 40), 2, False),     # four planes, several tiles per column
     ((2, 5, 9), 1, False),       # two planes: the chunk's extra planes wrap onto its own
+    ((6, 20, 30), 0, False),     # no CPML: no psi blocks at all
 ])
 def test_update_he2_equals_he_twice(shape, T, nonuniform):
     """One fdtd_update_he2 = fdtd_update_he twice (sources of both steps),
