@@ -443,6 +443,9 @@ def main():
         "config": {"workload": w.name, "shape": list(w.shape), "pml": "CPML-8 on 6 faces",
                    "poynting": w.do_poynting, "cells": w.cells,
                    "steps_per_pass": 2 if two else 1,
+                   "s_stored": ("for the second H half-step of each two-step pass; the first S of a pass is "
+                                "overwritten in the same launch, and the S after step_n is bit-identical to the "
+                                "per-step loop's") if two else "every H half-step",
                    "bytes_per_cell_update": step_bytes / w.cells,
                    "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS,
                    "hbm_frac_one_step_pass_equivalent": gcell * one_pass_bytes / w.cells / HBM_PEAK_GBS},
