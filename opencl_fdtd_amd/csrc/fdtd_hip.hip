@@ -2455,7 +2455,6 @@ int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, c
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
     if (int rc = check_fields(P, f, true)) return rc;
-    if (int rc = check_he_fields(P, f)) return rc;
     if (int rc = check_he2_fields(P, f)) return rc;
     if (src_amp && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));

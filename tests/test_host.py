@@ -287,5 +287,5 @@ def test_update_he2_rejects_bad_calls():
     rc, msg = call(psi_e_out_same=True)
     assert rc != 0 and "psi_e_out" in msg
     rc, msg = call(e_out_same=True)
-    assert rc != 0 and "E_out" in msg
+    assert rc != 0 and "fdtd_update_he2 needs E_out" in msg
     assert lib.fdtd_update_he2(None, None, -1, None, None) != 0
