@@ -1452,11 +1452,14 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
 // the fields of step n+1 never leave the chip.
 //
 // The march runs two stages one plane apart.  At plane x the first stage
-// evaluates H^{n+1/2}(x) and E^{n+1}(x) exactly as he_march does (H through
-// LDS, E^{n+1} at y-1, z-1 from LDS, x-1 from registers); the second stage
-// then evaluates H^{n+3/2}(x-1), which needs E^{n+1} at x-1 (registers), at x
-// (just computed) and at y+1, z+1 of plane x-1 (LDS, written one iteration
-// before), and E^{n+2}(x-1) and S(x-1) from H^{n+3/2} through LDS.  Each
+// evaluates H^{n+1/2}(x) and E^{n+1}(x) as he_march does (H at y-1, z-1
+// through LDS, at x-1 from the previous iteration); the second stage then
+// evaluates H^{n+3/2}(x-1), which needs E^{n+1} at x-1 (previous
+// iteration), at x (just computed) and at y+1, z+1 of plane x-1 (LDS,
+// written one iteration before, kept by plane parity), and E^{n+2}(x-1) and
+// S(x-1) from H^{n+3/2} through LDS.  The values of the previous iteration
+// are read back from LDS (LC) or carried in registers.  Both E updates of a
+// cell use one quotient dt / eps, divided once.  Each
 // stage widens the work in y by one row, so a tile computes rows y0-2 .. y1
 // for the owned rows y0 .. y1-1 (whole rows, z periodic inside a row), and
 // an x chunk x0 .. x1-1 marches planes x0-2 .. x1.  The psi pairs of step
