@@ -192,6 +192,29 @@ def test_update_he2_sources_on_tile_and_chunk_edges():
         _assert_same(_state(a), _state(b), "update_HE2 source at %s" % ((comp, x, y, z),))
 
 
+def test_step_n_two_step_schedule_equals_unfused():
+    """step_n on a grid where the two-step pass is chosen, for n = 5 .. 10:
+    the pairs left after the two-step passes (1, 2 or 3 of them) run as
+    one-step passes or unfused, and E, H, S and both psi blocks end as the
+    unfused loop leaves them, in the caller's buffers, call after call."""
+    shape = (120, 120, 73)
+    eps, pmls, init, _ = _case(shape, 8, seed=33, pec=True)
+    a = _sim(eps, pmls, init, None, numpy.float32, True)
+    b = _sim(eps, pmls, init, None, numpy.float32, False)
+    assert a.two_step_passes
+    idx = 1 * a.N + 60 * shape[1] * shape[2] + 61 * shape[2] + 36
+    E0, psi0 = a._E.data_ptr(), a._psi_e.data_ptr()
+    k0 = 0
+    for n in (5, 6, 7, 8, 9, 10):
+        wave = numpy.sin(0.21 * numpy.arange(k0, k0 + n)).astype(numpy.float32)
+        a.step_n(n, idx, wave)
+        b.step_n(n, idx, wave)
+        k0 += n
+        torch.cuda.synchronize()
+        assert a._E.data_ptr() == E0 and a._psi_e.data_ptr() == psi0
+        _assert_same(_state(a), _state(b), "step_n(%d) two-step schedule vs unfused" % n)
+
+
 @pytest.mark.parametrize("shape, T, two", [
     ((14, 11, 150), 4, False),       # rows of 150 cells (1024-thread tiles)
     ((6, 7, 5), 4, False),           # overlapping slabs on every axis
