@@ -105,7 +105,7 @@ namespace {
 #define FDTD_HE2_EARLY 0             // two-step pass: next plane's loads issued before the last stage
 #endif
 #ifndef FDTD_HE2_COST
-#define FDTD_HE2_COST 13             // two-step pass: plane iteration cost / fused pass's, x10
+#define FDTD_HE2_COST 11             // two-step pass: plane iteration cost / fused pass's, x10
 #endif
 #ifndef FDTD_HE2_S2
 #define FDTD_HE2_S2 0                // two-step pass: also store S of its first H half-step
@@ -2259,8 +2259,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
     // Time per step of each pass: rounds of workgroups x planes a chunk
     // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
-    // of the fused pass's (measured on MI355X: 1.20 with rows of 73 cells,
-    // 1.29 with rows of 128) and covering two steps.  Long rows leave the two-step tiles few owned rows
+    // of the fused pass's (measured on MI355X: 1.15 with rows of 73 cells,
+    // 1.10 with rows of 128) and covering two steps.  Long rows leave the two-step tiles few owned rows
     // of their 1024 threads; the cost model then keeps the fused pass.
     P->he2_pref = false;
     if (P->lhe2.mode == 2) {
