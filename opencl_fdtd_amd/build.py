@@ -33,19 +33,5 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
-def build_probe(verbose: bool = False) -> str:
-    """The HBM copy probe (csrc/copy_probe.hip), a standalone executable."""
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    out = os.path.join(HERE, "copy_probe")
-    cmd = [hipcc, "--offload-arch=" + ARCH, "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-o", out, os.path.join(HERE, "csrc", "copy_probe.hip")]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    return out
-
-
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
-    if "--probe" in sys.argv:
-        print(build_probe(verbose="-v" in sys.argv))

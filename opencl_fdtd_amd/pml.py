@@ -51,14 +51,17 @@ def face_coefficients(pml: Dict, dt: float, dtype) -> List[numpy.ndarray]:
     T = int(pml["thickness"])
     one = t(1)
     sigma_max = t(-pml["ln_R_per_layer"] / 2 * (pml["m"] + 1))
-    kappa_max = t(numpy.sqrt(pml["mu_eff"] * pml["epsilon_eff"]))
+    # kappa_max is a float64 scalar in the reference (numpy.sqrt of Python
+    # floats, simulation.py:285); kappa_max - 1 is formed between scalars, in
+    # float64, and rounded once when it meets the float32 array (:291).
+    km1 = t(numpy.float64(numpy.sqrt(pml["mu_eff"] * pml["epsilon_eff"])) - 1)
     alpha_max = t(pml["cfs_alpha"])
     m, ma, Tt, dtt = t(pml["m"]), t(pml["ma"]), t(T), t(dt)
 
     def par(x):
         scaling = (x / Tt) ** m
         sigma = scaling * sigma_max
-        kappa = one + scaling * (kappa_max - one)
+        kappa = one + scaling * km1
         alpha = ((one - x / Tt) ** ma) * alpha_max
         with numpy.errstate(invalid="ignore", divide="ignore"):
             p0 = numpy.exp(-(sigma / kappa + alpha) * dtt)

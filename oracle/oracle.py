@@ -104,7 +104,9 @@ def pml_coefficients(pml: Dict, dt, dtype) -> tuple:
     dtype = numpy.dtype(dtype).type
     T = pml["thickness"]
     sigma_max = dtype(-pml["ln_R_per_layer"] / 2 * (pml["m"] + 1))
-    kappa_max = dtype(numpy.sqrt(pml["mu_eff"] * pml["epsilon_eff"]))
+    # simulation.py:285,291: kappa_max is a float64 scalar, kappa_max - 1 is
+    # a scalar-scalar operation (float64), rounded to the array dtype once.
+    km1 = dtype(numpy.float64(numpy.sqrt(pml["mu_eff"] * pml["epsilon_eff"])) - 1)
     alpha_max = dtype(pml["cfs_alpha"])
     m = dtype(pml["m"])
     ma = dtype(pml["ma"])
@@ -115,7 +117,7 @@ def pml_coefficients(pml: Dict, dt, dtype) -> tuple:
     def par(x):
         scaling = (x / Tt) ** m
         sigma = scaling * sigma_max
-        kappa = one + scaling * (kappa_max - one)
+        kappa = one + scaling * km1
         alpha = ((one - x / Tt) ** ma) * alpha_max
         with numpy.errstate(invalid="ignore", divide="ignore"):
             p0 = numpy.exp(-(sigma / kappa + alpha) * dtt)

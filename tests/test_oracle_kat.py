@@ -340,3 +340,265 @@ def test_bloch_cell_equals_supercell():
         A = _as3(getattr(a, mine), shape)
         B = _as3(getattr(sup, part), big)[:, :L]
         assert numpy.abs(A - B).max() < 1e-12 * numpy.abs(B).max(), mine
+
+
+# ---------------------------------------------------------------- S, cell by cell
+def _cyc(d):
+    return (d + 1) % 3, (d + 2) % 3
+
+
+# physical offsets of the Yee components from their cell index (E_a on the
+# primal edge along a, H_a on the dual edge along a)
+_DE = numpy.eye(3) * 0.5
+_DH = (numpy.ones((3, 3)) - numpy.eye(3)) * 0.5
+
+
+def _s_closed_form(shape, kvec, Eh, Hh, lam, t, dt):
+    """Poynting vector of the discrete plane wave E = Re(Eh lam^t e^{ik.idx}),
+    H = Re(Hh lam^t e^{ik.idx}) after step t, sampled at the S_d points.
+
+    S_d sits on the E_d point.  It is dt/8 times the sum of the four E x H
+    products on the edges around that point: E_a H_b at the two points half a
+    cell along a from it, E_b H_a (negated) at the two points half a cell
+    along b, (d, a, b) cyclic; E is averaged over the two cells along d and H
+    over the two half-steps around E's time.  For a plane wave these averages
+    are closed forms:
+      2 cos(k_d/2) Re(A e^{ik.r} lam^t),  Re(B e^{ik.r} lam^t (1 + 1/lam)),
+    with A, B the amplitudes at the physical positions, and the pair of
+    points +-1/2 along a turns e^{2ik.r} into 2 cos(k_a) e^{2ik.r}."""
+    k = numpy.asarray(kvec, dtype=float)
+    A = [Eh[a] * numpy.exp(-1j * (k @ _DE[a])) for a in range(3)]
+    B = [Hh[a] * numpy.exp(-1j * (k @ _DH[a])) for a in range(3)]
+    idx = numpy.indices(shape).astype(float)
+    out = []
+    for d in range(3):
+        a, b = _cyc(d)
+        r = idx + _DE[d][:, None, None, None]
+        e2 = numpy.exp(2j * sum(k[q] * r[q] for q in range(3)))
+        tm = lam ** t * (1 + 1 / lam)          # H^t + H^{t-1}, relative to lam^t
+        steady = numpy.real((A[a] * numpy.conj(B[b]) - A[b] * numpy.conj(B[a])) * numpy.conj(tm) * lam ** t)
+        osc = numpy.real((math.cos(k[a]) * A[a] * B[b] - math.cos(k[b]) * A[b] * B[a]) * e2 * lam ** t * tm)
+        out.append(dt / 4 * math.cos(k[d] / 2) * (steady + osc))
+    return numpy.array(out)
+
+
+@pytest.mark.parametrize("m", [(1, -2, 3), (2, 1, -1), (0, 3, 2)])
+def test_poynting_oblique_wave_cell_by_cell(m):
+    """Every S component of an oblique travelling wave, at every cell and
+    step, equals the closed form above (FP64, 1e-12 of max|S|).  With k along
+    two or three axes the four products sit at distinct phases, so a
+    neighbour taken on the wrong side (+e instead of -e), a dropped product,
+    a wrong sign or a missing time average moves S by O(1)."""
+    shape = (8, 10, 12)
+    kvec = [2 * math.pi * m[r] / shape[r] for r in range(3)]
+    dt = 0.99 / math.sqrt(3)
+    Eh, Hh, lam = _eigenmode(shape, kvec, 1.0, dt, a=(0.4, -1.0, 0.7))
+    init = {"E": _mode_fields(shape, kvec, Eh, 1.0), "H": _mode_fields(shape, kvec, Hh, 1.0)}
+    o = OracleSimulation([numpy.ones(shape)] * 3, [], initial_fields=init, float_type=numpy.float64, dt=dt)
+    for t in range(1, 9):
+        o.step()
+        S = _as3(o.S, shape)
+        ref = _s_closed_form(shape, kvec, Eh, Hh, lam, t, dt)
+        scale = numpy.abs(ref).max()
+        assert scale > 1e-3
+        for d in range(3):
+            err = numpy.abs(S[d] - ref[d]).max()
+            assert err < 1e-12 * scale, (t, d, err, scale)
+        # the oscillating part is a large share of S, so the test sees offsets
+        assert numpy.abs(ref - ref.mean(axis=(1, 2, 3), keepdims=True)).max() > 0.2 * scale
+
+
+# ---------------------------------------------------------------- CPML, 1-D model
+def _rg_coefficients(depth, T, sigma_max, kappa_max, alpha_max, m, ma, dt):
+    """Roden & Gedney's CPML (Microw. Opt. Tech. Lett. 27, 2000) in units
+    eps0 = mu0 = 1, with the graded profiles sampled at a node of the given
+    depth: b = exp(-(sigma/kappa + alpha) dt),
+    a = sigma (b - 1) / (sigma kappa + kappa^2 alpha)."""
+    g = (depth / T) ** m
+    sigma = sigma_max * g
+    kappa = 1 + (kappa_max - 1) * g
+    alpha = alpha_max * (1 - depth / T) ** ma
+    b = math.exp(-(sigma / kappa + alpha) * dt)
+    a = sigma * (b - 1) / (sigma * kappa + kappa * kappa * alpha)
+    return a, b, kappa
+
+
+def _cpml_1d(L, faces, eps_a, E0, H0, sgn, dt, steps):
+    """Independent FP64 1-D Yee + CPML line along one axis: E at x = i,
+    H at x = i + 1/2, periodic.  dE/dt = sgn/eps dH/dx, dH/dt = sgn dE/dx,
+    with dx -> dx/kappa + psi inside a PML.  A face of thickness T is a layer
+    T cells thick whose outer edge is the outermost node of the line on its
+    side: E_0 (x = 0) on the low side, H_{L-1} (x = L - 1/2) on the high side,
+    so a node's depth is T - x (low) or x - (L - 1/2 - T) (high), and it is
+    inside when 0 < depth <= T."""
+    xE = numpy.arange(L, dtype=float)
+    xH = xE + 0.5
+
+    def layer(xs):
+        a = numpy.zeros(L)
+        b = numpy.ones(L)
+        kap = numpy.ones(L)
+        inside = numpy.zeros(L, bool)
+        for side, T, par in faces:
+            depth = (T - xs) if side == "n" else (xs - (L - 0.5 - T))
+            for i in range(L):
+                if 0 < depth[i] <= T:
+                    a[i], b[i], kap[i] = _rg_coefficients(depth[i], T, *par, dt)
+                    inside[i] = True
+        return a, b, kap, inside
+
+    aE, bE, kE, inE = layer(xE)
+    aH, bH, kH, inH = layer(xH)
+    E, H = E0.copy(), H0.copy()
+    psiE, psiH = numpy.zeros(L), numpy.zeros(L)
+    for _ in range(steps):
+        D = H - numpy.roll(H, 1)
+        psiE = numpy.where(inE, bE * psiE + aE * D, 0.0)
+        E = E + dt / eps_a * sgn * (D / kE + psiE)
+        D = numpy.roll(E, -1) - E
+        psiH = numpy.where(inH, bH * psiH + aH * D, 0.0)
+        H = H + dt * sgn * (D / kH + psiH)
+    return E, H
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_cpml_matches_independent_1d_model(d):
+    """A 3-D run whose fields are uniform across axis d matches the
+    independent 1-D CPML line above on both polarisations, to 1e-12: the
+    E/H half-cell depths, the psi recursion and its coefficients (sigma,
+    kappa, CFS alpha, both grading exponents), the derivative each psi
+    takes and the sign it enters the curl with on every axis."""
+    L, Tn, Tp = 36, 5, 7
+    dt = 0.99 / math.sqrt(3)
+    rng = numpy.random.default_rng(20 + d)
+    shape = [3, 2, 2]
+    shape[d] = L
+    shape = tuple(shape)
+    fn = {"axis": "xyz"[d], "polarity": "n", "thickness": Tn, "ln_R_per_layer": -1.9, "m": 3.0,
+          "cfs_alpha": 0.08, "ma": 1, "epsilon_eff": 2.25}
+    fp = {"axis": "xyz"[d], "polarity": "p", "thickness": Tp, "ln_R_per_layer": -1.3, "m": 2.5,
+          "cfs_alpha": 0.03, "ma": 2, "epsilon_eff": 1.0, "mu_eff": 1.7}
+
+    def par(f):
+        kappa_max = math.sqrt(f.get("mu_eff", 1.0) * f["epsilon_eff"])
+        return (-f["ln_R_per_layer"] / 2 * (f["m"] + 1), kappa_max, f["cfs_alpha"], f["m"], f["ma"])
+
+    faces = [("n", Tn, par(fn)), ("p", Tp, par(fp))]
+    line = [slice(None) if r == d else 0 for r in range(3)]
+
+    def bcast(v):
+        sh = [1, 1, 1]
+        sh[d] = L
+        return numpy.broadcast_to(v.reshape(sh), shape).copy()
+
+    eps1 = [rng.uniform(1, 3, L) for _ in range(3)]
+    e0 = [rng.standard_normal(L) for _ in range(3)]
+    h0 = [rng.standard_normal(L) for _ in range(3)]
+    o = OracleSimulation([bcast(e) for e in eps1], [dict(fn), dict(fp)], float_type=numpy.float64, dt=dt,
+                         initial_fields={"E": [bcast(v) for v in e0], "H": [bcast(v) for v in h0]},
+                         do_poynting=False)
+    steps = 80
+    _run(o, steps)
+    E3, H3 = _as3(o.E, shape), _as3(o.H, shape)
+    a, b = _cyc(d)
+    # (E_a, H_b): sgn = eps_{d b a} = -1; (E_b, H_a): +1
+    for ea, hb, sgn in ((a, b, -1.0), (b, a, 1.0)):
+        E1, H1 = _cpml_1d(L, faces, eps1[ea], e0[ea], h0[hb], sgn, dt, steps)
+        scale = max(numpy.abs(E1).max(), numpy.abs(H1).max())
+        assert numpy.abs(E3[ea][tuple(line)] - E1).max() < 1e-12 * scale, ("E", ea)
+        assert numpy.abs(H3[hb][tuple(line)] - H1).max() < 1e-12 * scale, ("H", hb)
+        # the PML did act: the same line without it differs by O(1)
+        E0_, _ = _cpml_1d(L, [], eps1[ea], e0[ea], h0[hb], sgn, dt, steps)
+        assert numpy.abs(E0_ - E1).max() > 0.05 * scale
+    # the component along d does not move
+    assert numpy.array_equal(E3[d][tuple(line)], e0[d])
+    assert numpy.array_equal(H3[d][tuple(line)], h0[d])
+
+
+def test_kappa_max_minus_one_rounds_once():
+    """simulation.py:285,291: kappa_max is a float64 scalar, so kappa_max - 1
+    is formed in float64 and rounded to float32 once, where it meets the
+    float32 profile.  For eps_eff = 6, rounding sqrt(6) to float32 first
+    gives a kappa one ulp off on some layers; the coefficients must follow
+    the single rounding."""
+    f32 = numpy.float32
+    p = pml_defaults({"axis": "x", "polarity": "n", "epsilon_eff": 6.0, "thickness": 8, "m": 3.5})
+    c = pml_coefficients(p, 0.5, f32)
+    km1 = f32(math.sqrt(6.0) - 1.0)
+    km1_bad = f32(f32(math.sqrt(6.0)) - f32(1))
+    assert km1 != km1_bad                      # the two orders differ for this value
+    x = numpy.arange(8, 0, -1).astype(f32)     # E depths on an 'n' face
+    s = (x / f32(8)) ** f32(3.5)
+    good = f32(1) / (f32(1) + s * km1)
+    bad = f32(1) / (f32(1) + s * km1_bad)
+    assert not numpy.array_equal(good, bad)
+    assert numpy.array_equal(c[2], good)
+
+
+# ---------------------------------------------------------------- J source
+@pytest.mark.parametrize("box", [(0, 8, 0, 9, 0, 10), (0, 3, 4, 9, 7, 10), (5, 8, 0, 1, 2, 3), (2, 2, 0, 9, 0, 10)])
+def test_update_j_adds_c_jr_plus_s_ji_inside_the_box(box):
+    """update_j.cl:23-32: E += c Jr + s Ji on cells with min <= coord < max on
+    every axis, nothing elsewhere; boxes touching 0 and the far faces, a
+    one-cell-thick box and an empty one."""
+    rng = numpy.random.default_rng(30)
+    shape = (8, 9, 10)
+    o = OracleSimulation([numpy.ones(shape)] * 3, [], float_type=numpy.float64, do_fieldsrc=True,
+                         initial_fields={"E": [rng.standard_normal(shape) for _ in range(3)]})
+    o.Jr[:] = rng.standard_normal(o.Jr.size)
+    o.Ji[:] = rng.standard_normal(o.Ji.size)
+    before = _as3(o.E, shape).copy()
+    c, s = 0.37, -1.21
+    o.update_J(c, s, *box)
+    after = _as3(o.E, shape)
+    inside = numpy.zeros(shape, bool)
+    inside[box[0]:box[1], box[2]:box[3], box[4]:box[5]] = True
+    J = c * _as3(o.Jr, shape) + s * _as3(o.Ji, shape)
+    assert numpy.array_equal(after[:, ~inside], before[:, ~inside])
+    if inside.any():
+        assert numpy.abs(after[:, inside] - (before[:, inside] + J[:, inside])).max() <= 1e-15 * 8
+    assert inside.sum() == (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+
+
+# ---------------------------------------------------------------- non-uniform dx
+def test_nonuniform_energy_with_cell_volumes():
+    """With per-index spacings, E_a lives on a primal edge (length dx_e along
+    a) crossing a dual face (dx_h along the other two axes), H_a on a dual
+    edge crossing a primal face.  Weighting the leapfrog energy with those
+    volumes, sum eps E_a^2 V^E_a + H_a H'_a V^H_a is exactly conserved when
+    the E step divides by dx_h and the H step by dx_e, each indexed by its
+    own axis (update_e.cl:29-33, update_h.cl:29-33).  Alternating spacings
+    make every pairing mistake visible."""
+    rng = numpy.random.default_rng(31)
+    shape = (8, 6, 10)
+    alt = lambda n, p, q: numpy.where(numpy.arange(n) % 2 == 0, p, q)
+    de = [alt(shape[0], 0.8, 1.3), alt(shape[1], 1.1, 0.7), rng.uniform(0.7, 1.3, shape[2])]
+    dh = [alt(shape[0], 1.2, 0.9), alt(shape[1], 0.75, 1.25), rng.uniform(0.7, 1.3, shape[2])]
+    eps = [rng.uniform(1, 4, shape) for _ in range(3)]
+    init = {"E": [rng.standard_normal(shape) for _ in range(3)], "H": [rng.standard_normal(shape) for _ in range(3)]}
+    o = OracleSimulation(eps, [], dxes=[de, dh], initial_fields=init, float_type=numpy.float64,
+                         do_poynting=False)
+
+    def vol(own, other, a):
+        g = numpy.meshgrid(*[(own if r == a else other)[r] for r in range(3)], indexing="ij")
+        return g[0] * g[1] * g[2]
+
+    VE = [vol(de, dh, a) for a in range(3)]
+    VH = [vol(dh, de, a) for a in range(3)]
+    VE_bad = [vol(dh, de, a) for a in range(3)]
+    VH_bad = [vol(de, dh, a) for a in range(3)]
+    epsv = numpy.array(eps)
+
+    def energy(E, H, Hp, ve, vh):
+        return sum(float(numpy.sum(epsv[a] * E[a] ** 2 * ve[a]) + numpy.sum(H[a] * Hp[a] * vh[a])) for a in range(3))
+
+    w, w_bad = [], []
+    for _ in range(40):
+        hp = _as3(o.H, shape).copy()
+        o.step()
+        E, H = _as3(o.E, shape), _as3(o.H, shape)
+        w.append(energy(E, H, hp, VE, VH))
+        w_bad.append(energy(E, H, hp, VE_bad, VH_bad))
+    w, w_bad = numpy.array(w), numpy.array(w_bad)
+    assert numpy.abs(w - w[0]).max() < 1e-11 * abs(w[0])
+    assert numpy.abs(w_bad - w_bad[0]).max() > 1e-4 * abs(w_bad[0])
