@@ -369,6 +369,7 @@ def main():
         sim.update_E([])
         b.record(stream)
         sim.update_H([])
+        sim.flush()                  # the H half-step itself, not deferred into the next update_E
         c.record(stream)
         evs.append((a, b, c))
     # the fused H+E pass step_n runs for all but the first E and last H
@@ -419,6 +420,28 @@ def main():
     dom = "HE2" if two else "HE" if fused else max(kern, key=lambda k: kern[k]["avg_us"])
     print("bench: kernels %s" % json.dumps(kern), file=sys.stderr, flush=True)
 
+    # the drop-in path: the reference caller loop call by call (fdtd.py:162-175,
+    # with its two host waits per step); update_H is deferred into the next
+    # update_E, which then runs the fused H+E pass
+    loop_steps = min(args.steps, 200)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for t in range(loop_steps):
+        sim.update_E([]).wait()
+        sim.E[w.src_index] += wave[t % wave.numel()]
+        e = sim.update_H([])
+        if sim.update_S:
+            e = sim.update_S([e])
+        e.wait()
+    sim.flush()
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t1
+    per_call = {"steps": loop_steps, "ms_per_step": loop_s * 1e3 / loop_steps,
+                "gcell_updates_s": w.cells * loop_steps / loop_s / 1e9,
+                "timing": "host wall clock around the loop, two host waits per step as in fdtd.py",
+                "h_deferred_into_fused_pass": bool(sim._defer)}
+    print("bench: per-call loop %s" % json.dumps(per_call), file=sys.stderr, flush=True)
+
     # algorithmic bytes per cell update of the pass step_n runs, and of the
     # one-step fused pass (what a pass per step would have to move)
     step_bytes = (kern["HE2"]["bytes"] / 2 if two else kern["HE"]["bytes"] if fused
@@ -443,16 +466,21 @@ def main():
         "config": {"workload": w.name, "shape": list(w.shape), "pml": "CPML-8 on 6 faces",
                    "poynting": w.do_poynting, "cells": w.cells,
                    "steps_per_pass": 2 if two else 1,
-                   "s_stored": ("for the second H half-step of each two-step pass; the first S of a pass is "
-                                "overwritten in the same launch, and the S after step_n is bit-identical to the "
-                                "per-step loop's") if two else "every H half-step",
+                   "s_stored": ("only for the second H half-step of each two-step pass (the first is not "
+                                "computed); the S after step_n is bit-identical to the per-step loop's")
+                               if two else "every H half-step",
                    "bytes_per_cell_update": step_bytes / w.cells,
                    "hbm_frac_whole_step": gcell * step_bytes / w.cells / HBM_PEAK_GBS,
                    "hbm_frac_one_step_pass_equivalent": gcell * one_pass_bytes / w.cells / HBM_PEAK_GBS},
+        "per_call_loop": per_call,
         "roofline": {"bound": "hbm", "kernel": names[dom][0],
+                     "limiter": ("per-plane latency: one 1024-thread workgroup per CU, two barriers and one "
+                                 "memory round trip per plane" if dom == "HE2" else "HBM streaming"),
                      "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, names[dom][1]),
+                     "traffic_source": "profiles/traffic_%s.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, "
+                                       "calibrated on a copy kernel; recorded, not measured by this run)" % w.name,
                      "avg_us": kern[dom]["avg_us"], "bytes_per_launch": kern[dom]["bytes"],
                      "steps_per_launch": 2 if dom == "HE2" else 1,
                      "other_kernel": {k: v for k, v in kern.items() if k != dom}},

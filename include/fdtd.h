@@ -24,7 +24,7 @@
  *   fdtd_add_point     <- the caller's soft source  sim.E[ind] += f(t)
  *                         (fdtd.py:168-169), with f(t) read from device memory
  *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
- *                         (E, F, source, H, G, S) without host round trips
+ *   fdtd_step_n_moved     (E, F, source, H, G, S) without host round trips
  *   fdtd_update_he     <- sim.update_H + sim.update_S of one iteration of that
  *                         loop followed by sim.update_E and the soft source of
  *                         the next, in one pass over the grid
@@ -153,6 +153,15 @@ int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp
  * roles every step: after an odd n the newest H is in f->H_out. */
 int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
                 const void* src_wave, void* stream);
+/* fdtd_step_n (same loop, same results) without bringing E and psi_E back
+ * to their own buffers: with f->E_out (and f->psi_e_out) set it runs
+ * (n - 1) / 2 two-step passes and at most one one-step pass, where
+ * fdtd_step_n keeps the pass counts even.  *moved receives bit 0 when the
+ * new E is in f->E_out, bit 1 when psi_E is in f->psi_e_out, bit 2 when the
+ * newest H (with psi_H, G, psi_G) is in f->H_out.  The caller swaps its
+ * buffer roles accordingly. */
+int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
+                      const void* src_wave, void* stream, int32_t* moved);
 
 /* The H half-step with S followed by the next E half-step in one pass:
  * H_out = H-step(E, H); S; E_out = E-step(E, H_out); then, when src_amp is

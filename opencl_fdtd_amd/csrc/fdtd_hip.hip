@@ -2360,10 +2360,18 @@ int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp
     return check_launch("fdtd_add_point");
 }
 
-int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
-                void* stream) {
+}  // extern "C"
+
+namespace {
+
+// n steps of the caller loop.  home: E and psi_e end in f->E and f->psi_e
+// (pass counts kept even); otherwise they may end in f->E_out / f->psi_e_out
+// and *moved says where (bit 0: E, bit 1: psi_e).
+int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
+                void* stream, bool home, int32_t* moved) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
+    if (moved) *moved = 0;
     if (n < 0) return fail(3, "n = %d < 0", n);
     if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
@@ -2395,10 +2403,11 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     // Half-steps E_0 H_0 E_1 H_1 ... E_{n-1} H_{n-1}.  Fused: E_0, then the
     // m = n-1 pairs (H_{k-1}, E_k), then H_{n-1}.  A pair runs unfused, in
     // one he_march, or two pairs in one he2_march.  Each he_march moves E,
-    // and each he2_march E and psi_e, to the other buffer and H once, so
+    // and each he2_march E and psi_e, to the other buffer and H once.  home:
     // both counts are kept even (E and psi_e end in f->E and f->psi_e, and
     // H swaps n times in all, as unfused): 2 * (m / 4) he2_march passes,
     // then of the rest an even number of he_march and at most one unfused.
+    // Otherwise m / 2 he2_march passes and at most one he_march.
     const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P);
     const bool fuse2 = fuse && he2_supported(P) && P.he2_pref &&
                        (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
@@ -2407,14 +2416,14 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
         if (int rc = half_e(0)) return rc;
         k = 1;
         const int m = n - 1;
-        const int n2 = fuse2 ? 2 * (m / 4) : 0;
+        const int n2 = fuse2 ? (home ? 2 * (m / 4) : m / 2) : 0;
         const int r = m - 2 * n2;
-        if (r % 2 == 1) {
+        if (home && r % 2 == 1) {
             if (int rc = half_h()) return rc;
             if (int rc = half_e(1)) return rc;
             k = 2;
         }
-        for (int j = 0; j < r / 2 * 2; ++j, ++k) {
+        for (int j = 0; j < (home ? r / 2 * 2 : r); ++j, ++k) {
             int rc = (P.d.dtype == FDTD_F32) ? launch_he<float>(P, &cur, st, (const float*)amp_of(k), src_index)
                                              : launch_he<double>(P, &cur, st, (const double*)amp_of(k), src_index);
             if (rc) return rc;
@@ -2429,13 +2438,33 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
             std::swap(cur.E, cur.E_out);
             std::swap(cur.psi_e, cur.psi_e_out);
         }
-        return half_h();
-    }
-    for (k = 0; k < n; ++k) {
-        if (int rc = half_e(k)) return rc;
         if (int rc = half_h()) return rc;
+    } else {
+        for (k = 0; k < n; ++k) {
+            if (int rc = half_e(k)) return rc;
+            if (int rc = half_h()) return rc;
+        }
     }
+    // a two-step pass swaps H once for two H half-steps, so with an odd
+    // number of them the newest H is not where n's parity puts it
+    if (moved)
+        *moved = (cur.E != f->E ? 1 : 0) | (cur.psi_e != f->psi_e ? 2 : 0) | (cur.H != f->H ? 4 : 0);
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
+                void* stream) {
+    return step_n_impl(plan, f, n, src_index, src_wave, stream, true, nullptr);
+}
+
+int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
+                      void* stream, int32_t* moved) {
+    if (!moved) return fail(2, "moved is NULL");
+    return step_n_impl(plan, f, n, src_index, src_wave, stream, false, moved);
 }
 
 int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,

@@ -214,6 +214,7 @@ class ShellRunner:
         Ec = sim._E.clone()
         self.apply(Ec, sim._H[sim._h])
         sim.update_H([])
+        sim.flush()
         sim._E.copy_(Ec)
 
     def step_n(self, n: int, src_index: Optional[int] = None, src_wave: Optional[torch.Tensor] = None) -> None:

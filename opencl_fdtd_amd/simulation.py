@@ -25,9 +25,18 @@ Differences from the reference, all documented behaviour:
 * ``step_n`` (n steps without host round trips) runs each H half-step and
   the next E half-step as one pass over the grid when the plan allows it
   (fused Poynting, no Bloch boundaries, rows of at most 341 cells;
-  ``fuse_steps=False`` turns it off and saves the second E buffer it needs).
-  E ends in the same buffer it started in; results are bit-identical.
+  ``fuse_steps=False`` turns it off and saves the second E buffer it needs),
+  and two steps per pass where that is faster.  E may end in either of its
+  two buffers: read ``sim.E`` after each call.  Results are bit-identical.
   ``update_HE`` exposes that pass once; it moves E to the other buffer.
+* The per-call loop gets the same pass (``defer_h=True``, the default, where
+  the pass is available): ``update_H`` records the H half-step and returns
+  at once, and the next ``update_E`` runs it together with the E half-step
+  in one pass.  Reading ``sim.E``, ``sim.H``, ``sim.S`` (or F, G, oS), and
+  every other call, runs a recorded H half-step first, so the fields read
+  are always those of the reference loop; ``flush()`` runs it explicitly.
+  An event returned by a deferred ``update_H`` completes when the work
+  enqueued before it has; the deferred work itself runs with the next call.
 * An unstable ``dt`` warns and is then used (the reference warns and leaves
   ``dt`` unset); malformed initial fields raise (the reference builds the
   exception without raising it).
@@ -70,14 +79,21 @@ class DeviceField(torch.Tensor):
 
 
 class Event:
-    """Completion marker of an enqueued update (pyopencl.Event stand-in)."""
+    """Completion marker of an enqueued update (pyopencl.Event stand-in).
 
-    def __init__(self, stream: torch.cuda.Stream):
+    ``deferred``: the event of an ``update_H`` whose half-step is recorded
+    for the next call's fused pass.  Its work is not enqueued yet, and every
+    way to observe H or S runs it first on the simulation's stream, so
+    ``wait()`` has nothing to wait for and returns at once."""
+
+    def __init__(self, stream: torch.cuda.Stream, deferred: bool = False):
         self._event = torch.cuda.Event()
         self._event.record(stream)
+        self._deferred = deferred
 
     def wait(self) -> None:
-        self._event.synchronize()
+        if not self._deferred:
+            self._event.synchronize()
 
     @property
     def torch_event(self) -> torch.cuda.Event:
@@ -121,7 +137,8 @@ class Simulation:
                  do_fieldsrc: bool = False,
                  poynting_mode: str = "fused",
                  bloch_mode: str = "phase",
-                 fuse_steps: bool = True):
+                 fuse_steps: bool = True,
+                 defer_h: bool = True):
         if initial_fields is None:
             initial_fields = {}
         self._lib = _lib.load()
@@ -251,6 +268,10 @@ class Simulation:
         self._E_out = None
         if fuse_steps and self._lib.fdtd_update_he_supported(plan):
             self._E_out = torch.empty_like(self._E)
+        # deferred H half-step of the per-call loop (run by the next update_E
+        # as one pass, or by flush())
+        self._defer = bool(defer_h) and self._E_out is not None
+        self._pending_h = False
         n_psi = int(self._lib.fdtd_psi_elems(plan))
         self._coef = _vec([numpy.concatenate(coefs)] if coefs else [numpy.zeros(1)], self._tdtype, self.device)
         zeros = lambda: torch.zeros(max(n_psi, 1), dtype=self._tdtype, device=self.device)
@@ -293,33 +314,55 @@ class Simulation:
 
     @property
     def E(self) -> DeviceField:
+        self.flush()
         return self._E.as_subclass(DeviceField)
 
     @property
     def H(self) -> DeviceField:
+        self.flush()
         return self._H[self._h].as_subclass(DeviceField)
 
     @property
     def F(self) -> Optional[DeviceField]:
+        self.flush()
         return None if self._F is None else self._F.as_subclass(DeviceField)
 
     @property
     def G(self) -> Optional[DeviceField]:
+        self.flush()
         return None if self._G is None else self._G.as_subclass(DeviceField)
 
     @property
     def S(self) -> Optional[DeviceField]:
+        self.flush()
         return None if self._S is None else self._S.as_subclass(DeviceField)
 
     @property
     def oS(self) -> Optional[DeviceField]:
+        self.flush()
         return None if self._oS is None else self._oS.as_subclass(DeviceField)
+
+    @property
+    def pending_h(self) -> bool:
+        """True while an H half-step recorded by ``update_H`` has not run."""
+        return self._pending_h
+
+    def flush(self) -> None:
+        """Runs the H half-step a deferred ``update_H`` recorded, if any."""
+        if self._pending_h:
+            self._pending_h = False
+            self._launch(self._lib.fdtd_update_h, (), "fdtd_update_h")
+            self._h = 1 - self._h
 
     # ------------------------------------------------------------------ plumbing
     def _stream(self) -> torch.cuda.Stream:
         return self.queue if self.queue is not None else torch.cuda.current_stream(self.device)
 
-    def _fields(self) -> _lib.Fields:
+    def _fields(self, flush: bool = True) -> _lib.Fields:
+        """The C-ABI view of the buffers; runs a deferred H half-step first
+        unless ``flush`` is False."""
+        if flush:
+            self.flush()
         f = _lib.Fields()
         ptr = lambda t: None if t is None else t.data_ptr()
         f.E = ptr(self._E)
@@ -347,15 +390,31 @@ class Simulation:
         stream = self._stream()
         for e in events or ():
             stream.wait_event(e.torch_event if isinstance(e, Event) else e)
-        f = self._fields()
+        f = self._fields(flush=False)
         with torch.cuda.device(self.device):
             _lib.check(fn(self._plan, ctypes.byref(f), *extra, ctypes.c_void_p(stream.cuda_stream)), what)
         return Event(stream)
 
     def _update_E(self, events: Sequence = ()) -> Event:
+        if self._pending_h:
+            # the recorded H half-step and this E half-step in one pass
+            self._pending_h = False
+            ev = self._launch(self._lib.fdtd_update_he, events, "fdtd_update_he", ctypes.c_int64(-1),
+                              ctypes.c_void_p(None))
+            self._h = 1 - self._h
+            self._E, self._E_out = self._E_out, self._E
+            return ev
+        self.flush()
         return self._launch(self._lib.fdtd_update_e, events, "fdtd_update_e")
 
     def _update_H(self, events: Sequence = ()) -> Event:
+        self.flush()
+        if self._defer:
+            stream = self._stream()
+            for e in events or ():
+                stream.wait_event(e.torch_event if isinstance(e, Event) else e)
+            self._pending_h = True
+            return Event(stream, deferred=True)
         ev = self._launch(self._lib.fdtd_update_h, events, "fdtd_update_h")
         if len(self._H) == 2:
             self._h = 1 - self._h
@@ -364,7 +423,10 @@ class Simulation:
     def _update_S(self, events: Sequence = ()) -> Event:
         if self.poynting_mode == "two_pass":
             return self._launch(self._lib.fdtd_update_s, events, "fdtd_update_s")
-        return self._noop(events)
+        ev = self._noop(events)
+        if self._pending_h:
+            ev._deferred = True          # S comes with the recorded H half-step
+        return ev
 
     def _noop(self, events: Sequence = ()) -> Event:
         stream = self._stream()
@@ -373,6 +435,7 @@ class Simulation:
         return Event(stream)
 
     def _update_J(self, events, c, s, xmin, xmax, ymin, ymax, zmin, zmax) -> Event:
+        self.flush()
         box = (ctypes.c_int64 * 6)(int(xmin), int(xmax), int(ymin), int(ymax), int(zmin), int(zmax))
         return self._launch(self._lib.fdtd_update_j, events, "fdtd_update_j",
                             ctypes.c_void_p(self.Jr.data_ptr()), ctypes.c_void_p(self.Ji.data_ptr()),
@@ -387,6 +450,7 @@ class Simulation:
         new E is in a second buffer afterwards: read ``sim.E`` again."""
         if self._E_out is None:
             raise _lib.FdtdError("update_HE is not available for this simulation (fdtd_update_he_supported)")
+        self.flush()
         amp = None
         if src_amp is not None:
             if src_index is None:
@@ -409,6 +473,7 @@ class Simulation:
         afterwards: read ``sim.E`` again."""
         if self._psi_e_out is None:
             raise _lib.FdtdError("update_HE2 is not available for this simulation (fdtd_update_he2_supported)")
+        self.flush()
         amp = None
         if src_amp is not None:
             if src_index is None:
@@ -429,7 +494,9 @@ class Simulation:
         """``n`` iterations of the reference caller loop (fdtd.py:162-175)
         enqueued without host synchronisation: E (+F), ``E[src_index] +=
         src_wave[k]``, H (+G) with S.  ``src_wave`` is a device tensor (or
-        array) of ``n`` source values in the field dtype."""
+        array) of ``n`` source values in the field dtype.  E and psi_E may end
+        in either of their two buffers (``sim.E`` follows)."""
+        self.flush()
         stream = self._stream()
         for e in events or ():
             stream.wait_event(e.torch_event if isinstance(e, Event) else e)
@@ -444,12 +511,18 @@ class Simulation:
             self._wave_keepalive = wave
             wave_ptr = wave.data_ptr()
         f = self._fields()
+        moved = ctypes.c_int32(0)
         with torch.cuda.device(self.device):
-            _lib.check(self._lib.fdtd_step_n(self._plan, ctypes.byref(f), int(n),
-                                             int(src_index if src_index is not None else -1),
-                                             wave_ptr, ctypes.c_void_p(stream.cuda_stream)), "fdtd_step_n")
-        if len(self._H) == 2 and n % 2 == 1:
+            _lib.check(self._lib.fdtd_step_n_moved(self._plan, ctypes.byref(f), int(n),
+                                                   int(src_index if src_index is not None else -1),
+                                                   wave_ptr, ctypes.c_void_p(stream.cuda_stream),
+                                                   ctypes.byref(moved)), "fdtd_step_n_moved")
+        if len(self._H) == 2 and moved.value & 4:
             self._h = 1 - self._h
+        if moved.value & 1:
+            self._E, self._E_out = self._E_out, self._E
+        if moved.value & 2:
+            self._psi_e, self._psi_e_out = self._psi_e_out, self._psi_e
         return Event(stream)
 
     # ------------------------------------------------------------------ state
@@ -457,6 +530,7 @@ class Simulation:
         """Copies of everything the step reads: E, H, psi (and F, G with
         Bloch).  ``restore`` puts them back, so stepping resumes bit for bit
         (``initial_fields`` restores only E/H/F/G, as in the reference)."""
+        self.flush()
         st = {"E": self._E.clone(), "H": self._H[self._h].clone(), "psi_e": self._psi_e.clone(),
               "psi_h": self._psi_h[self._h if len(self._psi_h) == 2 else 0].clone()}
         if self._F is not None:
@@ -464,6 +538,7 @@ class Simulation:
         return st
 
     def restore(self, st: Dict[str, torch.Tensor]) -> None:
+        self.flush()
         self._E.copy_(st["E"])
         self._H[self._h].copy_(st["H"])
         self._psi_e.copy_(st["psi_e"])

@@ -63,3 +63,61 @@ def step_oracle(orc, steps, src_index=None, wave=None):
 def fetch(sim, name):
     v = getattr(sim, name)
     return None if v is None else v.get()
+
+
+def run_workload(w, steps, backend, dtype=numpy.float32, **sim_kwargs):
+    """Runs a ``workloads.Workload`` for ``steps`` steps in the reference
+    caller-loop order (E, F, soft source, H, G, S; fdtd.py:162-175) on the HIP
+    library ('gpu') or the CPU oracle ('oracle').  A plane source adds
+    Re(phase) * f(t) to E and Im(phase) * f(t) to F on its z plane, each
+    product rounded to the field type.  Returns the simulation object."""
+    import os
+    pmls = [dict(p) for p in w.pmls]
+    if backend == "gpu":
+        import torch
+        from opencl_fdtd_amd import Simulation
+        sim = Simulation(w.epsilon, pmls, dt=w.dt, float_type=dtype, bloch_boundaries=w.bloch_boundaries,
+                         do_poynting=w.do_poynting, **sim_kwargs)
+    else:
+        from oracle.oracle import OracleSimulation
+        sim = OracleSimulation(w.epsilon, pmls, dt=w.dt, float_type=dtype, bloch_boundaries=w.bloch_boundaries,
+                               do_poynting=w.do_poynting, nthreads=min(len(os.sched_getaffinity(0)), 16))
+    ps = w.plane_source
+    if ps is not None:
+        z, comp = ps["z"], ps["comp"]
+        pr = numpy.real(ps["phase"]).astype(dtype)
+        pi = numpy.imag(ps["phase"]).astype(dtype)
+        if backend == "gpu":
+            pr_t, pi_t = torch.as_tensor(pr, device="cuda"), torch.as_tensor(pi, device="cuda")
+    for t in range(steps):
+        if backend == "gpu":
+            sim.update_E([])
+        else:
+            sim.update_E()
+            if sim.bloch:
+                sim.update_F()
+        if ps is not None:
+            a = dtype(ps["wave"][t])
+            if backend == "gpu":
+                sim.E.view(3, *w.shape)[comp, :, :, z] += pr_t * float(a)
+                sim.F.view(3, *w.shape)[comp, :, :, z] += pi_t * float(a)
+            else:
+                sim.E.reshape(3, *w.shape)[comp, :, :, z] += pr * a
+                sim.F.reshape(3, *w.shape)[comp, :, :, z] += pi * a
+        elif w.src_wave is not None:
+            sim.E[w.src_index] += float(dtype(w.src_wave[t])) if backend == "gpu" else dtype(w.src_wave[t])
+        if backend == "gpu":
+            e = sim.update_H([])
+            if sim.update_G:
+                e = sim.update_G([e])
+            if sim.update_S:
+                e = sim.update_S([e])
+        else:
+            sim.update_H()
+            if sim.bloch:
+                sim.update_G()
+            if sim.do_poynting:
+                sim.update_S()
+    if backend == "gpu":
+        e.wait()
+    return sim

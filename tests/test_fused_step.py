@@ -71,12 +71,10 @@ def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
     N = a.N
     idx = 2 * N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 3) * shape[2] + shape[2] // 2
     wave = numpy.sin(0.3 * numpy.arange(3 * n)).astype(dtype)
-    E0 = a._E
-    for k in range(3):                           # three calls: E must stay in its buffer
+    for k in range(3):                           # three calls: E may change buffers between them
         a.step_n(n, idx, wave[k * n:(k + 1) * n])
         b.step_n(n, idx, wave[k * n:(k + 1) * n])
     torch.cuda.synchronize()
-    assert a._E.data_ptr() == E0.data_ptr()
     _assert_same(_state(a), _state(b), "step_n fused vs unfused")
 
 
@@ -193,26 +191,131 @@ def test_update_he2_sources_on_tile_and_chunk_edges():
 
 
 def test_step_n_two_step_schedule_equals_unfused():
-    """step_n on a grid where the two-step pass is chosen, for n = 5 .. 10:
-    the pairs left after the two-step passes (1, 2 or 3 of them) run as
-    one-step passes or unfused, and E, H, S and both psi blocks end as the
-    unfused loop leaves them, in the caller's buffers, call after call."""
+    """step_n on a grid where the two-step pass is chosen, for n = 1 .. 10:
+    (n - 1) / 2 two-step passes and at most one one-step pass; E, H, S and
+    both psi blocks end as the unfused loop leaves them, call after call,
+    wherever the schedule leaves E and psi_E."""
     shape = (120, 120, 73)
     eps, pmls, init, _ = _case(shape, 8, seed=33, pec=True)
     a = _sim(eps, pmls, init, None, numpy.float32, True)
     b = _sim(eps, pmls, init, None, numpy.float32, False)
     assert a.two_step_passes
     idx = 1 * a.N + 60 * shape[1] * shape[2] + 61 * shape[2] + 36
-    E0, psi0 = a._E.data_ptr(), a._psi_e.data_ptr()
     k0 = 0
-    for n in (5, 6, 7, 8, 9, 10):
+    homes = set()
+    for n in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
         wave = numpy.sin(0.21 * numpy.arange(k0, k0 + n)).astype(numpy.float32)
         a.step_n(n, idx, wave)
         b.step_n(n, idx, wave)
         k0 += n
         torch.cuda.synchronize()
-        assert a._E.data_ptr() == E0 and a._psi_e.data_ptr() == psi0
+        homes.add((a._E.data_ptr(), a._psi_e.data_ptr()))
         _assert_same(_state(a), _state(b), "step_n(%d) two-step schedule vs unfused" % n)
+    assert len(homes) > 1     # the schedule did move E / psi_E
+
+
+def test_step_n_c_abi_keeps_e_home():
+    """fdtd_step_n (the C-ABI entry point without the move report) keeps the
+    pass counts even: E and psi_E end in the caller's buffers, with the same
+    results as the unfused loop."""
+    import ctypes
+    shape = (120, 120, 73)
+    eps, pmls, init, _ = _case(shape, 8, seed=34, pec=True)
+    a = _sim(eps, pmls, init, None, numpy.float32, True)
+    b = _sim(eps, pmls, init, None, numpy.float32, False)
+    assert a.two_step_passes
+    idx = 1 * a.N + 60 * shape[1] * shape[2] + 61 * shape[2] + 36
+    k0 = 0
+    for n in (5, 6, 7, 8, 9, 10):
+        wave = torch.as_tensor(numpy.sin(0.21 * numpy.arange(k0, k0 + n)).astype(numpy.float32), device="cuda")
+        E0, psi0 = a._E.data_ptr(), a._psi_e.data_ptr()
+        f = a._fields()
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        assert a._lib.fdtd_step_n(a.plan, ctypes.byref(f), n, idx, ctypes.c_void_p(wave.data_ptr()), st) == 0
+        if n % 2 == 1:
+            a._h = 1 - a._h
+        b.step_n(n, idx, wave)
+        k0 += n
+        torch.cuda.synchronize()
+        assert a._E.data_ptr() == E0 and a._psi_e.data_ptr() == psi0
+        _assert_same(_state(a), _state(b), "fdtd_step_n(%d) vs unfused" % n)
+
+
+@pytest.mark.parametrize("shape, T, dtype", [
+    ((23, 30, 73), 8, numpy.float32),
+    ((120, 120, 73), 8, numpy.float32),
+    ((11, 13, 9), 3, numpy.float64),
+])
+def test_per_call_loop_runs_the_fused_pass(shape, T, dtype):
+    """The reference caller loop (update_E, E[idx] += f(t), update_H,
+    update_S, wait) with the H half-step deferred into the next update_E:
+    every step after the first runs as one fused pass, and E, H, S match the
+    oracle bit for bit after each of several reads between the steps."""
+    print("case per-call loop", shape, T, dtype.__name__, flush=True)
+    eps, pmls, init, _ = _case(shape, T, seed=41, pec=True)
+    sim, orc = build_pair(eps, pmls, init=init, dtype=dtype)
+    idx = 2 * sim.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
+    wave = numpy.cos(0.2 * numpy.arange(40))
+    t = 0
+    for block in (7, 1, 12):
+        for _ in range(block):
+            sim.update_E([]).wait()
+            assert not sim.pending_h
+            sim.E[idx] += float(dtype(wave[t]))
+            e = sim.update_H([])
+            assert sim.pending_h          # the H half-step waits for the next update_E
+            e = sim.update_S([e])
+            e.wait()
+            t += 1
+    step_oracle(orc, t, idx, wave)
+    for name in ("E", "H", "S"):
+        g, o = fetch(sim, name), getattr(orc, name)
+        bad = numpy.flatnonzero(g != o)
+        assert bad.size == 0, (name, bad.size, bad[:4])
+    assert not sim.pending_h
+
+
+def test_per_call_loop_mixed_with_other_calls():
+    """Deferred H half-steps next to reads, step_n, update_J, update_HE and
+    checkpoints: each of them runs the recorded half-step first, so the
+    fields equal those of the loop without deferral."""
+    from opencl_fdtd_amd import Simulation
+    shape = (12, 14, 73)
+    eps, pmls, init, _ = _case(shape, 3, seed=42, pec=True)
+    mk = lambda d: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], initial_fields=init,
+                              do_fieldsrc=True, defer_h=d)
+    a, b = mk(True), mk(False)
+    rng = numpy.random.default_rng(3)
+    for s in (a, b):
+        s.Jr.copy_(torch.as_tensor(rng.standard_normal(3 * s.N).astype(numpy.float32)))
+        s.Ji.copy_(torch.as_tensor(rng.standard_normal(3 * s.N).astype(numpy.float32)))
+        rng = numpy.random.default_rng(3)
+    idx = 1 * a.N + 6 * shape[1] * shape[2] + 7 * shape[2] + 30
+    for s in (a, b):
+        s.update_E([])
+        s.update_H([])
+        h_mid = s.H.clone()                       # read: runs the recorded H
+        s.update_E([])
+        s.update_H([])
+        s.update_J([], 0.5, -0.25, 2, 9, 3, 11, 0, 73)   # J after a recorded H
+        s.update_E([])
+        s.update_H([])
+        s.step_n(5, idx, numpy.linspace(-1, 1, 5).astype(numpy.float32))
+        s.update_E([])
+        s.update_H([])
+        cp = s.checkpoint()
+        s.update_E([])
+        s.update_H([])
+        s.update_HE([])
+        s.update_H([])
+        s.restore(cp)
+        s.update_E([])
+        s.update_H([])
+        s._h_mid = h_mid
+    torch.cuda.synchronize()
+    assert torch.equal(a._h_mid, b._h_mid)
+    _assert_same(_state(a), _state(b), "deferred vs immediate H")
+
 
 
 @pytest.mark.parametrize("shape, T, two", [

@@ -173,3 +173,54 @@ def test_phc_slab_shape():
     sim.step_n(40, w.src_index, w.src_wave.astype(numpy.float32)).wait()
     step_oracle(orc, 40, w.src_index, w.src_wave)
     _assert_parity(sim, orc)
+
+
+@pytest.fixture(scope="module")
+def bloch_c3_oracle():
+    """The Bloch supercell configuration at full size (64 x 64 x 256: phase on
+    x and y, CPML-16 on z, eps ~ U[1, 9] in 8^3 blocks, a 5-plane eps = inf
+    mirror at z = 200..204, plane-wave soft source on E and F at z = 24),
+    stepped 240 times by the oracle on the host cores."""
+    from opencl_fdtd_amd.workloads import bloch
+    from tests.helpers import run_workload
+    w = bloch(steps=240)
+    print("case bloch C3 oracle", w.shape, w.steps, flush=True)
+    return w, run_workload(w, w.steps, "oracle")
+
+
+@pytest.mark.parametrize("mode", ["fused", "two_pass"])
+def test_bloch_supercell_c3_bit_exact(bloch_c3_oracle, mode):
+    """C3 at full size for 240 steps, past the source pulse's peak: E, H, F,
+    G and S bit-identical to the oracle, with the paired E/F and H/G
+    kernels spanning many 1024-thread tiles per plane and crossing the PEC
+    mirror and both CPML slabs."""
+    from tests.helpers import run_workload
+    w, orc = bloch_c3_oracle
+    print("case bloch C3", mode, flush=True)
+    sim = run_workload(w, w.steps, "gpu", poynting_mode=mode)
+    _assert_parity(sim, orc, names=("E", "H", "F", "G", "S"))
+    E = fetch(sim, "E").reshape(3, *w.shape)
+    assert numpy.all(E[:, :, :, 200:205] == 0.0)
+    assert numpy.abs(E).max() > 1e-3
+
+
+def test_bloch_supercell_c3_full_run_fp32_vs_fp64():
+    """C3 for its full 4000 steps on the GPU in FP32 and FP64: every
+    component of E, H, F, G and S within relative L2 1e-4 of the FP64 run,
+    all finite, and E exactly zero inside the PEC mirror."""
+    from opencl_fdtd_amd.workloads import bloch
+    from tests.helpers import run_workload
+    w = bloch()
+    print("case bloch C3 fp32/fp64", w.shape, w.steps, flush=True)
+    a = run_workload(w, w.steps, "gpu", numpy.float32)
+    b = run_workload(w, w.steps, "gpu", numpy.float64)
+    n = a.N
+    for name in ("E", "H", "F", "G", "S"):
+        x = fetch(a, name).astype(numpy.float64)
+        y = fetch(b, name)
+        assert numpy.all(numpy.isfinite(x)), name
+        errs = comp_rel_l2(x, y, n)
+        print("C3 4000 steps %s fp32 vs fp64 rel-L2 %s" % (name, errs), flush=True)
+        assert max(errs) <= 1e-4, (name, errs)
+    E = fetch(a, "E").reshape(3, *w.shape)
+    assert numpy.all(E[:, :, :, 200:205] == 0.0)
