@@ -251,6 +251,17 @@ int fdtd_plane_sum(const void* plan, const void* field, int32_t comp, int32_t ax
 int64_t fdtd_reduce_partials(const void* plan);
 int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
                 double* partials, double* out, void* stream);
+/* fdtd_energy without the partials argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  It allocates nothing: the partials live in a file-scope
+ * device array of the library (1024 doubles, 8 KiB, one copy per GPU, held
+ * until the process ends); a plan whose reduction needs more is rejected.
+ * Same kernels and summation order as fdtd_energy, so the result is
+ * bit-identical.  Calls that overlap on one GPU share that array and race:
+ * serialise them (one stream, or events). */
+int fdtd_energy_noalloc(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
+                        double* out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Learned absorbing shell.  No counterpart in the reference: a thin shell of

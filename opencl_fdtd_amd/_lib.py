@@ -25,7 +25,7 @@ EXPORTS = (
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n", "fdtd_step_n_moved",
     "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
     "fdtd_update_he2_preferred",
-    "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy",
+    "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy", "fdtd_energy_noalloc",
     "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems",
     "fdtd_shell_apply", "fdtd_shell_step_n", "fdtd_shell_backward",
 )
@@ -109,6 +109,7 @@ def load() -> ctypes.CDLL:
     lib.fdtd_reduce_partials.argtypes = [P]
     lib.fdtd_reduce_partials.restype = ctypes.c_int64
     lib.fdtd_energy.argtypes = [P, P, P, P, P, P, P, P]
+    lib.fdtd_energy_noalloc.argtypes = [P, P, P, P, P, P, P]
     lib.fdtd_shell_weight_elems.argtypes = []
     lib.fdtd_shell_hidden_elems.argtypes = [P, ctypes.c_int32]
     lib.fdtd_shell_partials_elems.argtypes = []

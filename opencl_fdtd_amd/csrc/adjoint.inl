@@ -387,6 +387,45 @@ int fdtd_plane_sum(const void* plan, const void* field, int32_t comp, int32_t ax
 
 int64_t fdtd_reduce_partials(const void* plan) { return plan ? kReduceBlocks : -1; }
 
+}  // extern "C"
+
+// Scratch of fdtd_energy_noalloc: the FP64 partial sums of energy_partial,
+// one per workgroup (8 KiB).  A file-scope device array, so the entry point
+// needs no allocation; each GPU the library runs on holds its own copy until
+// the process ends.  Host code must not pass the array's name to a kernel
+// (on the host it is only a stand-in for the symbol): it uses the device
+// address hipGetSymbolAddress returns, kept per device.
+constexpr int kScratchPartials = 1024;
+static __device__ __align__(256) double scratch[kScratchPartials];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.
+int scratch_address(double** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(4, "hipGetDevice failed");
+    if (dev < 0 || dev >= kScratchDevices) return fail(4, "device %d beyond the %d supported", dev, kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(4, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
                 double* partials, double* out, void* stream) {
     if (!plan) return fail(1, "plan is NULL");
@@ -401,6 +440,21 @@ int fdtd_energy(const void* plan, const void* E, const void* H, const void* H_pr
                            (const double*)H, (const double*)H_prev, (const double*)eps, 3 * P.N, partials);
     hipLaunchKernelGGL(reduce_final, dim3(1), dim3(1024), 0, st, partials, kReduceBlocks, out);
     return check_launch("fdtd_energy");
+}
+
+int fdtd_energy_noalloc(const void* plan, const void* E, const void* H, const void* H_prev, const void* eps,
+                        double* out, void* stream) {
+    static_assert(kReduceBlocks <= kScratchPartials, "the energy reduction must fit the scratch array");
+    if (!plan) return fail(1, "plan is NULL");
+    if (fdtd_reduce_partials(plan) > kScratchPartials)
+        return fail(3, "the reduction needs %lld partials, the scratch holds %d",
+                    (long long)fdtd_reduce_partials(plan), kScratchPartials);
+    double* partials = nullptr;
+    if (int rc = scratch_address(&partials)) return rc;
+    // energy_partial writes every partial before reduce_final reads it, so
+    // nothing needs clearing; calls that overlap on one GPU share the array
+    // and race
+    return fdtd_energy(plan, E, H, H_prev, eps, partials, out, stream);
 }
 
 }  // extern "C"
