@@ -130,6 +130,14 @@ const char* fdtd_last_error(void);
 int fdtd_plan_create(const fdtd_desc* desc, void** plan_out);
 int fdtd_plan_destroy(void* plan);
 
+/* Text of the specialisation behind one update (which: 0 update_E, 1
+ * update_H, 2 update_S, 3 update_F, 4 update_G, 5 step_n): kernel, template
+ * flags, launch shape, CPML faces and Bloch axes.  Replaces the rendered
+ * template sources the reference keeps in sim.sources (simulation.py:209-230).
+ * Copies at most size-1 chars and a NUL into buf (when buf is not NULL) and
+ * returns the full length, or -1 on error. */
+int64_t fdtd_plan_describe(const void* plan, int32_t which, char* buf, int64_t size);
+
 /* Sizes (in elements of the field type) of the caller-owned blocks. */
 int64_t fdtd_psi_elems(const void* plan);
 int64_t fdtd_coef_elems(const void* plan);

@@ -20,7 +20,7 @@ BLOCH_PHASE, BLOCH_DECOUPLED = 0, 1
 
 # Every symbol include/fdtd.h declares.
 EXPORTS = (
-    "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy",
+    "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy", "fdtd_plan_describe",
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n", "fdtd_step_n_moved",
     "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
@@ -88,6 +88,8 @@ def load() -> ctypes.CDLL:
     lib.fdtd_last_error.argtypes = []
     lib.fdtd_plan_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_void_p)]
     lib.fdtd_plan_destroy.argtypes = [P]
+    lib.fdtd_plan_describe.argtypes = [P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+    lib.fdtd_plan_describe.restype = ctypes.c_int64
     lib.fdtd_psi_elems.argtypes = [P]
     lib.fdtd_psi_elems.restype = ctypes.c_int64
     lib.fdtd_coef_elems.argtypes = [P]
@@ -118,7 +120,7 @@ def load() -> ctypes.CDLL:
     lib.fdtd_shell_apply.argtypes = [P, ctypes.c_int32, P, P, P, P, P]
     lib.fdtd_shell_backward.argtypes = [P, ctypes.c_int32] + [P] * 10 + [P]
     lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int64, P, P]
-    sized = ("fdtd_last_error", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials",
+    sized = ("fdtd_last_error", "fdtd_plan_describe", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials",
              "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems")
     for name in EXPORTS:
         if name not in sized:

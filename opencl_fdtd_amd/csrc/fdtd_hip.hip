@@ -2496,6 +2496,102 @@ int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, c
 
 int fdtd_update_he2_supported(const void* plan) { return plan && he2_supported(*(const Plan*)plan) ? 1 : 0; }
 
+}  // extern "C"
+
+namespace {
+
+std::string launch_text(const Launch& L) {
+    char b[160];
+    snprintf(b, sizeof(b), "%d workgroups of %d threads (%d tile columns of %d plane cells x %d x-chunks of %d planes)",
+             L.pblocks * L.nchunk, L.block, L.pblocks, L.tile, L.nchunk, L.xchunk);
+    return b;
+}
+
+// The specialisation behind one update of the plan: what the reference's
+// rendered template source held (simulation.py:202-230: the E/H/S/F/G
+// templates rendered with pmls, do_poynting, bloch and uniform_dx), as the
+// kernel, its template flags, launch shape and face list chosen here.
+std::string describe(const Plan& P, int which) {
+    const fdtd_desc& d = P.d;
+    const char* ty = d.dtype == FDTD_F32 ? "float" : "double";
+    const bool pair = d.bloch_mask != 0;
+    const bool seam = pair && d.bloch_mode == FDTD_BLOCH_PHASE;
+    char b[512];
+    std::string out;
+    const char* what[] = {"update_E", "update_H", "update_S", "update_F", "update_G", "step_n"};
+    snprintf(b, sizeof(b), "/* %s: libfdtd_hip.so, gfx950; grid %d x %d x %d (%lld cells), %s, dt = %.17g */\n",
+             what[which], P.sx, P.sy, P.sz, (long long)P.N, ty, d.dt);
+    out += b;
+    if (d.nonuniform) out += "/* dx: per-index 1/dx_e (H step) and 1/dx_h (E step) arrays */\n";
+    else {
+        snprintf(b, sizeof(b), "/* dx: uniform, 1/dx = %.17g */\n", d.inv_dx);
+        out += b;
+    }
+    for (int k = 0; k < d.n_face; ++k) {
+        const fdtd_face& f = d.face[k];
+        snprintf(b, sizeof(b), "/* pml %d: axis %c, polarity %c, thickness %d, slab %d..%d, psi offset %d */\n", k,
+                 "xyz"[f.axis], f.polarity ? 'p' : 'n', f.thickness, P.face[k].lo, P.face[k].hi - 1,
+                 P.face[k].psi_off);
+        out += b;
+    }
+    for (int r = 0; r < 3; ++r)
+        if ((d.bloch_mask >> r) & 1) {
+            snprintf(b, sizeof(b), "/* bloch: axis %c, real %.17g, imag %.17g (%s) */\n", "xyz"[r], d.bloch_re[r],
+                     d.bloch_im[r], seam ? "phase applied at the seam" : "decoupled, as the reference renders it");
+            out += b;
+        }
+    const char* pm[] = {"off", "fused", "two_pass"};
+    switch (which) {
+        case 0:
+        case 3:
+            snprintf(b, sizeof(b), "e_march<%s, PAIR=%d, SEAM=%d>: %s%s\n", ty, (int)pair, (int)seam,
+                     launch_text(P.le).c_str(), which == 3 ? "; F is advanced by update_E's paired launch" : "");
+            out += b;
+            break;
+        case 1:
+        case 4:
+            snprintf(b, sizeof(b), "h_march<%s, PAIR=%d, SEAM=%d, MODE=%d>: Poynting %s; %s%s\n", ty, (int)pair,
+                     (int)seam, P.lh.mode, pm[d.poynting], launch_text(P.lh).c_str(),
+                     which == 4 ? "; G is advanced by update_H's paired launch" : "");
+            out += b;
+            break;
+        case 2:
+            if (d.poynting == FDTD_POYNTING_TWO_PASS)
+                snprintf(b, sizeof(b), "s_from_os<%s>: S from the 6N oS products, one thread per cell\n", ty);
+            else
+                snprintf(b, sizeof(b), "S is computed by h_march (fused Poynting); update_S enqueues nothing\n");
+            out += b;
+            break;
+        default:
+            if (he_supported(P)) {
+                out += "he_march<" + std::string(ty) + ">: H+S then E in one pass; " + launch_text(P.lhe) + "\n";
+                if (he2_supported(P)) {
+                    out += "he2_march<float>: two steps per pass; " + launch_text(P.lhe2) +
+                           (P.he2_pref ? " (chosen by the time model)\n" : " (not chosen: the one-step pass is faster)\n");
+                }
+            } else {
+                out += "step_n runs update_E and update_H per step (no fused pass for this plan)\n";
+            }
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fdtd_plan_describe(const void* plan, int32_t which, char* buf, int64_t size) {
+    if (!plan) return fail(1, "plan is NULL"), -1;
+    if (which < 0 || which > 5) return fail(3, "which = %d not in 0..5", which), -1;
+    const std::string s = describe(*(const Plan*)plan, which);
+    if (buf && size > 0) {
+        const int64_t n = (int64_t)s.size() < size - 1 ? (int64_t)s.size() : size - 1;
+        memcpy(buf, s.data(), (size_t)n);
+        buf[n] = 0;
+    }
+    return (int64_t)s.size();
+}
+
 int fdtd_update_he2_preferred(const void* plan) {
     return plan && he2_supported(*(const Plan*)plan) && ((const Plan*)plan)->he2_pref ? 1 : 0;
 }

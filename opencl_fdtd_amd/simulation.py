@@ -288,12 +288,16 @@ class Simulation:
             self._psi_g = zeros()
         self._inv_d = torch.cat(self.inv_dxes).contiguous() if self.inv_dxes is not None else None
 
-        names = {"float": "f32", "double": "f64"}[ctype]
-        pair = ", paired with F/G ({})".format(bloch_mode) if self.bloch_boundaries else ""
-        self.sources["E"] = "e_march<{}>{}: E-step, {} CPML faces".format(names, pair, len(self.pmls))
-        self.sources["H"] = "h_march<{}>{}: H-step, Poynting {}".format(names, pair, self.poynting_mode)
+        # sources (simulation.py:209-230 keeps the rendered template text of
+        # each update): the specialisation the plan chose for each update
+        self.sources["E"] = self._describe(0)
+        self.sources["H"] = self._describe(1)
         if self.do_poynting:
-            self.sources["S"] = ("fused into H" if self.poynting_mode == "fused" else "s_from_os<{}>".format(names))
+            self.sources["S"] = self._describe(2)
+        if self.bloch_boundaries:
+            self.sources["F"] = self._describe(3)
+            self.sources["G"] = self._describe(4)
+        self.sources["step_n"] = self._describe(5)
 
         self.update_E = self._update_E
         self.update_H = self._update_H
@@ -301,6 +305,14 @@ class Simulation:
         self.update_F = self._noop if self.bloch_boundaries else None
         self.update_G = self._noop if self.bloch_boundaries else None
         self.update_J = self._update_J if do_fieldsrc else None
+
+    def _describe(self, which: int) -> str:
+        n = int(self._lib.fdtd_plan_describe(self._plan, which, None, 0))
+        if n < 0:
+            _lib.check(1, "fdtd_plan_describe")
+        buf = ctypes.create_string_buffer(n + 1)
+        self._lib.fdtd_plan_describe(self._plan, which, buf, n + 1)
+        return buf.value.decode()
 
     # ------------------------------------------------------------------ fields
     def _create_field(self, initial_value):

@@ -86,28 +86,16 @@ def phc_slab(seed: int = 2, steps: int = 8000, shape=(395, 345, 73), pitch: floa
     (12.25, 12.25, 11.9), air holes of radius 7 on a triangular lattice of
     pitch 20 with an L3 defect (3 holes removed on the centre row); CPML-8;
     Ey dipole at the defect centre with the fdtd.py pulse at wavelength 45."""
+    from .geometry import draw_holes, ln_defect, slab_epsilon
     rng = numpy.random.default_rng(seed)
     sx, sy, sz = shape
     cx, cy, cz = sx // 2, sy // 2, sz // 2
     z0 = cz - thickness // 2
-    xs = numpy.arange(sx)[:, None] - cx
-    ys = numpy.arange(sy)[None, :] - cy
-    holes = numpy.zeros((sx, sy), bool)
-    row_h = pitch * numpy.sqrt(3) / 2
-    nrow = int(numpy.ceil(sy / row_h)) + 2
-    ncol = int(numpy.ceil(sx / pitch)) + 2
-    for j in range(-nrow, nrow + 1):
-        yc = j * row_h
-        off = 0.5 * pitch if (j % 2) else 0.0
-        for i in range(-ncol, ncol + 1):
-            xc = i * pitch + off
-            if j == 0 and i in (-1, 0, 1):
-                continue   # L3 defect
-            holes |= (xs - xc) ** 2 + (ys - yc) ** 2 <= radius ** 2
-    eps = [numpy.ones(shape) for _ in range(3)]
-    for comp, val in enumerate((12.25, 12.25, 11.9)):
-        layer = numpy.where(holes, 1.0, val)
-        eps[comp][:, :, z0:z0 + thickness] = layer[:, :, None]
+    # lattice sites (in pitches) reaching past every cell of the grid
+    mx = int(numpy.ceil((sx / 2 + radius) / pitch)) + 1
+    my = int(numpy.ceil((sy / 2 + radius) / (pitch * numpy.sqrt(3) / 2))) + 1
+    holes = draw_holes((sx, sy), ln_defect((mx, my), 3), (cx, cy), pitch=pitch, radius=radius)
+    eps = slab_epsilon(shape, holes, z0, thickness, (12.25, 12.25, 11.9))
     dt = 0.99 / numpy.sqrt(3)
     wl = 45.0
     phase = float(rng.uniform(0, 2 * numpy.pi))

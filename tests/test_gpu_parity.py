@@ -224,3 +224,24 @@ def test_bloch_supercell_c3_full_run_fp32_vs_fp64():
         assert max(errs) <= 1e-4, (name, errs)
     E = fetch(a, "E").reshape(3, *w.shape)
     assert numpy.all(E[:, :, :, 200:205] == 0.0)
+
+
+def test_sources_and_saved_fields(tmp_path):
+    """sim.sources holds each update's specialisation (E, H, S, and F, G with
+    Bloch), and save_fields writes the unvec'd E, H, S that load_fields reads
+    back, equal to the device fields (fdtd.py:145-156, 181-191)."""
+    from opencl_fdtd_amd import fieldio
+    eps, pmls, init = _tiny(shape=(9, 8, 7), T=2, seed=8)
+    sim, orc = build_pair(eps, pmls, init=init)
+    assert {"E", "H", "S", "step_n"} <= set(sim.sources) and "F" not in sim.sources
+    assert "e_march" in sim.sources["E"] and "thickness 2" in sim.sources["H"]
+    step_gpu(sim, 5)
+    step_oracle(orc, 5)
+    d = fieldio.load_fields(fieldio.save_fields(str(tmp_path / "sim"), sim, step=5))
+    for name in ("E", "H", "S"):
+        assert numpy.array_equal(fieldio.vec(d[name]), getattr(orc, name)), name
+    bl = [{"axis": "x", "real": 1.0, "imag": 0.0}]
+    simb, _ = build_pair(eps, [], bloch=bl)
+    assert {"F", "G"} <= set(simb.sources)
+    d = fieldio.field_dict(simb)
+    assert set(d) == {"E", "H", "S", "F", "G"}
