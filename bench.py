@@ -237,16 +237,74 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
     return out
 
 
+def learned_summary(steps: int = 100, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), refl_steps: int = 600):
+    """The metric's second half, sized for the default run: interior
+    Gcell-updates/s of the learned 4-cell shell and of CPML-16 on the same
+    256 x 256 x 128 interior, with Poynting off and on, and the reflection
+    (dB vs the 48-cell padded run) of the learned shell, its CPML-4 base and
+    CPML-16 on unseen scenarios (seeds never used in training)."""
+    import torch
+    from opencl_fdtd_amd import Simulation, learned as L
+    t_start = time.perf_counter()
+    net = L.ShellNet.load(L.WEIGHTS)
+    interior = (256, 256, 128)
+    n_int = int(numpy.prod(interior))
+    sc = L.make_scenario(seeds[0], interior=interior, steps=steps + warmup)
+    speed = {}
+    for poynting in (False, True):
+        for name, pad, faces, shell in (("cpml16", 16, L.cpml_faces(16), None), ("learned4", 4, net.faces(), net)):
+            shape, eps, idx, sl = L.embed(sc, pad)
+            sim = Simulation(eps, faces, dt=sc.dt, do_poynting=poynting)
+            wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device="cuda")
+            runner = L.ShellRunner(sim, shell) if shell is not None else None
+            run = (lambda n0, n: runner.step_n(n, idx, wave[n0:n0 + n])) if runner else \
+                  (lambda n0, n: sim.step_n(n, idx, wave[n0:n0 + n]))
+            run(0, warmup)
+            torch.cuda.synchronize()
+            st = torch.cuda.current_stream()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            run(warmup, steps)
+            b.record(st)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            speed["%s_poynting_%s" % (name, "on" if poynting else "off")] = {
+                "grid": list(shape), "ms_per_step": ms / steps, "interior_gcell_s": n_int * steps / ms / 1e6}
+            del sim, runner
+    ratio = {k: speed["learned4_poynting_" + k]["interior_gcell_s"] / speed["cpml16_poynting_" + k]["interior_gcell_s"]
+             for k in ("off", "on")}
+    base = L.ShellNet(net.T, weights=torch.zeros_like(net.master), base_pml=net.base_pml)
+    refl = []
+    for sd in seeds:
+        scr = L.make_scenario(sd, interior=interior, steps=refl_steps)
+        wide = L.run_interior(scr, L.WIDE_PAD, L.cpml_faces(L.WIDE_PAD), sample_every=10)
+        refl.append({"seed": sd, "wavelength": round(scr.wavelength, 1),
+                     "learned4": L.reflection_db(L.run_learned(scr, net, sample_every=10), wide),
+                     "cpml4_base": L.reflection_db(L.run_learned(scr, base, sample_every=10), wide),
+                     "cpml16": L.reflection_db(L.run_interior(scr, 16, L.cpml_faces(16), sample_every=10), wide)})
+        del wide
+        torch.cuda.empty_cache()
+    out = {"interior": list(interior), "timed_steps": steps, "speed": speed,
+           "learned4_vs_cpml16_interior_speedup": ratio,
+           "reflection_db": refl, "reflection_steps": refl_steps,
+           "worst_learned4_db": max(r["learned4"] for r in refl),
+           "mean_gain_over_base_db": float(numpy.mean([r["cpml4_base"] - r["learned4"] for r in refl])),
+           "seconds": round(time.perf_counter() - t_start, 1)}
+    print("bench: learned %s" % json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
 def configs_report(seconds: float = 1.0):
     """Throughput of every workload (C1 small 64^3, C2 README slab, C3 Bloch
-    supercell, C4 the adjoint's 128^3 forward grid): step_n timed with HIP
+    supercell, C4 the adjoint's 128^3 forward grid, a synthetic 512^3 grid
+    with CPML-8): step_n timed with HIP
     events, Gcell-updates/s and the fraction of 8 TB/s that the step's
     algorithmic bytes per cell-update represent.  One JSON line per config."""
     import torch
     from opencl_fdtd_amd import Simulation
     from opencl_fdtd_amd import workloads
     out = []
-    for name in ("small", "phc_slab", "bloch", "adjoint"):
+    for name in ("small", "phc_slab", "bloch", "adjoint", "large"):
         w = workloads.BUILDERS[name](steps=10)
         sim = Simulation(w.epsilon, [dict(p) for p in w.pmls], dt=w.dt, bloch_boundaries=w.bloch_boundaries,
                          do_poynting=w.do_poynting)
@@ -297,6 +355,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=200, help="steps timed kernel by kernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="leave out the learned-shell summary")
     ap.add_argument("--precision", action="store_true",
                     help="per-config FP32 vs FP64 and FP32 vs oracle report instead of the benchmark")
     ap.add_argument("--learned", action="store_true",
@@ -485,6 +544,10 @@ def main():
                      "steps_per_launch": 2 if dom == "HE2" else 1,
                      "other_kernel": {k: v for k, v in kern.items() if k != dom}},
     }
+    if not args.no_learned:
+        del sim
+        torch.cuda.empty_cache()
+        result["learned"] = learned_summary()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -177,7 +177,7 @@ int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t
  * the new E), psi_e is updated in place, psi_h -> psi_h_out.  Bit-identical to
  * fdtd_update_h, then fdtd_update_e on E_out := E, then fdtd_add_point.
  * Needs fdtd_update_he_supported(plan) (fused Poynting, no Bloch boundaries,
- * rows of at most 341 cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.
+ * rows of at most 512 cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.
  * With f->E_out set and a supported plan, fdtd_step_n uses it for all but
  * the first E and last H half-steps and arranges the count so that the new E
  * ends in f->E. */

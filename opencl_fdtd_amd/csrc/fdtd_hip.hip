@@ -114,6 +114,10 @@ namespace {
 #ifndef FDTD_HE_TARGET
 #define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
 #endif
+#ifndef FDTD_ROW_MAX
+#define FDTD_ROW_MAX 512             // fused-S H kernel and fused H+E pass: longest rows (a 1024-thread tile
+                                     // holds at least one owned row and its halo row)
+#endif
 
 constexpr int kBlock = 256;          // E kernel / H kernel without fused S
 constexpr int kCoefTab = 2048;       // largest coefficient block the kernels copy to LDS
@@ -2091,7 +2095,8 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
 
 int check_he_fields(const Plan& P, const fdtd_fields* f) {
     if (!he_supported(P))
-        return fail(3, "fdtd_update_he needs fused Poynting, no Bloch boundaries and rows of at most 341 cells");
+        return fail(3, "fdtd_update_he needs fused Poynting, no Bloch boundaries and rows of at most %d cells",
+                    FDTD_ROW_MAX);
     if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he needs E_out != E");
     return 0;
 }
@@ -2226,11 +2231,11 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     P->coef_elems = coef_off;
     P->le = make_launch(P->plane, P->sx, kBlock, kBlock, 0, kTargetBlocks);
     if (d.poynting == FDTD_POYNTING_FUSED) {
-        // LDS exchange for rows of up to 341 cells: whole rows per workgroup
+        // LDS exchange for rows of up to FDTD_ROW_MAX cells: whole rows per workgroup
         // plus one halo row, at most FDTD_H_TARGET workgroups (as many x
         // chunks per tile column as fit); longer rows: per-cell
         // re-evaluation of the in-plane neighbours
-        if (P->sz <= 341) {
+        if (P->sz <= FDTD_ROW_MAX) {
             const int hb = P->sz <= 128 ? FDTD_HB_SMALL : 1024;
             P->lh = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_H_TARGET);
         } else {
@@ -2242,7 +2247,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     }
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
-    if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= 341 && 3 * N * esz <= (int64_t)UINT32_MAX) {
+    if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= FDTD_ROW_MAX &&
+        3 * N * esz <= (int64_t)UINT32_MAX) {
         // one workgroup per CU at a time (1024 threads, <= 128 VGPRs)
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
         P->lhe = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_HE_TARGET);

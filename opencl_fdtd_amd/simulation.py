@@ -24,7 +24,7 @@ Differences from the reference, all documented behaviour:
   the seam; ``'decoupled'`` reproduces the reference as shipped.
 * ``step_n`` (n steps without host round trips) runs each H half-step and
   the next E half-step as one pass over the grid when the plan allows it
-  (fused Poynting, no Bloch boundaries, rows of at most 341 cells;
+  (fused Poynting, no Bloch boundaries, rows of at most 512 cells;
   ``fuse_steps=False`` turns it off and saves the second E buffer it needs),
   and two steps per pass where that is faster.  E may end in either of its
   two buffers: read ``sim.E`` after each call.  Results are bit-identical.
@@ -458,7 +458,7 @@ class Simulation:
         """``update_H`` (with S) of one step and ``update_E`` of the next,
         then ``E[src_index] += src_amp[0]`` (a one-value device tensor), in
         one pass over the grid.  Needs ``fuse_steps`` and a plan that
-        supports it (fused Poynting, no Bloch boundaries, sz <= 341).  The
+        supports it (fused Poynting, no Bloch boundaries, sz <= 512).  The
         new E is in a second buffer afterwards: read ``sim.E`` again."""
         if self._E_out is None:
             raise _lib.FdtdError("update_HE is not available for this simulation (fdtd_update_he_supported)")

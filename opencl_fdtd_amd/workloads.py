@@ -10,6 +10,7 @@ Each builder returns a ``Workload``: the constructor arguments of
   bloch      64x64x256 Bloch-periodic supercell, CPML-16 on z, PEC mirror
   adjoint    128^3, CPML-8, 32^3 design region
   learned    256x256x128 interior with absorbing padding of a given width
+  large      512^3, the small configuration's construction (HBM-scale grid)
 """
 from __future__ import annotations
 
@@ -158,8 +159,18 @@ def learned(pad: int, seed: int = 5, steps: int = 500, interior=(256, 256, 128),
                     src_wave=modulated_sine(steps, dt, 20.0, 30.0, 120.0))
 
 
+def large(seed: int = 6, steps: int = 200, n: int = 512) -> Workload:
+    """Synthetic 512^3 grid (134 M cells, 1.6 GB per field): the small
+    configuration's construction (eps = 1 with 20 boxes of eps ~ U[2, 12],
+    CPML-8 on all faces, Ez point source at the centre) at a size far beyond
+    the 256 MB Infinity Cache, for an HBM figure free of cache effects."""
+    w = small(seed=seed, steps=steps, n=n)
+    w.name = "large_%d" % n
+    return w
+
+
 BUILDERS: Dict[str, Callable[..., Workload]] = {
-    "small": small, "phc_slab": phc_slab, "bloch": bloch, "adjoint": adjoint,
+    "small": small, "phc_slab": phc_slab, "bloch": bloch, "adjoint": adjoint, "large": large,
 }
 
 

@@ -53,13 +53,15 @@ def _assert_same(a, b, what):
     ((11, 13, 9), 3, 3, numpy.float32, False),
     ((11, 13, 9), 3, 8, numpy.float32, True),
     ((9, 40, 73), 4, 7, numpy.float32, False),       # rows of 73: 6 rows + halo per 512-thread tile
-    ((6, 9, 200), 2, 5, numpy.float32, False),       # 1024-thread tiles (rows of 129..341 cells)
+    ((6, 9, 200), 2, 5, numpy.float32, False),       # 1024-thread tiles (rows of 129..512 cells)
     ((5, 37, 3), 2, 6, numpy.float32, False),        # short rows, many per tile
     ((33, 8, 17), 3, 9, numpy.float64, True),
     ((5, 6, 160), 2, 4, numpy.float64, False),       # FP64, 1024-thread tiles
     ((6, 7, 5), 4, 5, numpy.float32, False),         # slabs of both faces overlap on every axis
     ((2, 9, 10), 1, 4, numpy.float32, False),        # two planes: every chunk is one plane
-    ((3, 4, 341), 2, 3, numpy.float32, False),       # longest rows the fused pass takes
+    ((3, 4, 341), 2, 3, numpy.float32, False),       # two owned rows per tile
+    ((3, 4, 512), 2, 4, numpy.float32, False),       # longest rows the fused pass takes: one owned row per tile
+    ((4, 3, 400), 3, 5, numpy.float64, True),        # one owned row per tile, FP64, nonuniform
     ((5, 6, 1), 1, 6, numpy.float32, True),          # one-cell rows
 ])
 def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
@@ -81,6 +83,7 @@ def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
 @pytest.mark.parametrize("shape, T, n", [
     ((160, 300, 250), 8, 6),     # 12 M cells, 1024-thread tiles of 3 rows, several rounds of workgroups
     ((40, 700, 100), 6, 5),      # 78 tile columns of 9 rows, short x
+    ((24, 40, 512), 8, 5),       # rows of 512: one owned row and its halo row per tile
 ])
 def test_fused_large_grids_equal_unfused(shape, T, n):
     """Launch shapes the cost model gives larger grids (multiple rounds of

@@ -163,7 +163,7 @@ def test_workloads():
 
 def test_update_he_support_rules():
     """The fused H+E pass needs fused Poynting, no Bloch boundaries and
-    rows of at most 341 cells (plan-level query, no GPU)."""
+    rows of at most 512 cells (plan-level query, no GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -176,8 +176,8 @@ def test_update_he_support_rules():
         lib.fdtd_plan_destroy(plan)
         return r
     assert supported(poynting=1) == 1
-    assert supported(poynting=1, shape=(4, 4, 341)) == 1
-    assert supported(poynting=1, shape=(4, 4, 342)) == 0
+    assert supported(poynting=1, shape=(4, 4, 512)) == 1
+    assert supported(poynting=1, shape=(4, 4, 513)) == 0
     assert supported(poynting=0) == 0
     assert supported(poynting=2) == 0
     assert supported(poynting=1, bloch_mask=1) == 0
