@@ -327,11 +327,13 @@ def configs_report(seconds: float = 1.0):
         two = bool(sim.two_step_passes)
         if fused:
             # one he_march per step (he2_march per two steps): 18 words per
-            # cell, 8 per slab cell and face, per pass
+            # cell, 8 per slab cell and face, per pass; the Bloch-pair pass
+            # (heb_march) moves both systems: 30 words per cell, 16 per slab
+            # cell and face
             nc = float(w.cells)
             slab = sum(min(int(p.get("thickness", 8)), w.shape["xyz".find(p["axis"])]) * nc /
                        w.shape["xyz".find(p["axis"])] for p in w.pmls)
-            one = 4.0 * (18 + 8 * slab / nc)
+            one = 4.0 * ((30 + 16 * slab / nc) if w.bloch_boundaries else (18 + 8 * slab / nc))
             bpc = one / 2 if two else one
         else:
             bpc = one = workloads.algorithmic_bytes_per_cell(w.shape, w.pmls, poynting=w.do_poynting,

@@ -54,7 +54,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 3
+#define FDTD_ABI_VERSION 4
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -121,6 +121,9 @@ typedef struct {
                          NULL: no fusion */
     void* psi_e_out;  /* optional second psi_E block (!= psi_e): written by fdtd_update_he2;
                          with E_out it lets fdtd_step_n run two steps per pass.  NULL: none */
+    void* F_out;      /* optional second F buffer (3N, != F): with Bloch boundaries
+                         fdtd_update_he writes F there (with G_out != G, psi_g_out != psi_g).
+                         NULL: none */
 } fdtd_fields;
 
 int fdtd_abi_version(void);
@@ -176,8 +179,12 @@ int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t
  * not NULL, E_out[src_index] += src_amp[0].  E is left unchanged (E_out holds
  * the new E), psi_e is updated in place, psi_h -> psi_h_out.  Bit-identical to
  * fdtd_update_h, then fdtd_update_e on E_out := E, then fdtd_add_point.
- * Needs fdtd_update_he_supported(plan) (fused Poynting, no Bloch boundaries,
- * rows of at most 512 cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.
+ * Needs fdtd_update_he_supported(plan) (fused Poynting, rows of at most 512
+ * cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.  With Bloch
+ * boundaries (FP32, uniform dx) the pass advances the pair: F -> F_out and
+ * G -> G_out, psi_g -> psi_g_out, psi_f in place, all distinct; it is then
+ * bit-identical to the paired fdtd_update_h and fdtd_update_e (the four
+ * update_H/G/E/F launches of simulation.py:252-264).
  * With f->E_out set and a supported plan, fdtd_step_n uses it for all but
  * the first E and last H half-steps and arranges the count so that the new E
  * ends in f->E. */

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_FACES = 12
 
 F32, F64 = 0, 1
@@ -51,7 +51,7 @@ class Desc(ctypes.Structure):
 class Fields(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "E", "H", "H_out", "eps", "S", "oS", "F", "G", "G_out",
-        "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d", "E_out", "psi_e_out")]
+        "psi_e", "psi_h", "psi_h_out", "psi_f", "psi_g", "psi_g_out", "coef", "inv_d", "E_out", "psi_e_out", "F_out")]
 
 
 class AdjointBufs(ctypes.Structure):

@@ -114,6 +114,11 @@ namespace {
 #ifndef FDTD_HE_TARGET
 #define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
 #endif
+#ifndef FDTD_HEB_SPLIT
+#define FDTD_HEB_SPLIT 0             // Bloch-pair pass: the second system's loads after the first's H update
+                                     // (measured on the 64x64x256 Bloch supercell: 45.9 us per step against
+                                     // 36.6 us with every load of the plane up front)
+#endif
 #ifndef FDTD_ROW_MAX
 #define FDTD_ROW_MAX 512             // fused-S H kernel and fused H+E pass: longest rows (a 1024-thread tile
                                      // holds at least one owned row and its halo row)
@@ -964,12 +969,12 @@ __device__ __forceinline__ void h_fetch(const Geom<T>& g, const S& s, const Cell
     v.hz = ldc<FDTD_NT_H>(s.Hin, 2, ob + c.op);
     if constexpr (SEAM) {
         if ((g.seam_mask & 2) && c.y == g.sy - 1) {
-            v.ex_y = seam_mix(v.ex_y, ld(s.P[0], ob + c.op_y), s.sa[1], s.sb[1]);
-            v.ez_y = seam_mix(v.ez_y, ld(s.P[2], ob + c.op_y), s.sa[1], s.sb[1]);
+            v.ex_y = seam_mix(v.ex_y, ldc<false>(s.P, 0, ob + c.op_y), s.sa[1], s.sb[1]);
+            v.ez_y = seam_mix(v.ez_y, ldc<false>(s.P, 2, ob + c.op_y), s.sa[1], s.sb[1]);
         }
         if ((g.seam_mask & 4) && c.z == g.sz - 1) {
-            v.ex_z = seam_mix(v.ex_z, ld(s.P[0], ob + c.op_z), s.sa[2], s.sb[2]);
-            v.ey_z = seam_mix(v.ey_z, ld(s.P[1], ob + c.op_z), s.sa[2], s.sb[2]);
+            v.ex_z = seam_mix(v.ex_z, ldc<false>(s.P, 0, ob + c.op_z), s.sa[2], s.sb[2]);
+            v.ey_z = seam_mix(v.ey_z, ldc<false>(s.P, 1, ob + c.op_z), s.sa[2], s.sb[2]);
         }
     }
 }
@@ -985,8 +990,8 @@ __device__ __forceinline__ HCell<T> h_update(const Geom<T>& g, const S& s, const
     T ey1 = v.eyn, ez1 = v.ezn;
     if constexpr (SEAM) {
         if ((g.seam_mask & 1) && x == g.sx - 1) {
-            ey1 = seam_mix(ey1, ld(s.P[1], c.op), s.sa[0], s.sb[0]);
-            ez1 = seam_mix(ez1, ld(s.P[2], c.op), s.sa[0], s.sb[0]);
+            ey1 = seam_mix(ey1, ldc<false>(s.P, 1, c.op), s.sa[0], s.sb[0]);
+            ez1 = seam_mix(ez1, ldc<false>(s.P, 2, c.op), s.sa[0], s.sb[0]);
         }
     }
     T dxy = (v.ex_y - ex) * iy;
@@ -1446,6 +1451,203 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
         qy = cur.h.eyn;
         qz = cur.h.ezn;
         if (PF) cur = nxt;
+        x = xn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The fused pass for a Bloch pair (heb_march): he_march's H+S then E pass for
+// the two coupled systems at once, E/H (with S) and their Bloch partners F/G
+// (simulation.py:187-196, 214-224, 252-264 launch four kernels per step for
+// them).  Per plane x, every thread of the tile evaluates H^{n+1/2} and
+// G^{n+1/2} (seam-mixed E/F neighbours across x, y and z seams as
+// h_march<PAIR, SEAM>); the oS products of the E/H system and both new H
+// and G go through LDS, then the owned cells finish S and update E and F,
+// mixing H and G of the -y / -z neighbours (LDS) and of the -x neighbour
+// (registers) across a seam as e_march<PAIR, SEAM> does.  Every buffer is
+// read from a copy the pass does not write: E -> E_out, F -> F_out,
+// H -> H_out, G -> G_out, psi_h -> psi_h_out, psi_g -> psi_g_out (psi_e and
+// psi_f, touched only by their own cell, in place).  Per cell the
+// arithmetic is that of h_update + S and e_update for each system, in the
+// same order, so the results are bit-identical to the paired h_march and
+// e_march.  FP32 with the coefficient block in LDS.
+// ---------------------------------------------------------------------------
+template <class T>
+struct HSysB {
+    Packed3<const T> E, Hin, P;     // P: the partner of E across a seam
+    Packed3<T> Hout;
+    const T* psi_in;
+    T* psi_out;
+    T sa[3], sb[3];                 // seam neighbour = sa*own + sb*partner
+};
+
+template <class T>
+struct HEBArgs {
+    Geom<T> g;
+    HSysB<T> hs[2];                 // 0: H from E (partner F), 1: G from F (partner E)
+    ESysP<T> es[2];                 // 0: E_out, psi_e; 1: F_out, psi_f
+    T esa[2][3], esb[2][3];         // E-step seams: 0: H with partner G, 1: G with partner H
+    Packed3<T> S;
+    Packed3<const T> eps;
+    const T* src_amp;               // E_out[src] += *src_amp after the E update; null: none
+    int src_x, src_p, src_comp;
+    int coef_elems;
+};
+
+template <class T, bool SEAM>
+__global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
+    constexpr int HB = 1024;
+    __shared__ T lds[2][10][HB];
+    __shared__ T ctab[kCoefTab];
+    const Geom<T>& g = a.g;
+    int pb, ck;
+    block_coords(g.pblocks, g.nchunk, &pb, &ck);
+    const int t = threadIdx.x;
+    const int halo = g.sz;
+    const int praw = pb * g.tile - halo + t;
+    const bool valid = (t < g.tile + halo) && praw < g.plane;
+    const int p = praw < 0 ? praw + g.plane : praw;
+    const bool own = valid && t >= halo;
+    const Cell c = make_cell<T>(g, valid ? p : 0);
+    const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
+    const int x0 = ck * g.xchunk;
+    const int x1 = min(x0 + g.xchunk, g.sx);
+    const T sf = g.dt * (T)0.125;
+    const T iu = g.inv_u;
+    const int xs = (x0 == 0) ? g.sx - 1 : x0 - 1;
+    const int n_it = (x1 - x0) + 1;
+    const int src_comp = (own && a.src_amp != nullptr && p == a.src_p) ? a.src_comp : -1;
+    for (int i = t; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
+    lds_barrier();
+    const T* tab = ctab;
+    CpmlCols<T> cols;
+    cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
+    cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
+    const int ty = t - g.sz;
+    const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
+
+    T q[2][3];                      // E^n and F^n at the current plane
+    {
+        const uint32_t ob = (uint32_t)xs * pbytes + c.op;
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) q[k][r] = ldc<false>(a.hs[k].E, r, ob);
+    }
+    T oSyx_m = (T)0, oSzx_m = (T)0;           // oS products of the -x neighbour
+    T hym[2] = {(T)0, (T)0}, hzm[2] = {(T)0, (T)0};   // H^{n+1/2}, G^{n+1/2} of the -x neighbour
+    int x = xs;
+    for (int it = 0; it < n_it; ++it) {
+        const int xn = (x == g.sx - 1) ? 0 : x + 1;
+        const bool dry = it == 0;   // uniform
+        const uint32_t oi = (uint32_t)x * pbytes + c.op;
+        // ---- loads of the plane: both H-step inputs, the four psi pairs, eps
+        cols.hx = slab_hit(g, 0, x, g.sx);
+        HIn<T> in[2];
+        PsiPair<T> qh, qe[2];
+        T e0 = (T)1, e1 = (T)1, e2 = (T)1;
+        // ---- H^{n+1/2} and G^{n+1/2} on every row of the tile; oS of E/H.
+        // FDTD_HEB_SPLIT: the second system's loads (and the E-step loads)
+        // are issued after the first system's H update, which keeps fewer
+        // values live at once.
+        T h1[2][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
+        T oSxy = (T)0, oSxz = (T)0, oSyz = (T)0, oSyx = (T)0, oSzx = (T)0, oSzy = (T)0;
+        const auto fetch_e = [&]() {
+            if (own && !dry) {
+                psi_fetch_cols(g, (const T*)a.es[0].psi, x, c, cols, qe[0]);
+                psi_fetch_cols(g, (const T*)a.es[1].psi, x, c, cols, qe[1]);
+                e0 = ldc<FDTD_NT_E>(a.eps, 0, oi);
+                e1 = ldc<FDTD_NT_E>(a.eps, 1, oi);
+                e2 = ldc<FDTD_NT_E>(a.eps, 2, oi);
+            }
+        };
+        if (!FDTD_HEB_SPLIT) {
+            h_fetch<T, SEAM>(g, a.hs[1], c, x, pbytes, in[1]);
+            fetch_e();
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k == 0 || FDTD_HEB_SPLIT) {
+                if (valid) psi_fetch_cols(g, a.hs[k].psi_in, x, c, cols, qh);
+                h_fetch<T, SEAM>(g, a.hs[k], c, x, pbytes, in[k]);
+                if (k == 1) fetch_e();
+            } else if (valid) {
+                psi_fetch_cols(g, a.hs[k].psi_in, x, c, cols, qh);
+            }
+            if (valid) {
+                const HCell<T> h = h_update<T, SEAM, true>(g, a.hs[k], c, x, pbytes, q[k][0], q[k][1], q[k][2], in[k],
+                                                           iu, iu, iu, own && !dry, (const CpmlCols<T>*)&cols,
+                                                           nullptr, &qh, tab);
+                h1[k][0] = h.hx1; h1[k][1] = h.hy1; h1[k][2] = h.hz1;
+                if (k == 0) {
+                    const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+                    oSxy = h.aEyx * aHz;
+                    oSxz = -h.aEzx * aHy;
+                    oSyz = h.aEzy * aHx;
+                    oSyx = -h.aExy * aHz;
+                    oSzx = h.aExz * aHy;
+                    oSzy = -h.aEyz * aHx;
+                }
+            }
+        }
+        const int b = it & 1;
+        lds[b][0][t] = oSxy;
+        lds[b][1][t] = oSzy;
+        lds[b][2][t] = oSxz;
+        lds[b][3][t] = oSyz;
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) lds[b][4 + 3 * k + r][t] = h1[k][r];
+        lds_barrier();
+        // ---- S, E^{n+1} and F^{n+1} on the owned cells
+        if (own && !dry) {
+            const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
+            const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
+            stc<FDTD_NT_H>(a.S, 0, oi, (oSxy + oSxz + nxy + nxz) * sf);
+            stc<FDTD_NT_H>(a.S, 1, oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+            stc<FDTD_NT_H>(a.S, 2, oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            const bool src = src_comp >= 0 && x == a.src_x;
+            const T amp = src ? *a.src_amp : (T)0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int o = 4 + 3 * k, po = 4 + 3 * (1 - k);
+                EIn<T> v;
+                v.hx = h1[k][0]; v.hy = h1[k][1]; v.hz = h1[k][2];
+                v.hx_ym = lds[b][o + 0][ty];
+                v.hz_ym = lds[b][o + 2][ty];
+                v.hx_zm = lds[b][o + 0][tz];
+                v.hy_zm = lds[b][o + 1][tz];
+                T hy_x = hym[k], hz_x = hzm[k];
+                if constexpr (SEAM) {
+                    if ((g.seam_mask & 2) && c.y == 0) {
+                        v.hx_ym = seam_mix(v.hx_ym, lds[b][po + 0][ty], a.esa[k][1], a.esb[k][1]);
+                        v.hz_ym = seam_mix(v.hz_ym, lds[b][po + 2][ty], a.esa[k][1], a.esb[k][1]);
+                    }
+                    if ((g.seam_mask & 4) && c.z == 0) {
+                        v.hx_zm = seam_mix(v.hx_zm, lds[b][po + 0][tz], a.esa[k][2], a.esb[k][2]);
+                        v.hy_zm = seam_mix(v.hy_zm, lds[b][po + 1][tz], a.esa[k][2], a.esb[k][2]);
+                    }
+                    if ((g.seam_mask & 1) && x == 0) {
+                        hy_x = seam_mix(hym[k], hym[1 - k], a.esa[k][0], a.esb[k][0]);
+                        hz_x = seam_mix(hzm[k], hzm[1 - k], a.esa[k][0], a.esb[k][0]);
+                    }
+                }
+                v.ex = q[k][0]; v.ey = q[k][1]; v.ez = q[k][2];
+                e_update<T, true>(g, a.es[k], c, x, oi, v, hy_x, hz_x, e0, e1, e2, iu, iu, iu,
+                                  k == 0 && src ? src_comp : -1, amp, (const CpmlCols<T>*)&cols, nullptr, &qe[k], tab);
+            }
+        }
+        oSyx_m = oSyx;
+        oSzx_m = oSzx;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            hym[k] = h1[k][1];
+            hzm[k] = h1[k][2];
+            q[k][0] = in[k].exn;
+            q[k][1] = in[k].eyn;
+            q[k][2] = in[k].ezn;
+        }
         x = xn;
     }
 }
@@ -2036,13 +2238,60 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     return check_launch("fdtd_update_h");
 }
 
+// (lhe is set up only where the pass applies: fused Poynting, rows of at
+// most FDTD_ROW_MAX cells, 3N values within 32-bit byte offsets, and, with
+// Bloch boundaries, FP32, uniform dx and the coefficient block in LDS)
 bool he_supported(const Plan& P) {
-    return P.d.poynting == FDTD_POYNTING_FUSED && P.d.bloch_mask == 0 && P.lhe.mode == 2;
+    return P.d.poynting == FDTD_POYNTING_FUSED && P.lhe.mode == 2;
+}
+
+int launch_heb(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index) {
+    using T = float;
+    HEBArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    fill_geom<T>(P, P.lhe, f, a.g);
+    const uint32_t nb = (uint32_t)(P.N * (int64_t)sizeof(T));
+    const double* re = P.d.bloch_re;
+    const double* im = P.d.bloch_im;
+    // H from E (partner F): seam E(s) = (E + iF)(c + is) -> Ec - Fs; G from F: Fc + Es
+    a.hs[0] = {{(const T*)f->E, nb}, {(const T*)f->H, nb}, {(const T*)f->F, nb}, {(T*)f->H_out, nb},
+               (const T*)f->psi_h, (T*)f->psi_h_out, {}, {}};
+    a.hs[1] = {{(const T*)f->F, nb}, {(const T*)f->G, nb}, {(const T*)f->E, nb}, {(T*)f->G_out, nb},
+               (const T*)f->psi_g, (T*)f->psi_g_out, {}, {}};
+    for (int r = 0; r < 3; ++r) {
+        a.hs[0].sa[r] = (T)re[r]; a.hs[0].sb[r] = (T)(-im[r]);
+        a.hs[1].sa[r] = (T)re[r]; a.hs[1].sb[r] = (T)im[r];
+        // E from H (partner G): seam H(-1) = (H + iG)(c - is) -> Hc + Gs; F from G: Gc - Hs
+        a.esa[0][r] = (T)re[r]; a.esb[0][r] = (T)im[r];
+        a.esa[1][r] = (T)re[r]; a.esb[1][r] = (T)(-im[r]);
+    }
+    a.es[0] = {{(T*)f->E_out, nb}, (T*)f->psi_e};
+    a.es[1] = {{(T*)f->F_out, nb}, (T*)f->psi_f};
+    a.S = {(T*)f->S, nb};
+    a.eps = {(const T*)f->eps, nb};
+    a.coef_elems = (int)P.coef_elems;
+    a.src_amp = src_amp;
+    if (src_amp) {
+        const int64_t N = P.N;
+        const int64_t comp = src_index / N;
+        const int64_t i = src_index - comp * N;
+        a.src_comp = (int)comp;
+        a.src_x = (int)(i / P.plane);
+        a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
+    }
+    const dim3 grid(P.lhe.pblocks * P.lhe.nchunk);
+    if (P.d.bloch_mode == FDTD_BLOCH_PHASE) hipLaunchKernelGGL((heb_march<T, true>), grid, dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL((heb_march<T, false>), grid, dim3(1024), 0, st, a);
+    return check_launch("fdtd_update_he");
 }
 
 template <class T>
 int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_amp, int64_t src_index,
               const float* ycorr = nullptr, int ycT = 0, int64_t ycells = 0) {
+    if (P.d.bloch_mask) {
+        if (sizeof(T) != 4 || ycorr) return fail(3, "the Bloch-pair pass is FP32 and takes no shell correction");
+        return launch_heb(P, f, st, (const float*)src_amp, src_index);
+    }
     HEArgs<T> a;
     memset(&a, 0, sizeof(a));
     if (ycorr) {
@@ -2095,9 +2344,15 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
 
 int check_he_fields(const Plan& P, const fdtd_fields* f) {
     if (!he_supported(P))
-        return fail(3, "fdtd_update_he needs fused Poynting, no Bloch boundaries and rows of at most %d cells",
-                    FDTD_ROW_MAX);
+        return fail(3, "fdtd_update_he needs fused Poynting, rows of at most %d cells and, with Bloch boundaries, "
+                       "FP32, uniform dx and at most %d CPML coefficients", FDTD_ROW_MAX, kCoefTab);
     if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he needs E_out != E");
+    if (P.d.bloch_mask) {
+        if (!f->F_out || f->F_out == f->F) return fail(2, "fdtd_update_he with Bloch boundaries needs F_out != F");
+        if (!f->G_out || f->G_out == f->G) return fail(2, "fdtd_update_he with Bloch boundaries needs G_out != G");
+        if (P.d.n_face > 0 && (!f->psi_g_out || f->psi_g_out == f->psi_g))
+            return fail(2, "fdtd_update_he with Bloch boundaries needs psi_g_out != psi_g");
+    }
     return 0;
 }
 
@@ -2247,7 +2502,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     }
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
-    if (d.poynting == FDTD_POYNTING_FUSED && d.bloch_mask == 0 && P->sz <= FDTD_ROW_MAX &&
+    const bool pair_ok = d.bloch_mask == 0 || (d.dtype == FDTD_F32 && !d.nonuniform && P->coef_elems <= kCoefTab);
+    if (d.poynting == FDTD_POYNTING_FUSED && pair_ok && P->sz <= FDTD_ROW_MAX &&
         3 * N * esz <= (int64_t)UINT32_MAX) {
         // one workgroup per CU at a time (1024 threads, <= 128 VGPRs)
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
@@ -2262,7 +2518,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         for (int j = k + 1; j < d.n_face; ++j)
             if (d.face[k].axis == d.face[j].axis && P->face[k].lo < P->face[j].hi && P->face[j].lo < P->face[k].hi)
                 apart = false;
-    if (FDTD_HE2 && P->lhe.mode == 2 && d.dtype == FDTD_F32 && P->sz <= 256 && P->coef_elems <= kCoefTab && apart)
+    if (FDTD_HE2 && P->lhe.mode == 2 && d.bloch_mask == 0 && d.dtype == FDTD_F32 && P->sz <= 256 &&
+        P->coef_elems <= kCoefTab && apart)
         P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
     // Time per step of each pass: rounds of workgroups x planes a chunk
     // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
@@ -2415,7 +2672,10 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     // H swaps n times in all, as unfused): 2 * (m / 4) he2_march passes,
     // then of the rest an even number of he_march and at most one unfused.
     // Otherwise m / 2 he2_march passes and at most one he_march.
-    const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P);
+    const bool pair_bufs = !P.d.bloch_mask ||
+                           (f->F_out && f->F_out != f->F && f->G_out && f->G_out != f->G &&
+                            (P.d.n_face == 0 || (f->psi_g_out && f->psi_g_out != f->psi_g)));
+    const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P) && pair_bufs;
     const bool fuse2 = fuse && he2_supported(P) && P.he2_pref &&
                        (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
@@ -2437,6 +2697,11 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
             std::swap(cur.E, cur.E_out);
+            if (P.d.bloch_mask) {
+                std::swap(cur.F, cur.F_out);
+                std::swap(cur.G, cur.G_out);
+                std::swap(cur.psi_g, cur.psi_g_out);
+            }
         }
         for (int j = 0; j < n2; ++j, k += 2) {
             if (int rc = launch_he2(P, &cur, st, (const float*)amp_of(k), src_index)) return rc;
@@ -2483,6 +2748,7 @@ int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, co
     if (src_amp && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
     hipStream_t st = (hipStream_t)stream;
+    if (P.d.bloch_mask) return launch_heb(P, f, st, (const float*)src_amp, src_index);
     if (P.d.dtype == FDTD_F32) return launch_he<float>(P, f, st, (const float*)src_amp, src_index);
     return launch_he<double>(P, f, st, (const double*)src_amp, src_index);
 }

@@ -24,7 +24,8 @@ Differences from the reference, all documented behaviour:
   the seam; ``'decoupled'`` reproduces the reference as shipped.
 * ``step_n`` (n steps without host round trips) runs each H half-step and
   the next E half-step as one pass over the grid when the plan allows it
-  (fused Poynting, no Bloch boundaries, rows of at most 512 cells;
+  (fused Poynting, rows of at most 512 cells; with Bloch boundaries FP32 and
+  uniform dx, and then the pass advances E/H and F/G together;
   ``fuse_steps=False`` turns it off and saves the second E buffer it needs),
   and two steps per pass where that is faster.  E may end in either of its
   two buffers: read ``sim.E`` after each call.  Results are bit-identical.
@@ -216,10 +217,13 @@ class Simulation:
         if pingpong:
             self._H.append(torch.empty_like(self._H[0]))
         self._h = 0
-        self._F = self._G = None
+        self._F = None
+        self._G = None        # list: G, and with fused Poynting its ping-pong partner (indexed like H)
         if self.bloch_boundaries:
             self._F = self._create_field(initial_fields.get("F"))
-            self._G = self._create_field(initial_fields.get("G"))
+            self._G = [self._create_field(initial_fields.get("G"))]
+            if pingpong:
+                self._G.append(torch.empty_like(self._G[0]))
         self._S = self._oS = None
         if self.do_poynting:
             self._S = torch.zeros_like(self._E)
@@ -283,9 +287,12 @@ class Simulation:
             self._psi_e_out = zeros()
         self._psi_h = [zeros()] + ([zeros()] if pingpong else [])
         self._psi_f = self._psi_g = None
+        self._F_out = None
         if self.bloch_boundaries:
             self._psi_f = zeros()
-            self._psi_g = zeros()
+            self._psi_g = [zeros()] + ([zeros()] if pingpong else [])
+            if self._E_out is not None:
+                self._F_out = torch.empty_like(self._F)   # the fused pass moves F with E
         self._inv_d = torch.cat(self.inv_dxes).contiguous() if self.inv_dxes is not None else None
 
         # sources (simulation.py:209-230 keeps the rendered template text of
@@ -342,7 +349,7 @@ class Simulation:
     @property
     def G(self) -> Optional[DeviceField]:
         self.flush()
-        return None if self._G is None else self._G.as_subclass(DeviceField)
+        return None if self._G is None else self._G[self._h if len(self._G) == 2 else 0].as_subclass(DeviceField)
 
     @property
     def S(self) -> Optional[DeviceField]:
@@ -384,14 +391,21 @@ class Simulation:
         f.S = ptr(self._S)
         f.oS = ptr(self._oS)
         f.F = ptr(self._F)
-        f.G = ptr(self._G)
-        f.G_out = f.G
+        f.F_out = ptr(self._F_out)
+        if self._G is not None:
+            f.G = ptr(self._G[self._h if len(self._G) == 2 else 0])
+            f.G_out = ptr(self._G[1 - self._h]) if len(self._G) == 2 else f.G
+        else:
+            f.G = f.G_out = None
         f.psi_e = ptr(self._psi_e)
         f.psi_h = ptr(self._psi_h[self._h if len(self._psi_h) == 2 else 0])
         f.psi_h_out = ptr(self._psi_h[1 - self._h]) if len(self._psi_h) == 2 else f.psi_h
         f.psi_f = ptr(self._psi_f)
-        f.psi_g = ptr(self._psi_g)
-        f.psi_g_out = f.psi_g
+        if self._psi_g is not None:
+            f.psi_g = ptr(self._psi_g[self._h if len(self._psi_g) == 2 else 0])
+            f.psi_g_out = ptr(self._psi_g[1 - self._h]) if len(self._psi_g) == 2 else f.psi_g
+        else:
+            f.psi_g = f.psi_g_out = None
         f.coef = ptr(self._coef)
         f.inv_d = ptr(self._inv_d)
         f.E_out = ptr(self._E_out)
@@ -414,10 +428,16 @@ class Simulation:
             ev = self._launch(self._lib.fdtd_update_he, events, "fdtd_update_he", ctypes.c_int64(-1),
                               ctypes.c_void_p(None))
             self._h = 1 - self._h
-            self._E, self._E_out = self._E_out, self._E
+            self._swap_e()
             return ev
         self.flush()
         return self._launch(self._lib.fdtd_update_e, events, "fdtd_update_e")
+
+    def _swap_e(self) -> None:
+        """E (and F with Bloch boundaries) moved to the other buffer."""
+        self._E, self._E_out = self._E_out, self._E
+        if self._F_out is not None:
+            self._F, self._F_out = self._F_out, self._F
 
     def _update_H(self, events: Sequence = ()) -> Event:
         self.flush()
@@ -458,7 +478,8 @@ class Simulation:
         """``update_H`` (with S) of one step and ``update_E`` of the next,
         then ``E[src_index] += src_amp[0]`` (a one-value device tensor), in
         one pass over the grid.  Needs ``fuse_steps`` and a plan that
-        supports it (fused Poynting, no Bloch boundaries, sz <= 512).  The
+        supports it (fused Poynting, sz <= 512; with Bloch boundaries FP32 and
+        uniform dx, F moving with E).  The
         new E is in a second buffer afterwards: read ``sim.E`` again."""
         if self._E_out is None:
             raise _lib.FdtdError("update_HE is not available for this simulation (fdtd_update_he_supported)")
@@ -473,7 +494,7 @@ class Simulation:
                           ctypes.c_int64(int(src_index) if amp is not None else -1),
                           ctypes.c_void_p(amp.data_ptr() if amp is not None else None))
         self._h = 1 - self._h
-        self._E, self._E_out = self._E_out, self._E
+        self._swap_e()
         return ev
 
     def update_HE2(self, events: Sequence = (), src_index: Optional[int] = None, src_amp=None) -> Event:
@@ -532,7 +553,7 @@ class Simulation:
         if len(self._H) == 2 and moved.value & 4:
             self._h = 1 - self._h
         if moved.value & 1:
-            self._E, self._E_out = self._E_out, self._E
+            self._swap_e()
         if moved.value & 2:
             self._psi_e, self._psi_e_out = self._psi_e_out, self._psi_e
         return Event(stream)
@@ -546,7 +567,9 @@ class Simulation:
         st = {"E": self._E.clone(), "H": self._H[self._h].clone(), "psi_e": self._psi_e.clone(),
               "psi_h": self._psi_h[self._h if len(self._psi_h) == 2 else 0].clone()}
         if self._F is not None:
-            st.update(F=self._F.clone(), G=self._G.clone(), psi_f=self._psi_f.clone(), psi_g=self._psi_g.clone())
+            g = self._h if len(self._G) == 2 else 0
+            st.update(F=self._F.clone(), G=self._G[g].clone(), psi_f=self._psi_f.clone(),
+                      psi_g=self._psi_g[g].clone())
         return st
 
     def restore(self, st: Dict[str, torch.Tensor]) -> None:
@@ -556,10 +579,11 @@ class Simulation:
         self._psi_e.copy_(st["psi_e"])
         self._psi_h[self._h if len(self._psi_h) == 2 else 0].copy_(st["psi_h"])
         if self._F is not None:
+            g = self._h if len(self._G) == 2 else 0
             self._F.copy_(st["F"])
-            self._G.copy_(st["G"])
+            self._G[g].copy_(st["G"])
             self._psi_f.copy_(st["psi_f"])
-            self._psi_g.copy_(st["psi_g"])
+            self._psi_g[g].copy_(st["psi_g"])
 
     @property
     def fused_steps(self) -> bool:

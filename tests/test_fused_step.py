@@ -6,6 +6,8 @@ The fused pass does, per cell, the same arithmetic in the same order as the
 H kernel followed by the E kernel, so every comparison here is bit-exact:
 fields, S and both CPML psi blocks.
 """
+import math
+
 import numpy
 import pytest
 import torch
@@ -353,7 +355,11 @@ def test_fusion_is_off_where_unsupported():
     assert Simulation(eps, [], poynting_mode="two_pass")._E_out is None
     assert Simulation(eps, [], do_poynting=False)._E_out is None
     bloch = [{"axis": "x", "real": 1.0, "imag": 0.0}]
-    assert Simulation(eps, [], bloch_boundaries=bloch)._E_out is None
+    sb = Simulation(eps, [], bloch_boundaries=bloch)
+    assert sb._E_out is not None and sb._F_out is not None and not sb.two_step_passes
+    assert Simulation(eps, [], bloch_boundaries=bloch, float_type=numpy.float64)._E_out is None
+    nu = [[numpy.ones(n) for n in shape], [numpy.ones(n) for n in shape]]
+    assert Simulation(eps, [], bloch_boundaries=bloch, dxes=nu)._E_out is None
     s = Simulation(eps, [], poynting_mode="two_pass")
     with pytest.raises(Exception):
         s.update_HE([])
@@ -363,3 +369,73 @@ def test_fusion_is_off_where_unsupported():
     assert not Simulation(eps, [], float_type=numpy.float64).two_step_passes
     with pytest.raises(Exception):
         Simulation(eps, [], float_type=numpy.float64).update_HE2([])
+
+
+def _bloch_case(shape, T, seed, axes="xyz"):
+    rng = numpy.random.default_rng(seed)
+    eps = [rng.uniform(1, 6, shape) for _ in range(3)]
+    for e in eps:
+        e[shape[0] // 2, :, :shape[2] // 3] = numpy.inf
+    bl = [{"axis": a, "real": math.cos(ph), "imag": math.sin(ph)} for a, ph in zip(axes, (0.7, -1.9, 2.4))]
+    pmls = [{"axis": "z", "polarity": p, "thickness": T} for p in "np"] if T else []
+    init = {k: random_fields(rng, shape) for k in "EHFG"}
+    return eps, pmls, bl, init
+
+
+@pytest.mark.parametrize("mode", ["phase", "decoupled"])
+@pytest.mark.parametrize("shape, T, axes", [
+    ((8, 10, 12), 3, "xy"),      # one tile per plane
+    ((6, 24, 40), 4, "xyz"),     # seams on every axis, several tiles per plane
+    ((5, 9, 256), 16, "xy"),     # rows of 256 (the Bloch supercell's): three owned rows per tile
+    ((4, 3, 512), 2, "xz"),      # one owned row per tile, z seam inside the row
+])
+def test_bloch_pair_pass_equals_paired_kernels(mode, shape, T, axes):
+    """The fused Bloch-pair pass (update_HE on a Bloch plan) against the
+    paired h_march + e_march, bit for bit: E, H, F, G, S and the psi blocks,
+    with a source."""
+    from opencl_fdtd_amd import Simulation
+    print("case bloch pair pass", mode, shape, T, axes, flush=True)
+    eps, pmls, bl, init = _bloch_case(shape, T, seed=sum(shape), axes=axes)
+    mk = lambda fuse: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], bloch_boundaries=bl,
+                                 initial_fields=init, bloch_mode=mode, fuse_steps=fuse, defer_h=False)
+    a, b = mk(True), mk(False)
+    assert a._E_out is not None and b._E_out is None
+    for s in (a, b):                  # non-zero psi first
+        for _ in range(2):
+            s.update_E([])
+            s.update_H([])
+    idx = 1 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + 1
+    for amp in (0.75, -1.5, 2.0):
+        t_amp = torch.tensor([amp], dtype=torch.float32, device="cuda")
+        a.update_HE([], idx, t_amp)
+        b.update_H([])
+        b.update_E([])
+        b.E[idx] += t_amp[0]
+        torch.cuda.synchronize()
+        st = lambda s: dict(_state(s), F=fetch(s, "F"), G=fetch(s, "G"), psi_f=s._psi_f.cpu().numpy(),
+                            psi_g=s._psi_g[s._h].cpu().numpy())
+        _assert_same(st(a), st(b), "Bloch pair pass amp %g" % amp)
+
+
+def test_bloch_step_n_and_loop_match_oracle():
+    """step_n and the per-call loop (update_H deferred into the fused Bloch
+    pair pass) against the oracle with Bloch phases on x and y, CPML on z
+    and eps = inf cells, on rows of 256 cells."""
+    print("case bloch step_n vs oracle", flush=True)
+    eps, pmls, bl, init = _bloch_case((6, 12, 256), 8, seed=5, axes="xy")
+    sim, orc = build_pair(eps, pmls, bloch=bl, init=init)
+    assert sim._F_out is not None
+    sim.step_n(7)
+    for _ in range(5):
+        sim.update_E([])
+        sim.update_F([])
+        sim.update_H([])
+        sim.update_G([])
+        sim.update_S([]).wait()
+    sim.step_n(4)
+    for _ in range(16):
+        orc.step()
+    for name in ("E", "H", "F", "G", "S"):
+        g, o = fetch(sim, name), getattr(orc, name)
+        bad = numpy.flatnonzero(g != o)
+        assert bad.size == 0, (name, bad.size, bad[:4])

@@ -34,7 +34,7 @@ def test_struct_layouts_match_header():
     # fdtd_face: 4 int32; fdtd_desc: 3 int64 + 2 int32 + 12 faces + 2 double + 4 int32 + 6 double
     assert ctypes.sizeof(_lib.Face) == 16
     assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
-    assert ctypes.sizeof(_lib.Fields) == 19 * 8
+    assert ctypes.sizeof(_lib.Fields) == 20 * 8
     assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4
 
 
@@ -162,8 +162,8 @@ def test_workloads():
 
 
 def test_update_he_support_rules():
-    """The fused H+E pass needs fused Poynting, no Bloch boundaries and
-    rows of at most 512 cells (plan-level query, no GPU)."""
+    """The fused H+E pass needs fused Poynting and rows of at most 512 cells,
+    and with Bloch boundaries FP32 (plan-level query, no GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -180,7 +180,11 @@ def test_update_he_support_rules():
     assert supported(poynting=1, shape=(4, 4, 513)) == 0
     assert supported(poynting=0) == 0
     assert supported(poynting=2) == 0
-    assert supported(poynting=1, bloch_mask=1) == 0
+    # a Bloch pair runs the pass in FP32 on uniform grids
+    assert supported(poynting=1, bloch_mask=1) == 1
+    assert supported(poynting=1, bloch_mask=3, bloch_mode=1) == 1
+    assert supported(poynting=1, bloch_mask=1, dtype=1) == 0
+    assert supported(poynting=2, bloch_mask=1) == 0
     assert lib.fdtd_update_he_supported(None) == 0
 
 
