@@ -114,6 +114,9 @@ namespace {
 #ifndef FDTD_HE_TARGET
 #define FDTD_HE_TARGET 256           // fused H+E: workgroups resident at once (256 CUs)
 #endif
+#ifndef FDTD_HE2_HB
+#define FDTD_HE2_HB 1024             // two-step pass: threads per workgroup (whole rows plus three halo rows)
+#endif
 #ifndef FDTD_HEB_SPLIT
 #define FDTD_HEB_SPLIT 0             // Bloch-pair pass: the second system's loads after the first's H update
                                      // (measured on the 64x64x256 Bloch supercell: 45.9 us per step against
@@ -1701,8 +1704,8 @@ struct HE2Args {
 // EARLY: the loads of plane x+1 are issued after the second barrier, so
 // they are in flight while the second stage finishes plane x-1.
 template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0), bool EARLY = (FDTD_HE2_EARLY != 0)>
-__global__ __launch_bounds__(1024, FDTD_HE_WAVES) void he2_march(HE2Args<T> a) {
-    constexpr int HB = 1024;
+__global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 3) void he2_march(HE2Args<T> a) {
+    constexpr int HB = FDTD_HE2_HB;
     __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
     __shared__ T lp[LC ? 2 : 1][3][LC ? HB : 1];   // dt / eps
     __shared__ T le[2][3][HB];         // E^{n+1} by plane parity
@@ -2386,8 +2389,8 @@ int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
     }
     const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
     const bool nu = P.d.nonuniform != 0;
-    if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0>), grid, dim3(1024), 0, st, a);
-    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0>), grid, dim3(1024), 0, st, a);
+    if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0>), grid, dim3(FDTD_HE2_HB), 0, st, a);
     return check_launch("fdtd_update_he2");
 }
 
@@ -2518,9 +2521,10 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
         for (int j = k + 1; j < d.n_face; ++j)
             if (d.face[k].axis == d.face[j].axis && P->face[k].lo < P->face[j].hi && P->face[j].lo < P->face[k].hi)
                 apart = false;
-    if (FDTD_HE2 && P->lhe.mode == 2 && d.bloch_mask == 0 && d.dtype == FDTD_F32 && P->sz <= 256 &&
+    if (FDTD_HE2 && P->lhe.mode == 2 && d.bloch_mask == 0 && d.dtype == FDTD_F32 && 4 * P->sz <= FDTD_HE2_HB &&
         P->coef_elems <= kCoefTab && apart)
-        P->lhe2 = make_round_launch(P->plane, P->sx, 1024, ((1024 - 3 * P->sz) / P->sz) * P->sz, FDTD_HE_TARGET, 3);
+        P->lhe2 = make_round_launch(P->plane, P->sx, FDTD_HE2_HB, ((FDTD_HE2_HB - 3 * P->sz) / P->sz) * P->sz,
+                                    FDTD_HE_TARGET, 3);
     // Time per step of each pass: rounds of workgroups x planes a chunk
     // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
     // of the fused pass's (measured on MI355X: 1.15 with rows of 73 cells,
