@@ -1,0 +1,223 @@
+"""Checks of the library's host logic through the launch-recording build
+(test infrastructure; no GPU): plan creation over many shapes, the launch
+decomposition of every kernel (tiles of whole rows with their halo rows,
+x chunks covering the grid), and the step_n schedule: the half-steps in
+the order of the caller loop (fdtd.py:162-175), each pass reading the
+buffers the previous one wrote, the source values in order, and the moved
+bits reporting where E, psi_E and H end.
+
+``run(lib_path)`` raises AssertionError on the first failure; run as a
+script it exits 0 on success (used under AddressSanitizer)."""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import re
+import sys
+
+SLOTS = 16
+(E_IN, E_OUT, H_IN, H_OUT, PSIE_IN, PSIE_OUT, PSIH_IN, PSIH_OUT, F_IN, F_OUT, G_IN, G_OUT, S, SRC) = range(14)
+
+
+class Record(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_char * 128), ("grid", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("has_geom", ctypes.c_int32)] + \
+               [(n, ctypes.c_int32) for n in ("sx", "sy", "sz", "plane", "xchunk", "nchunk", "pblocks", "tile",
+                                              "src_x", "src_p", "src_comp")] + \
+               [("buf", ctypes.c_void_p * SLOTS)]
+
+
+def _load(path):
+    from opencl_fdtd_amd import _lib as L
+    lib = ctypes.CDLL(path)
+    P, F = ctypes.c_void_p, ctypes.POINTER(L.Fields)
+    lib.fdtd_plan_create.argtypes = [ctypes.POINTER(L.Desc), ctypes.POINTER(ctypes.c_void_p)]
+    lib.fdtd_plan_destroy.argtypes = [P]
+    lib.fdtd_plan_describe.argtypes = [P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+    lib.fdtd_plan_describe.restype = ctypes.c_int64
+    lib.fdtd_psi_elems.argtypes = [P]
+    lib.fdtd_psi_elems.restype = ctypes.c_int64
+    lib.fdtd_step_n_moved.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.POINTER(ctypes.c_int32)]
+    lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_update_he2_preferred.argtypes = [P]
+    lib.fdtd_update_he_supported.argtypes = [P]
+    lib.fdtd_record_count.restype = ctypes.c_int64
+    lib.fdtd_record_get.argtypes = [ctypes.c_int64, ctypes.POINTER(Record)]
+    lib.fdtd_last_error.restype = ctypes.c_char_p
+    return lib, L
+
+
+def _records(lib):
+    out = []
+    for i in range(lib.fdtd_record_count()):
+        r = Record()
+        assert lib.fdtd_record_get(i, ctypes.byref(r)) == 0
+        out.append(r)
+    lib.fdtd_record_clear()
+    return out
+
+
+def _name(r):
+    m = re.search(r"(\w+)<", r.kernel.decode())
+    return m.group(1) if m else r.kernel.decode()
+
+
+def _halo_rows(r):
+    """Halo rows a marching launch's workgroup holds besides its tile."""
+    name, text = _name(r), r.kernel.decode()
+    if name in ("he_march", "heb_march"):
+        return 1
+    if name == "he2_march":
+        return 3
+    if name == "h_march":
+        mode = int(re.search(r"h_march<[^,]+,[^,]+,[^,]+,\s*(\d)", text).group(1))
+        return 1 if mode == 2 else 0
+    return 0
+
+
+def check_decomposition(r):
+    """The tiles cover the plane once and the x chunks cover every plane."""
+    if not r.has_geom:
+        return
+    if _name(r) == "s_from_os":             # one thread per cell: plane blocks x planes
+        assert r.grid == -(-r.plane // r.block) * r.sx, (r.grid, r.plane, r.block, r.sx)
+        return
+    assert r.pblocks * r.nchunk == r.grid, (_name(r), r.pblocks, r.nchunk, r.grid)
+    assert r.tile > 0 and r.tile * r.pblocks >= r.plane > r.tile * (r.pblocks - 1), (_name(r), r.tile, r.plane)
+    assert r.xchunk > 0 and r.xchunk * r.nchunk >= r.sx > r.xchunk * (r.nchunk - 1), (_name(r), r.xchunk, r.sx)
+    assert r.plane == r.sy * r.sz
+    halo = _halo_rows(r)
+    if halo:
+        assert r.tile % r.sz == 0, (_name(r), r.tile, r.sz)        # whole rows
+        assert r.tile + halo * r.sz <= r.block, (_name(r), r.tile, halo, r.sz, r.block)
+
+
+HALF = {"e_march": "E", "h_march": "H", "he_march": "HE", "heb_march": "HE", "he2_march": "HEHE"}
+
+
+def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong):
+    """Half-steps E H E H ..., each launch reading what the previous wrote."""
+    loc = {"E": f.E, "H": f.H, "psi_e": f.psi_e, "psi_h": f.psi_h, "F": f.F, "G": f.G}
+    seq = []
+    k_src = 0
+    for r in recs:
+        name = _name(r)
+        if name in ("s_from_os", "add_point"):
+            continue
+        assert name in HALF, name
+        b = r.buf
+        if name == "e_march":
+            assert b[E_IN] == loc["E"] and b[H_IN] == loc["H"], ("e_march reads", n)
+            assert b[PSIE_IN] == loc["psi_e"]
+            if bloch:
+                assert b[F_IN] == loc["F"] and b[G_IN] == loc["G"]
+        elif name == "h_march":
+            assert b[E_IN] == loc["E"] and b[H_IN] == loc["H"] and b[PSIH_IN] == loc["psi_h"], ("h_march", n)
+            loc["H"], loc["psi_h"] = b[H_OUT], b[PSIH_OUT]
+            if pingpong:
+                assert b[H_OUT] != b[H_IN]
+            if bloch:
+                assert b[F_IN] == loc["F"] and b[G_IN] == loc["G"]
+                loc["G"] = b[G_OUT]
+        else:
+            assert b[E_IN] == loc["E"] and b[H_IN] == loc["H"], (name, n)
+            assert b[PSIE_IN] == loc["psi_e"] and b[PSIH_IN] == loc["psi_h"], (name, n)
+            assert b[E_OUT] != b[E_IN] and b[H_OUT] != b[H_IN]
+            loc["E"], loc["H"] = b[E_OUT], b[H_OUT]
+            loc["psi_e"], loc["psi_h"] = b[PSIE_OUT], b[PSIH_OUT]
+            if name == "heb_march":
+                assert b[F_IN] == loc["F"] and b[G_IN] == loc["G"]
+                assert b[F_OUT] != b[F_IN] and b[G_OUT] != b[G_IN]
+                loc["F"], loc["G"] = b[F_OUT], b[G_OUT]
+        # the source of step k goes with the k-th E half-step, from wave + k
+        if "E" in HALF[name] and wave:
+            first_e = sum(1 for t in seq if t == "E")
+            if name == "h_march":
+                pass
+            elif b[SRC]:
+                assert b[SRC] == wave + first_e * esz, (name, first_e)
+                k_src += HALF[name].count("E")
+        seq += list(HALF[name])
+    assert "".join(seq) == "EH" * n, ("".join(seq), n)
+    if wave and n:
+        assert k_src == n, (k_src, n)
+    if moved is not None:
+        assert bool(moved & 1) == (loc["E"] == f.E_out) and (loc["E"] in (f.E, f.E_out)), moved
+        assert bool(moved & 2) == (loc["psi_e"] == f.psi_e_out) or f.psi_e_out is None
+        assert bool(moved & 4) == (loc["H"] == f.H_out and f.H_out != f.H), moved
+    else:
+        assert loc["E"] == f.E and loc["psi_e"] == f.psi_e
+
+
+def _desc(L, shape, faces, dtype, poynting, bloch_mask=0, bloch_mode=0):
+    d = L.Desc()
+    d.shape[:] = list(shape)
+    d.dtype = dtype
+    d.n_face = len(faces)
+    for k, (a, p, t) in enumerate(faces):
+        d.face[k].axis, d.face[k].polarity, d.face[k].thickness = a, p, t
+    d.dt, d.inv_dx, d.poynting = 0.5, 1.0, poynting
+    d.bloch_mask, d.bloch_mode = bloch_mask, bloch_mode
+    d.bloch_re[:] = [1.0, 1.0, 1.0]
+    d.bloch_im[:] = [0.0, 0.0, 0.0]
+    return d
+
+
+SHAPES = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (7, 5, 73), (5, 37, 3), (33, 4, 130), (2, 9, 10), (4, 25, 40),
+          (120, 120, 73), (395, 345, 73), (64, 64, 256), (128, 128, 128), (24, 40, 512), (5, 6, 513),
+          (1000, 10, 3), (3, 700, 400), (264, 264, 136), (17, 1, 1), (1, 17, 1), (1, 1, 300)]
+
+
+def run(lib_path: str, quick: bool = False) -> int:
+    lib, L = _load(lib_path)
+    lib.fdtd_record_clear()
+    checked = 0
+    shapes = SHAPES[:8] if quick else SHAPES
+    for shape, dtype, poynting, bloch in itertools.product(shapes, (0, 1), (0, 1, 2), (0, 3)):
+        T = max(1, min(4, min(shape) // 3))
+        faces = [(a, p, T) for a in range(3) for p in (0, 1)]
+        d = _desc(L, shape, faces, dtype, poynting, bloch_mask=bloch)
+        plan = ctypes.c_void_p()
+        rc = lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan))
+        assert rc == 0, (shape, dtype, poynting, bloch, lib.fdtd_last_error())
+        try:
+            for which in range(6):
+                nch = lib.fdtd_plan_describe(plan, which, None, 0)
+                buf = ctypes.create_string_buffer(nch + 1)
+                assert lib.fdtd_plan_describe(plan, which, buf, nch + 1) == nch > 0
+            pingpong = poynting == 1
+            esz = 4 if dtype == 0 else 8
+            for n, with_out, homekeep in itertools.product((0, 1, 2, 3, 4, 5, 8, 11), (False, True), (False, True)):
+                f = L.Fields()
+                addrs = iter(range(0x100000, 0x10000000, 0x100000))
+                for name, _ in L.Fields._fields_:
+                    setattr(f, name, next(addrs))
+                if not pingpong:
+                    f.H_out, f.psi_h_out, f.G_out, f.psi_g_out = f.H, f.psi_h, f.G, f.psi_g
+                if not with_out:
+                    f.E_out = f.psi_e_out = f.F_out = None
+                if not bloch:
+                    f.F = f.G = f.G_out = f.psi_f = f.psi_g = f.psi_g_out = f.F_out = None
+                if poynting != 2:
+                    f.oS = None
+                wave = 0x70000000
+                moved = ctypes.c_int32(-1)
+                if homekeep:
+                    rc = lib.fdtd_step_n(plan, ctypes.byref(f), n, 1, wave, None)
+                else:
+                    rc = lib.fdtd_step_n_moved(plan, ctypes.byref(f), n, 1, wave, None, ctypes.byref(moved))
+                assert rc == 0, (shape, n, lib.fdtd_last_error())
+                recs = _records(lib)
+                for r in recs:
+                    check_decomposition(r)
+                check_schedule(recs, f, n, None if homekeep else moved.value, wave, esz, bool(bloch), pingpong)
+                checked += 1
+        finally:
+            lib.fdtd_plan_destroy(plan)
+    return checked
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, sys.argv[2]) if len(sys.argv) > 2 else None
+    n = run(sys.argv[1])
+    print("host schedule checks: %d schedules" % n, flush=True)
