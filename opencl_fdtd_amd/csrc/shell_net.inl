@@ -108,255 +108,6 @@ __device__ __forceinline__ size_t gidx(const ShellArgs<T>& a, int x, int y, int 
     return ((size_t)wrapi(x, a.sx) * a.sy + wrapi(y, a.sy)) * a.sz + wrapi(z, a.sz);
 }
 
-// global linear cell of the grown box -> (box, local coords)
-template <class T>
-__device__ __forceinline__ bool locate(const ShellArgs<T>& a, int cell, int* b, int* lx, int* ly, int* lz) {
-    for (int k = 0; k < a.nbox; ++k) {
-        const ShellBox& B = a.box[k];
-        if (cell >= B.hoff && cell < B.hoff + B.hcells) {
-            const int l = cell - B.hoff;
-            *b = k;
-            *lz = l % B.e[2];
-            *ly = (l / B.e[2]) % B.e[1];
-            *lx = l / (B.e[2] * B.e[1]);
-            return true;
-        }
-    }
-    return false;
-}
-
-// layer 1 over every grown-box cell
-template <class T>
-__global__ __launch_bounds__(256) void shell_l1(ShellArgs<T> a, int total) {
-    const int cell = blockIdx.x * 256 + threadIdx.x;
-    int b, lx, ly, lz;
-    if (cell >= total || !locate(a, cell, &b, &lx, &ly, &lz)) return;
-    const ShellBox& B = a.box[b];
-    const int x = hid_global(B, 0, lx), y = hid_global(B, 1, ly), z = hid_global(B, 2, lz);
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const uint16_t* w1 = a.w + (size_t)b * kW1;
-    float acc[kHid];
-#pragma unroll
-    for (int k = 0; k < kHid; ++k) acc[k] = 0.f;
-    for (int t = 0; t < kTaps; ++t) {
-        const size_t gi = gidx(a, x + t / 9 - 1, y + (t / 3) % 3 - 1, z + t % 3 - 1);
-        float xv[kIn];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xv[c] = bf16_round((float)a.E[c * N + gi]);
-            xv[3 + c] = bf16_round((float)a.H[c * N + gi]);
-        }
-#pragma unroll
-        for (int k = 0; k < kHid; ++k)
-#pragma unroll
-            for (int c = 0; c < kIn; ++c) acc[k] += bf16_to_f(w1[(k * kIn + c) * kTaps + t]) * xv[c];
-    }
-    float* h = a.hid + (size_t)cell * kHid;
-#pragma unroll
-    for (int k = 0; k < kHid; ++k) h[k] = bf16_round(acc[k]);
-}
-
-// ---- backward (training): adjoint of E_out = E + mask * y(E, H) ----------
-
-// hbar_k(j) = sum_{o,t} W2[o][k][t] ybar_o(j - t), ybar = Ebar on the box's
-// output cells (0 elsewhere); j over the grown box
-template <class T>
-__global__ __launch_bounds__(256) void shell_l2_t(ShellArgs<T> a, int total) {
-    const int cell = blockIdx.x * 256 + threadIdx.x;
-    int b, lx, ly, lz;
-    if (cell >= total || !locate(a, cell, &b, &lx, &ly, &lz)) return;
-    const ShellBox& B = a.box[b];
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const uint16_t* w2b = a.w + (size_t)a.nbox * kW1 + (size_t)b * kDepth * kW2;
-    const int r = b / 2;                       // the box's thin axis: depth = output-local index along it
-    float acc[kHid];
-#pragma unroll
-    for (int k = 0; k < kHid; ++k) acc[k] = 0.f;
-    const int gx = hid_global(B, 0, lx), gy = hid_global(B, 1, ly), gz = hid_global(B, 2, lz);
-    for (int t = 0; t < kTaps; ++t) {
-        // output cell i = j - t
-        const int ox = out_local(B, 0, gx - (t / 9 - 1));
-        const int oy = out_local(B, 1, gy - ((t / 3) % 3 - 1));
-        const int oz = out_local(B, 2, gz - (t % 3 - 1));
-        if (ox < 0 || oy < 0 || oz < 0) continue;
-        const uint16_t* w2 = w2b + (size_t)(r == 0 ? ox : (r == 1 ? oy : oz)) * kW2;
-        const size_t gi = gidx(a, B.o0[0] + ox, B.o0[1] + oy, B.o0[2] + oz);
-        float yb[kOut];
-#pragma unroll
-        for (int o = 0; o < kOut; ++o) yb[o] = (float)a.Ebar[o * N + gi];
-#pragma unroll
-        for (int k = 0; k < kHid; ++k)
-#pragma unroll
-            for (int o = 0; o < kOut; ++o) acc[k] += bf16_to_f(w2[(o * kHid + k) * kTaps + t]) * yb[o];
-    }
-    float* hb = a.hbar + (size_t)cell * kHid;
-#pragma unroll
-    for (int k = 0; k < kHid; ++k) hb[k] = acc[k];
-}
-
-// xbar_c(i) = sum_{k,t} W1[k][c][t] hbar_k(i - t) for one box (part_box);
-// i over the box grown by two cells; accumulated into Ebar_out / Hbar_out.
-template <class T>
-__global__ __launch_bounds__(256) void shell_l1_t(ShellArgs<T> a) {
-    const ShellBox& B = a.box[a.part_box];
-    const int g0 = in_extent(B, 0), g1 = in_extent(B, 1), g2 = in_extent(B, 2);
-    const int cell = blockIdx.x * 256 + threadIdx.x;
-    if (cell >= g0 * g1 * g2) return;
-    const int lz = cell % g2, ly = (cell / g2) % g1, lx = cell / (g2 * g1);   // grown-by-2 coords
-    const int gx = in_global(B, 0, lx), gy = in_global(B, 1, ly), gz = in_global(B, 2, lz);
-    const uint16_t* w1 = a.w + (size_t)a.part_box * kW1;
-    float acc[kIn];
-#pragma unroll
-    for (int c = 0; c < kIn; ++c) acc[c] = 0.f;
-    for (int t = 0; t < kTaps; ++t) {
-        // hidden cell j = i - t
-        const int hx = hid_local(B, 0, gx - (t / 9 - 1));
-        const int hy = hid_local(B, 1, gy - ((t / 3) % 3 - 1));
-        const int hz = hid_local(B, 2, gz - (t % 3 - 1));
-        if (hx < 0 || hy < 0 || hz < 0) continue;
-        const float* hb = a.hbar + ((size_t)B.hoff + ((size_t)hx * B.e[1] + hy) * B.e[2] + hz) * kHid;
-#pragma unroll
-        for (int k = 0; k < kHid; ++k) {
-            const float v = hb[k];
-#pragma unroll
-            for (int c = 0; c < kIn; ++c) acc[c] += bf16_to_f(w1[(k * kIn + c) * kTaps + t]) * v;
-        }
-    }
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const size_t gi = gidx(a, gx, gy, gz);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        a.Ebar_out[c * N + gi] += (T)acc[c];
-        a.Hbar_out[c * N + gi] += (T)acc[3 + c];
-    }
-}
-
-// Weight-gradient partials of one box (part_box), GEMM-shaped:
-//   W1bar[k][(c,t)] = sum_j hbar_k(j) x_c(j + t)   over the grown box
-//   W2bar[o][(k,t)] = sum_i ybar_o(i) h_k(i + t)   over the output box
-// Workgroup g takes a contiguous slice of the cells; chunks of the slice are
-// staged in LDS (adjoint rows and im2col rows) and each thread accumulates a
-// fixed subset of the weights.  partials[g][w], summed in a fixed order later.
-constexpr int kChunk1 = 32;
-constexpr int kChunk2 = 16;
-
-template <class T>
-__global__ __launch_bounds__(256) void shell_wgrad1(ShellArgs<T> a) {
-    __shared__ float sh[kChunk1][kHid];
-    __shared__ float sx[kChunk1][kIn * kTaps];
-    const ShellBox& B = a.box[a.part_box];
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const int per = (B.hcells + gridDim.x - 1) / gridDim.x;
-    const int c0 = blockIdx.x * per, c1 = min(c0 + per, B.hcells);
-    constexpr int NW = kHid * kIn * kTaps;   // 2592
-    constexpr int PER = (NW + 255) / 256;
-    float acc[PER];
-#pragma unroll
-    for (int m = 0; m < PER; ++m) acc[m] = 0.f;
-    for (int base = c0; base < c1; base += kChunk1) {
-        const int n = min(kChunk1, c1 - base);
-        for (int e = threadIdx.x; e < kChunk1 * kHid; e += 256) {
-            const int l = e / kHid, k = e % kHid;
-            sh[l][k] = l < n ? a.hbar[((size_t)B.hoff + base + l) * kHid + k] : 0.f;
-        }
-        for (int e = threadIdx.x; e < kChunk1 * kIn * kTaps; e += 256) {
-            const int l = e / (kIn * kTaps), ct = e % (kIn * kTaps), c = ct / kTaps, t = ct % kTaps;
-            float v = 0.f;
-            if (l < n) {
-                const int cl = base + l;
-                const int lz = cl % B.e[2], ly = (cl / B.e[2]) % B.e[1], lx = cl / (B.e[2] * B.e[1]);
-                const size_t gi = gidx(a, hid_global(B, 0, lx) + t / 9 - 1, hid_global(B, 1, ly) + (t / 3) % 3 - 1,
-                                       hid_global(B, 2, lz) + t % 3 - 1);
-                v = bf16_round((float)(c < 3 ? a.E[c * N + gi] : a.H[(c - 3) * N + gi]));
-            }
-            sx[l][ct] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int wi = threadIdx.x + 256 * m;
-            if (wi < NW) {
-                const int k = wi / (kIn * kTaps), ct = wi % (kIn * kTaps);
-                float s_ = 0.f;
-                for (int l = 0; l < kChunk1; ++l) s_ += sh[l][k] * sx[l][ct];
-                acc[m] += s_;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int wi = threadIdx.x + 256 * m;
-        if (wi < NW) a.wgrad_part[(size_t)blockIdx.x * NW + wi] = acc[m];
-    }
-}
-
-// (layer 2: all kDepth heads, each summing only the cells at its depth)
-template <class T>
-__global__ __launch_bounds__(256) void shell_wgrad2(ShellArgs<T> a) {
-    __shared__ float sy[kChunk2][kOut];
-    __shared__ int sd[kChunk2];
-    __shared__ float shc[kChunk2][kHid * kTaps];
-    const ShellBox& B = a.box[a.part_box];
-    const size_t N = (size_t)a.sx * a.sy * a.sz;
-    const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
-    const int nout = (B.o1[0] - B.o0[0]) * ny * nz;
-    const int per = (nout + gridDim.x - 1) / gridDim.x;
-    const int c0 = blockIdx.x * per, c1 = min(c0 + per, nout);
-    constexpr int NW = kDepth * kOut * kHid * kTaps;  // 4 x 1296
-    constexpr int PER = (NW + 255) / 256;
-    const int r = a.part_box / 2;
-    float acc[PER];
-#pragma unroll
-    for (int m = 0; m < PER; ++m) acc[m] = 0.f;
-    for (int base = c0; base < c1; base += kChunk2) {
-        const int n = min(kChunk2, c1 - base);
-        for (int e = threadIdx.x; e < kChunk2 * kOut; e += 256) {
-            const int l = e / kOut, o = e % kOut;
-            float v = 0.f;
-            int dep = -1;
-            if (l < n) {
-                const int cl = base + l;
-                const int lz = cl % nz, ly = (cl / nz) % ny, lx = cl / (nz * ny);
-                v = (float)a.Ebar[o * N + gidx(a, B.o0[0] + lx, B.o0[1] + ly, B.o0[2] + lz)];
-                dep = r == 0 ? lx : (r == 1 ? ly : lz);
-            }
-            sy[l][o] = v;
-            if (o == 0) sd[l] = dep;
-        }
-        for (int e = threadIdx.x; e < kChunk2 * kHid * kTaps; e += 256) {
-            const int l = e / (kHid * kTaps), kt = e % (kHid * kTaps), k = kt / kTaps, t = kt % kTaps;
-            float v = 0.f;
-            if (l < n) {
-                const int cl = base + l;
-                const int lz = cl % nz, ly = (cl / nz) % ny, lx = cl / (nz * ny);
-                const int hx = hid_local(B, 0, B.o0[0] + lx + t / 9 - 1);
-                const int hy = hid_local(B, 1, B.o0[1] + ly + (t / 3) % 3 - 1);
-                const int hz = hid_local(B, 2, B.o0[2] + lz + t % 3 - 1);
-                v = a.hid[((size_t)B.hoff + ((size_t)hx * B.e[1] + hy) * B.e[2] + hz) * kHid + k];
-            }
-            shc[l][kt] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int wi = threadIdx.x + 256 * m;
-            if (wi < NW) {
-                const int dep = wi / (kOut * kHid * kTaps), w = wi % (kOut * kHid * kTaps);
-                const int o = w / (kHid * kTaps), kt = w % (kHid * kTaps);
-                float s_ = 0.f;
-                for (int l = 0; l < kChunk2; ++l) s_ += sd[l] == dep ? sy[l][o] * shc[l][kt] : 0.f;
-                acc[m] += s_;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int wi = threadIdx.x + 256 * m;
-        if (wi < NW) a.wgrad_part[(size_t)blockIdx.x * NW + wi] = acc[m];
-    }
-}
 
 // wgrad[b][w] += sum_g partials[g][w], fixed order
 __global__ __launch_bounds__(256) void shell_wgrad_sum(const float* part, int groups, int nw, float* out) {
@@ -678,6 +429,287 @@ __global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, 
     a.Eout[2 * N + gi] += (T)y2;
 }
 
+
+// ---- training with MFMA (v_mfma_f32_16x16x4_f32) --------------------------
+//
+// The backward of one application of the net, and the forward hidden layer
+// it needs, as GEMMs on the FP32 matrix cores (FP32 operands, so the adjoint
+// keeps the precision of FP32 arithmetic; BF16 weights and BF16-rounded
+// inputs are exact in FP32).  Operand layout of the 16x16x4 instruction, 64
+// lanes: lane l holds A[l%16][l/16] and B[l/16][l%16]; D row 4(l/16)+i,
+// column l%16 in its register i.  Every wave works on its own 16 cells (or
+// its own column blocks), gathering its B (or A) values from global memory
+// (L1/L2-resident neighbourhoods); there is no LDS traffic and no atomics,
+// and every sum has a fixed order, so results are bit-reproducible.
+//
+//   hidden:  h[k](j)   = sum_(c,t)     W1[k][c][t] x_c(j+t)      M=k,    N=16 cells, K=(c,t) 162
+//   l2_t:    hbar_k(j) = sum_(d,o,t)   W2[d][o][k][t] ybar_o(i)  M=k,    N=16 cells, K=(d,o,t) 324,
+//                        i = j - t, the term kept where i is an output cell of depth d
+//   l1_t:    xbar_c(i) = sum_(k,t)     W1[k][c][t] hbar_k(i-t)   M=c(6), N=16 cells, K=(k,t) 432
+//   wgrad1:  W1bar[k][(c,t)]   = sum_j hbar_k(j) x_c(j+t)        M=k, N=(c,t) in blocks of 16, K=cells
+//   wgrad2:  W2bar[d][o][(k,t)] = sum_i [d(i)=d] ybar_o(i) h_k(i+t)  M=(d,o) 12, N=(k,t) blocks, K=cells
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// prefix of 16-cell groups per box (hidden cells)
+struct ShellGroups {
+    int gstart[7];
+};
+
+__device__ __forceinline__ void hid_coords(const ShellBox& B, int l, int* lx, int* ly, int* lz) {
+    *lz = l % B.e[2];
+    *ly = (l / B.e[2]) % B.e[1];
+    *lx = l / (B.e[2] * B.e[1]);
+}
+
+// layer 1 over every grown-box cell (the forward hidden values, BF16)
+template <class T>
+__global__ __launch_bounds__(256) void shell_l1_mfma(ShellArgs<T> a, ShellGroups gr) {
+    const int grp = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (grp >= gr.gstart[a.nbox]) return;                  // whole waves
+    int b = 0;
+    while (b < a.nbox - 1 && grp >= gr.gstart[b + 1]) ++b;
+    const ShellBox& B = a.box[b];
+    const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+    const int l = (grp - gr.gstart[b]) * 16 + n;
+    const bool ok = l < B.hcells;
+    int lx = 0, ly = 0, lz = 0;
+    if (ok) hid_coords(B, l, &lx, &ly, &lz);
+    const int x = hid_global(B, 0, lx), y = hid_global(B, 1, ly), z = hid_global(B, 2, lz);
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const uint16_t* w1 = a.w + (size_t)b * kW1;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < (kIn * kTaps + 3) / 4; ++s4) {
+        const int kk = 4 * s4 + q;                          // (c, t) = (kk / 27, kk % 27)
+        float av = 0.f, bv = 0.f;
+        if (kk < kIn * kTaps) {
+            const int c = kk / kTaps, t = kk % kTaps;
+            av = bf16_to_f(w1[(n * kIn + c) * kTaps + t]);
+            if (ok) {
+                const size_t gi = gidx(a, x + t / 9 - 1, y + (t / 3) % 3 - 1, z + t % 3 - 1);
+                bv = bf16_round((float)(c < 3 ? a.E[c * N + gi] : a.H[(c - 3) * N + gi]));
+            }
+        }
+        acc = mfma4(av, bv, acc);
+    }
+    if (ok) {
+        float* h = a.hid + ((size_t)B.hoff + l) * kHid + 4 * q;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = bf16_round(acc[i]);
+    }
+}
+
+// hbar over every grown-box cell from ybar = Ebar on the output cells
+template <class T>
+__global__ __launch_bounds__(256) void shell_l2_t_mfma(ShellArgs<T> a, ShellGroups gr) {
+    const int grp = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (grp >= gr.gstart[a.nbox]) return;
+    int b = 0;
+    while (b < a.nbox - 1 && grp >= gr.gstart[b + 1]) ++b;
+    const ShellBox& B = a.box[b];
+    const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+    const int l = (grp - gr.gstart[b]) * 16 + n;
+    const bool ok = l < B.hcells;
+    int lx = 0, ly = 0, lz = 0;
+    if (ok) hid_coords(B, l, &lx, &ly, &lz);
+    const int gx = hid_global(B, 0, lx), gy = hid_global(B, 1, ly), gz = hid_global(B, 2, lz);
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const uint16_t* w2b = a.w + (size_t)a.nbox * kW1 + (size_t)b * kDepth * kW2;
+    const int r = b / 2;                                   // the box's thin axis
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < kDepth * kOut * kTaps / 4; ++s4) {
+        const int kk = 4 * s4 + q;                          // (d, o, t)
+        const int t = kk % kTaps, o = (kk / kTaps) % kOut, d = kk / (kTaps * kOut);
+        const float av = bf16_to_f(w2b[(size_t)d * kW2 + (o * kHid + n) * kTaps + t]);
+        float bv = 0.f;
+        if (ok) {
+            const int ox = out_local(B, 0, gx - (t / 9 - 1));
+            const int oy = out_local(B, 1, gy - ((t / 3) % 3 - 1));
+            const int oz = out_local(B, 2, gz - (t % 3 - 1));
+            if (ox >= 0 && oy >= 0 && oz >= 0 && (r == 0 ? ox : (r == 1 ? oy : oz)) == d)
+                bv = (float)a.Ebar[o * N + gidx(a, B.o0[0] + ox, B.o0[1] + oy, B.o0[2] + oz)];
+        }
+        acc = mfma4(av, bv, acc);
+    }
+    if (ok) {
+        float* hb = a.hbar + ((size_t)B.hoff + l) * kHid + 4 * q;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hb[i] = acc[i];
+    }
+}
+
+// xbar over the cells of box part_box grown by two, accumulated into
+// Ebar_out / Hbar_out (boxes one launch after another: their regions overlap)
+template <class T>
+__global__ __launch_bounds__(256) void shell_l1_t_mfma(ShellArgs<T> a) {
+    const ShellBox& B = a.box[a.part_box];
+    const int g0 = in_extent(B, 0), g1 = in_extent(B, 1), g2 = in_extent(B, 2);
+    const int ncell = g0 * g1 * g2;
+    const int grp = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (grp * 16 >= ncell) return;
+    const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+    const int cell = grp * 16 + n;
+    const bool ok = cell < ncell;
+    const int lz = ok ? cell % g2 : 0, ly = ok ? (cell / g2) % g1 : 0, lx = ok ? cell / (g2 * g1) : 0;
+    const int gx = in_global(B, 0, lx), gy = in_global(B, 1, ly), gz = in_global(B, 2, lz);
+    const uint16_t* w1 = a.w + (size_t)a.part_box * kW1;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < kHid * kTaps / 4; ++s4) {
+        const int kk = 4 * s4 + q;                          // (k, t)
+        const int k = kk / kTaps, t = kk % kTaps;
+        const float av = n < kIn ? bf16_to_f(w1[(k * kIn + n) * kTaps + t]) : 0.f;
+        float bv = 0.f;
+        if (ok) {
+            const int hx = hid_local(B, 0, gx - (t / 9 - 1));
+            const int hy = hid_local(B, 1, gy - ((t / 3) % 3 - 1));
+            const int hz = hid_local(B, 2, gz - (t % 3 - 1));
+            if (hx >= 0 && hy >= 0 && hz >= 0)
+                bv = a.hbar[((size_t)B.hoff + ((size_t)hx * B.e[1] + hy) * B.e[2] + hz) * kHid + k];
+        }
+        acc = mfma4(av, bv, acc);
+    }
+    // rows c = 4q + i: E components 0..2 (q = 0), H components 0..2 (c = 3..5)
+    if (ok && q < 2) {
+        const size_t N = (size_t)a.sx * a.sy * a.sz;
+        const size_t gi = gidx(a, gx, gy, gz);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = 4 * q + i;
+            if (c < 3) a.Ebar_out[c * N + gi] += (T)acc[i];
+            else if (c < kIn) a.Hbar_out[(c - 3) * N + gi] += (T)acc[i];
+        }
+    }
+}
+
+// Weight-gradient partials of box part_box: workgroup g takes a contiguous
+// slice of the cells; its waves split the output column blocks and each
+// walks the whole slice, 4 cells per MFMA.  partials[g][w], in the weights'
+// own layout, summed in a fixed order by shell_wgrad_sum.
+constexpr int kWgGroups = 64;
+
+template <class T>
+__global__ __launch_bounds__(256) void shell_wgrad1_mfma(ShellArgs<T> a) {
+    constexpr int NCOL = kIn * kTaps;                       // 162 columns (c, t)
+    constexpr int NBLK = (NCOL + 15) / 16;                  // 11 blocks of 16
+    constexpr int PER_WAVE = (NBLK + 3) / 4;                // 3
+    const ShellBox& B = a.box[a.part_box];
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const int per = (B.hcells + gridDim.x - 1) / gridDim.x;
+    const int c0 = blockIdx.x * per, c1 = min(c0 + per, B.hcells);
+    const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+    const int wave = (int)(threadIdx.x >> 6);
+    f32x4 acc[PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) acc[j] = {0.f, 0.f, 0.f, 0.f};
+    // this lane's B columns (c, t) per block
+    int cc[PER_WAVE], tt[PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) {
+        const int col = 16 * (wave + 4 * j) + n;
+        cc[j] = col < NCOL ? col / kTaps : -1;
+        tt[j] = col < NCOL ? col % kTaps : 0;
+    }
+    for (int base = c0; base < c1; base += 4) {
+        const int l = base + q;                             // this lane's cell (K index)
+        float av = 0.f;
+        int x = 0, y = 0, z = 0;
+        const bool ok = l < c1;
+        if (ok) {
+            av = a.hbar[((size_t)B.hoff + l) * kHid + n];   // A[k = n][cell q]
+            int lx, ly, lz;
+            hid_coords(B, l, &lx, &ly, &lz);
+            x = hid_global(B, 0, lx); y = hid_global(B, 1, ly); z = hid_global(B, 2, lz);
+        }
+#pragma unroll
+        for (int j = 0; j < PER_WAVE; ++j) {
+            if (wave + 4 * j >= NBLK) break;                // uniform per wave
+            float bv = 0.f;
+            if (ok && cc[j] >= 0) {
+                const int t = tt[j], c = cc[j];
+                const size_t gi = gidx(a, x + t / 9 - 1, y + (t / 3) % 3 - 1, z + t % 3 - 1);
+                bv = bf16_round((float)(c < 3 ? a.E[c * N + gi] : a.H[(c - 3) * N + gi]));
+            }
+            acc[j] = mfma4(av, bv, acc[j]);
+        }
+    }
+    float* out = a.wgrad_part + (size_t)blockIdx.x * kW1;
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) {
+        const int col = 16 * (wave + 4 * j) + n;
+        if (wave + 4 * j < NBLK && col < NCOL) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[(4 * q + i) * NCOL + col] = acc[j][i];   // W1[k][(c, t)]
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void shell_wgrad2_mfma(ShellArgs<T> a) {
+    constexpr int NCOL = kHid * kTaps;                      // 432 columns (k, t)
+    constexpr int NBLK = NCOL / 16;                         // 27
+    constexpr int PER_WAVE = (NBLK + 3) / 4;                // 7
+    static_assert(kDepth * kOut <= 16, "rows (depth, output) fit one MFMA");
+    const ShellBox& B = a.box[a.part_box];
+    const size_t N = (size_t)a.sx * a.sy * a.sz;
+    const int ny = B.o1[1] - B.o0[1], nz = B.o1[2] - B.o0[2];
+    const int nout = (B.o1[0] - B.o0[0]) * ny * nz;
+    const int per = (nout + gridDim.x - 1) / gridDim.x;
+    const int c0 = blockIdx.x * per, c1 = min(c0 + per, nout);
+    const int r = a.part_box / 2;
+    const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+    const int wave = (int)(threadIdx.x >> 6);
+    const int ad = n / kOut, ao = n % kOut;                  // this lane's A row (depth, output)
+    f32x4 acc[PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) acc[j] = {0.f, 0.f, 0.f, 0.f};
+    int kk_[PER_WAVE], tt[PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) {
+        const int col = 16 * (wave + 4 * j) + n;
+        kk_[j] = col / kTaps;
+        tt[j] = col % kTaps;
+    }
+    for (int base = c0; base < c1; base += 4) {
+        const int l = base + q;
+        const bool ok = l < c1;
+        float av = 0.f;
+        int ox = 0, oy = 0, oz = 0;
+        if (ok) {
+            oz = l % nz; oy = (l / nz) % ny; ox = l / (nz * ny);
+            const int dep = r == 0 ? ox : (r == 1 ? oy : oz);
+            if (n < kDepth * kOut && dep == ad)
+                av = (float)a.Ebar[ao * N + gidx(a, B.o0[0] + ox, B.o0[1] + oy, B.o0[2] + oz)];
+        }
+#pragma unroll
+        for (int j = 0; j < PER_WAVE; ++j) {
+            if (wave + 4 * j >= NBLK) break;                // uniform per wave
+            float bv = 0.f;
+            if (ok) {
+                const int t = tt[j];
+                const int hx = hid_local(B, 0, B.o0[0] + ox + t / 9 - 1);
+                const int hy = hid_local(B, 1, B.o0[1] + oy + (t / 3) % 3 - 1);
+                const int hz = hid_local(B, 2, B.o0[2] + oz + t % 3 - 1);
+                bv = a.hid[((size_t)B.hoff + ((size_t)hx * B.e[1] + hy) * B.e[2] + hz) * kHid + kk_[j]];
+            }
+            acc[j] = mfma4(av, bv, acc[j]);
+        }
+    }
+    float* out = a.wgrad_part + (size_t)blockIdx.x * (kDepth * kW2);
+#pragma unroll
+    for (int j = 0; j < PER_WAVE; ++j) {
+        const int col = 16 * (wave + 4 * j) + n;
+        if (wave + 4 * j < NBLK) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 4 * q + i;                     // row (depth, output)
+                if (m < kDepth * kOut) out[(size_t)(m / kOut) * kW2 + (m % kOut) * NCOL + col] = acc[j][i];
+            }
+        }
+    }
+}
+
 struct ShellPlan {
     int nbox;
     ShellBox box[6];
@@ -924,7 +956,7 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     return check_launch("fdtd_shell_step_n");
 }
 
-int64_t fdtd_shell_partials_elems(void) { return (int64_t)64 * std::max(kW1, kDepth * kW2); }
+int64_t fdtd_shell_partials_elems(void) { return (int64_t)kWgGroups * std::max(kW1, kDepth * kW2); }
 
 int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, const void* E, const void* H,
                         float* hidden, float* hbar, const void* Ebar, void* Ebar_out, void* Hbar_out,
@@ -937,7 +969,7 @@ int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, 
     if (Ebar == Ebar_out) return fail(2, "Ebar_out must differ from Ebar");
     const ShellPlan S = make_shell(P, thickness);
     hipStream_t st = (hipStream_t)stream;
-    const int groups = 64;
+    const int groups = kWgGroups;
     auto run = [&](auto tag) -> int {
         using T = decltype(tag);
         ShellArgs<T> a;
@@ -945,23 +977,28 @@ int fdtd_shell_backward(const void* plan, int32_t thickness, const uint16_t* w, 
         a.w = w; a.E = (const T*)E; a.H = (const T*)H; a.hid = hidden; a.hbar = hbar;
         a.Ebar = (const T*)Ebar; a.Ebar_out = (T*)Ebar_out; a.Hbar_out = (T*)Hbar_out;
         a.wgrad_part = partials;
-        const int g1 = (S.hcells + 255) / 256;
+        // 16-cell groups of hidden cells, box by box (one group per wave)
+        ShellGroups gr;
+        gr.gstart[0] = 0;
+        for (int b = 0; b < 6; ++b)
+            gr.gstart[b + 1] = gr.gstart[b] + (b < S.nbox ? (S.box[b].hcells + 15) / 16 : 0);
+        const int g1 = (gr.gstart[S.nbox] + 3) / 4;
         // forward hidden (the weight gradient of layer 2 needs it)
-        hipLaunchKernelGGL(shell_l1<T>, dim3(g1), dim3(256), 0, st, a, S.hcells);
-        hipLaunchKernelGGL(shell_l2_t<T>, dim3(g1), dim3(256), 0, st, a, S.hcells);
+        hipLaunchKernelGGL(shell_l1_mfma<T>, dim3(g1), dim3(256), 0, st, a, gr);
+        hipLaunchKernelGGL(shell_l2_t_mfma<T>, dim3(g1), dim3(256), 0, st, a, gr);
         for (int b = 0; b < S.nbox; ++b) {
             a.part_box = b;
             const ShellBox& B = S.box[b];
             const int n2 = (B.full[0] ? B.s[0] : B.o1[0] - B.o0[0] + 4) *
                            (B.full[1] ? B.s[1] : B.o1[1] - B.o0[1] + 4) *
                            (B.full[2] ? B.s[2] : B.o1[2] - B.o0[2] + 4);
-            hipLaunchKernelGGL(shell_l1_t<T>, dim3((n2 + 255) / 256), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(shell_l1_t_mfma<T>, dim3((n2 + 63) / 64), dim3(256), 0, st, a);
             // W1 gradient of box b
-            hipLaunchKernelGGL(shell_wgrad1<T>, dim3(groups), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(shell_wgrad1_mfma<T>, dim3(groups), dim3(256), 0, st, a);
             hipLaunchKernelGGL(shell_wgrad_sum, dim3((kW1 + 255) / 256), dim3(256), 0, st, (const float*)partials,
                                groups, kW1, wgrad + (size_t)b * kW1);
             // W2 gradient of box b
-            hipLaunchKernelGGL(shell_wgrad2<T>, dim3(groups), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(shell_wgrad2_mfma<T>, dim3(groups), dim3(256), 0, st, a);
             hipLaunchKernelGGL(shell_wgrad_sum, dim3((kDepth * kW2 + 255) / 256), dim3(256), 0, st,
                                (const float*)partials, groups, kDepth * kW2,
                                wgrad + (size_t)S.nbox * kW1 + (size_t)b * kDepth * kW2);

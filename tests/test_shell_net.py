@@ -156,8 +156,14 @@ def test_hip_forward_matches_oracle(shape, T):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4)])
+@pytest.mark.parametrize("shape, T", [((12, 11, 13), 2), ((16, 14, 12), 4), ((33, 29, 47), 3),
+                                      ((20, 37, 25), 4), ((8, 9, 30), 2)])
 def test_hip_backward_matches_oracle(shape, T):
+    """The MFMA backward (FP32 matrix cores: hidden layer, both transposed
+    layers, both weight gradients) against the FP64 oracle; the larger
+    shapes give every box many 16-cell groups and every weight-gradient
+    workgroup a slice of several hundred cells."""
+    print("case shell backward", shape, T, flush=True)
     rng = numpy.random.default_rng(405)
     w = _rand_weights(rng)
     E, H = _fields(rng, shape)
