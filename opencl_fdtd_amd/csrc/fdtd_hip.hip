@@ -1703,7 +1703,8 @@ struct HE2Args {
 // plane parity) instead of being carried in registers (fewer spills).
 // EARLY: the loads of plane x+1 are issued after the second barrier, so
 // they are in flight while the second stage finishes plane x-1.
-template <class T, bool NU, bool S2, bool LC = (FDTD_HE2_LDSCARRY != 0), bool EARLY = (FDTD_HE2_EARLY != 0)>
+template <class T, bool NU, bool S2, bool E2 = true, bool LC = (FDTD_HE2_LDSCARRY != 0),
+          bool EARLY = (FDTD_HE2_EARLY != 0)>
 __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 3) void he2_march(HE2Args<T> a) {
     constexpr int HB = FDTD_HE2_HB;
     __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
@@ -1860,9 +1861,18 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
                 h1y = la[b ^ 1][1][t];
                 h1z = la[b ^ 1][2][t];
             }
+            // without E2 the pass ends after H^{n+3/2}: E^{n+1} and its psi
+            // pairs are the pass's E output (owned cells of the chunk's planes)
+            const bool wr1 = !E2 && own && it >= 2 && it < n_it - 1;
             e_value<T, true, ESysP<T>, const CpmlCols<T>*, true>(
-                g, eA, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1, src ? a.src_amp[0] : (T)0,
-                (const CpmlCols<T>*)&ka, nullptr, &qe, tab, false, &nke, n1x, n1y, n1z);
+                g, E2 ? eA : eB, c, xa, v, h1y, h1z, e0, e1, e2, jxa, jy, jz, src ? src_comp : -1,
+                src ? a.src_amp[0] : (T)0, (const CpmlCols<T>*)&ka, nullptr, &qe, tab, wr1, &nke, n1x, n1y, n1z);
+            if (wr1) {
+                const uint32_t o1 = (uint32_t)xa * pbytes + c.op;
+                stc<FDTD_NT_E>(eB.E, 0, o1, n1x);
+                stc<FDTD_NT_E>(eB.E, 1, o1, n1y);
+                stc<FDTD_NT_E>(eB.E, 2, o1, n1z);
+            }
         }
         le[b][0][t] = n1x;
         le[b][1][t] = n1y;
@@ -1919,6 +1929,9 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
             stc<FDTD_NT_H>(a.S, 0, oi, (oBxy + oBxz + lb[0][tym] + lb[2][tzm]) * sf);
             stc<FDTD_NT_H>(a.S, 1, oi, (oByz + oByx + lb[3][tzm] + oByx_m) * sf);
             stc<FDTD_NT_H>(a.S, 2, oi, (oBzx + oBzy + oBzx_m + lb[1][tym]) * sf);
+        }
+        if (E2 && own && wb) {
+            const uint32_t oi = (uint32_t)xb * pbytes + c.op;
             EIn<T> v;
             v.hx = g2x; v.hy = g2y; v.hz = g2z;
             v.hx_ym = lb[4][tym];
@@ -2369,7 +2382,10 @@ int check_he_fields(const Plan& P, const fdtd_fields* f) {
 
 bool he2_supported(const Plan& P) { return P.lhe2.mode == 2; }
 
-int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index) {
+// last_e false: the pass stops after the second H half-step (H, E with the
+// source, H with S), which ends a step_n schedule with an odd pair count
+int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index,
+               bool last_e = true) {
     using T = float;
     HE2Args<T> a;
     memset(&a, 0, sizeof(a));
@@ -2397,8 +2413,10 @@ int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
     }
     const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
     const bool nu = P.d.nonuniform != 0;
-    if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    else if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    else if (last_e) hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
     return check_launch("fdtd_update_he2");
 }
 
@@ -2696,7 +2714,10 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
         k = 1;
         const int m = n - 1;
         const int n2 = fuse2 ? (home ? 2 * (m / 4) : m / 2) : 0;
-        const int r = m - 2 * n2;
+        // shortest schedule with an odd pair count: the last pair and the
+        // final H half-step run as one pass (he2_march without its last E)
+        const bool heh = fuse2 && !home && (m - 2 * n2) == 1;
+        const int r = m - 2 * n2 - (heh ? 1 : 0);
         if (home && r % 2 == 1) {
             if (int rc = half_h()) return rc;
             if (int rc = half_e(1)) return rc;
@@ -2715,14 +2736,16 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
                 std::swap(cur.psi_g, cur.psi_g_out);
             }
         }
-        for (int j = 0; j < n2; ++j, k += 2) {
-            if (int rc = launch_he2(P, &cur, st, (const float*)amp_of(k), src_index)) return rc;
+        for (int j = 0; j < n2 + (heh ? 1 : 0); ++j, k += 2) {
+            const bool last_e = !(heh && j == n2);
+            if (int rc = launch_he2(P, &cur, st, (const float*)amp_of(k), src_index, last_e)) return rc;
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
             std::swap(cur.E, cur.E_out);
             std::swap(cur.psi_e, cur.psi_e_out);
         }
-        if (int rc = half_h()) return rc;
+        if (!heh)
+            if (int rc = half_h()) return rc;
     } else {
         for (k = 0; k < n; ++k) {
             if (int rc = half_e(k)) return rc;

@@ -95,6 +95,17 @@ def check_decomposition(r):
 HALF = {"e_march": "E", "h_march": "H", "he_march": "HE", "heb_march": "HE", "he2_march": "HEHE"}
 
 
+def _halves(r):
+    """The half-steps a launch performs; a two-step pass launched without
+    its last E (template argument E2 = false) does H, E, H."""
+    name = _name(r)
+    if name == "he2_march":
+        args = [x.strip() for x in re.search(r"<(.*)>", r.kernel.decode()).group(1).split(",")]
+        if len(args) >= 4 and args[3] == "false":
+            return "HEH"
+    return HALF[name]
+
+
 def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong):
     """Half-steps E H E H ..., each launch reading what the previous wrote."""
     loc = {"E": f.E, "H": f.H, "psi_e": f.psi_e, "psi_h": f.psi_h, "F": f.F, "G": f.G}
@@ -130,14 +141,14 @@ def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong):
                 assert b[F_OUT] != b[F_IN] and b[G_OUT] != b[G_IN]
                 loc["F"], loc["G"] = b[F_OUT], b[G_OUT]
         # the source of step k goes with the k-th E half-step, from wave + k
-        if "E" in HALF[name] and wave:
+        if "E" in _halves(r) and wave:
             first_e = sum(1 for t in seq if t == "E")
             if name == "h_march":
                 pass
             elif b[SRC]:
                 assert b[SRC] == wave + first_e * esz, (name, first_e)
-                k_src += HALF[name].count("E")
-        seq += list(HALF[name])
+                k_src += _halves(r).count("E")
+        seq += list(_halves(r))
     assert "".join(seq) == "EH" * n, ("".join(seq), n)
     if wave and n:
         assert k_src == n, (k_src, n)
