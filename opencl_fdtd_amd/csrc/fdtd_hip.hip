@@ -656,33 +656,38 @@ __device__ __forceinline__ void cpml_col(const Geom<T>& g, int x, const Cell& c,
 }
 
 // The fused pass's CPML: x from the uniform per-plane hit, y and z from the
-// per-thread columns; coefficients from the table tab.
-template <class T>
+// per-thread columns; coefficients from the table tab.  AX: the axes that
+// can carry CPML faces (bit r for axis r); the code and registers of the
+// other axes are left out at compile time.
+template <class T, int AX = 7>
 struct CpmlCols {
     PsiHit hx;
     PsiCol y, z;
 };
 
-template <class T>
+template <class T, int AX>
 __device__ __forceinline__ void psi_fetch_cols(const Geom<T>& g, const T* psi_in, int x, const Cell& c,
-                                               const CpmlCols<T>& k, PsiPair<T>& q) {
-    psi_fetch_axis<0>(g, psi_in, x, c, k.hx, q);
-    psi_fetch_col<1>(k.y, psi_in, x, q);
-    psi_fetch_col<2>(k.z, psi_in, x, q);
+                                               const CpmlCols<T, AX>& k, PsiPair<T>& q) {
+    if constexpr ((AX & 1) != 0) psi_fetch_axis<0>(g, psi_in, x, c, k.hx, q);
+    if constexpr ((AX & 2) != 0) psi_fetch_col<1>(k.y, psi_in, x, q);
+    if constexpr ((AX & 4) != 0) psi_fetch_col<2>(k.z, psi_in, x, q);
 }
 
-template <class T, bool ISE>
-__device__ __forceinline__ void cpml_cols(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>& k,
+template <class T, bool ISE, int AX>
+__device__ __forceinline__ void cpml_cols(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T, AX>& k,
                                           const T* tab, const PsiPair<T>& q, const T* psi_in, T* psi_out,
                                           bool write, T& dxy, T& dxz, T& dyx, T& dyz, T& dzx, T& dzy, T& px,
                                           T& py, T& pz, PsiPair<T>* keep = nullptr) {
     const FaceCoef<T> none{};
-    cpml_axis<0, T, ISE>(g, x, c, k.hx, none, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
-                         py, pz, keep);
-    cpml_col<1, T, ISE>(g, x, c, k.y, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
-                        keep);
-    cpml_col<2, T, ISE>(g, x, c, k.z, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
-                        keep);
+    if constexpr ((AX & 1) != 0)
+        cpml_axis<0, T, ISE>(g, x, c, k.hx, none, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px,
+                             py, pz, keep);
+    if constexpr ((AX & 2) != 0)
+        cpml_col<1, T, ISE>(g, x, c, k.y, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                            keep);
+    if constexpr ((AX & 4) != 0)
+        cpml_col<2, T, ISE>(g, x, c, k.z, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                            keep);
 }
 
 // CPML with prefetched psi: per-axis hits (cpml_apply) or columns (cpml_cols)
@@ -694,13 +699,13 @@ __device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c,
     cpml_apply<T, ISE>(g, x, c, hit, kc, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
                        keep);
 }
-template <class T, bool ISE>
-__device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T>* cols,
+template <class T, bool ISE, int AX>
+__device__ __forceinline__ void cpml_pre(const Geom<T>& g, int x, const Cell& c, const CpmlCols<T, AX>* cols,
                                          const FaceCoef<T>*, const T* tab, const PsiPair<T>& q,
                                          const T* psi_in, T* psi_out, bool write, T& dxy, T& dxz, T& dyx,
                                          T& dyz, T& dzx, T& dzy, T& px, T& py, T& pz, PsiPair<T>* keep = nullptr) {
-    cpml_cols<T, ISE>(g, x, c, *cols, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
-                      keep);
+    cpml_cols<T, ISE, AX>(g, x, c, *cols, tab, q, psi_in, psi_out, write, dxy, dxz, dyx, dyz, dzx, dzy, px, py, pz,
+                          keep);
 }
 
 // ---------------------------------------------------------------------------
@@ -1497,7 +1502,7 @@ struct HEBArgs {
     int coef_elems;
 };
 
-template <class T, bool SEAM>
+template <class T, bool SEAM, int AX>
 __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
     constexpr int HB = 1024;
     __shared__ T lds[2][10][HB];
@@ -1523,9 +1528,10 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
     for (int i = t; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
     lds_barrier();
     const T* tab = ctab;
-    CpmlCols<T> cols;
-    cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
-    cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
+    CpmlCols<T, AX> cols{};
+    cols.hx.code = cols.y.h.code = cols.z.h.code = -1;
+    if constexpr ((AX & 2) != 0) cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
+    if constexpr ((AX & 4) != 0) cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
     const int ty = t - g.sz;
     const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
 
@@ -1545,7 +1551,7 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
         const bool dry = it == 0;   // uniform
         const uint32_t oi = (uint32_t)x * pbytes + c.op;
         // ---- loads of the plane: both H-step inputs, the four psi pairs, eps
-        cols.hx = slab_hit(g, 0, x, g.sx);
+        if constexpr ((AX & 1) != 0) cols.hx = slab_hit(g, 0, x, g.sx);
         HIn<T> in[2];
         PsiPair<T> qh, qe[2];
         T e0 = (T)1, e1 = (T)1, e2 = (T)1;
@@ -1579,7 +1585,7 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
             }
             if (valid) {
                 const HCell<T> h = h_update<T, SEAM, true>(g, a.hs[k], c, x, pbytes, q[k][0], q[k][1], q[k][2], in[k],
-                                                           iu, iu, iu, own && !dry, (const CpmlCols<T>*)&cols,
+                                                           iu, iu, iu, own && !dry, (const CpmlCols<T, AX>*)&cols,
                                                            nullptr, &qh, tab);
                 h1[k][0] = h.hx1; h1[k][1] = h.hy1; h1[k][2] = h.hz1;
                 if (k == 0) {
@@ -1638,7 +1644,7 @@ __global__ __launch_bounds__(1024, FDTD_HE_WAVES) void heb_march(HEBArgs<T> a) {
                 }
                 v.ex = q[k][0]; v.ey = q[k][1]; v.ez = q[k][2];
                 e_update<T, true>(g, a.es[k], c, x, oi, v, hy_x, hz_x, e0, e1, e2, iu, iu, iu,
-                                  k == 0 && src ? src_comp : -1, amp, (const CpmlCols<T>*)&cols, nullptr, &qe[k], tab);
+                                  k == 0 && src ? src_comp : -1, amp, (const CpmlCols<T, AX>*)&cols, nullptr, &qe[k], tab);
             }
         }
         oSyx_m = oSyx;
@@ -2304,8 +2310,20 @@ int launch_heb(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
         a.src_p = (int)(i - (int64_t)a.src_x * P.plane);
     }
     const dim3 grid(P.lhe.pblocks * P.lhe.nchunk);
-    if (P.d.bloch_mode == FDTD_BLOCH_PHASE) hipLaunchKernelGGL((heb_march<T, true>), grid, dim3(1024), 0, st, a);
-    else hipLaunchKernelGGL((heb_march<T, false>), grid, dim3(1024), 0, st, a);
+    // the axes that carry CPML faces: none, z only (the usual setup with
+    // Bloch boundaries on x and y), or any
+    int axes = 0;
+    for (int k = 0; k < P.d.n_face; ++k) axes |= 1 << P.d.face[k].axis;
+    const bool seam = P.d.bloch_mode == FDTD_BLOCH_PHASE;
+#define FDTD_LAUNCH_HEB(AX_)                                                                                  \
+    do {                                                                                                      \
+        if (seam) hipLaunchKernelGGL((heb_march<T, true, AX_>), grid, dim3(1024), 0, st, a);                    \
+        else hipLaunchKernelGGL((heb_march<T, false, AX_>), grid, dim3(1024), 0, st, a);                        \
+    } while (0)
+    if (axes == 0) FDTD_LAUNCH_HEB(0);
+    else if (axes == 4) FDTD_LAUNCH_HEB(4);
+    else FDTD_LAUNCH_HEB(7);
+#undef FDTD_LAUNCH_HEB
     return check_launch("fdtd_update_he");
 }
 

@@ -371,31 +371,35 @@ def test_fusion_is_off_where_unsupported():
         Simulation(eps, [], float_type=numpy.float64).update_HE2([])
 
 
-def _bloch_case(shape, T, seed, axes="xyz"):
+def _bloch_case(shape, T, seed, axes="xyz", face_axes="z"):
     rng = numpy.random.default_rng(seed)
     eps = [rng.uniform(1, 6, shape) for _ in range(3)]
     for e in eps:
         e[shape[0] // 2, :, :shape[2] // 3] = numpy.inf
     bl = [{"axis": a, "real": math.cos(ph), "imag": math.sin(ph)} for a, ph in zip(axes, (0.7, -1.9, 2.4))]
-    pmls = [{"axis": "z", "polarity": p, "thickness": T} for p in "np"] if T else []
+    pmls = [{"axis": fa, "polarity": p, "thickness": T} for fa in face_axes for p in "np"] if T else []
     init = {k: random_fields(rng, shape) for k in "EHFG"}
     return eps, pmls, bl, init
 
 
 @pytest.mark.parametrize("mode", ["phase", "decoupled"])
-@pytest.mark.parametrize("shape, T, axes", [
-    ((8, 10, 12), 3, "xy"),      # one tile per plane
-    ((6, 24, 40), 4, "xyz"),     # seams on every axis, several tiles per plane
-    ((5, 9, 256), 16, "xy"),     # rows of 256 (the Bloch supercell's): three owned rows per tile
-    ((4, 3, 512), 2, "xz"),      # one owned row per tile, z seam inside the row
+@pytest.mark.parametrize("shape, T, axes, face_axes", [
+    ((8, 10, 12), 3, "xy", "z"),      # one tile per plane
+    ((6, 24, 40), 4, "xyz", "z"),     # seams on every axis, several tiles per plane
+    ((5, 9, 256), 16, "xy", "z"),     # rows of 256 (the Bloch supercell's): three owned rows per tile
+    ((4, 3, 512), 2, "xz", "z"),      # one owned row per tile, z seam inside the row
+    ((9, 12, 14), 2, "xy", "xyz"),    # CPML on every axis (the general variant of the pass)
+    ((7, 16, 20), 3, "z", "x"),       # CPML on x only, Bloch on z
+    ((6, 8, 10), 0, "xyz", ""),       # no CPML at all
 ])
-def test_bloch_pair_pass_equals_paired_kernels(mode, shape, T, axes):
+def test_bloch_pair_pass_equals_paired_kernels(mode, shape, T, axes, face_axes):
     """The fused Bloch-pair pass (update_HE on a Bloch plan) against the
     paired h_march + e_march, bit for bit: E, H, F, G, S and the psi blocks,
-    with a source."""
+    with a source; with CPML on z only, on other axes, and none (the pass is
+    specialised for the axes that carry faces)."""
     from opencl_fdtd_amd import Simulation
-    print("case bloch pair pass", mode, shape, T, axes, flush=True)
-    eps, pmls, bl, init = _bloch_case(shape, T, seed=sum(shape), axes=axes)
+    print("case bloch pair pass", mode, shape, T, axes, face_axes, flush=True)
+    eps, pmls, bl, init = _bloch_case(shape, T, seed=sum(shape), axes=axes, face_axes=face_axes)
     mk = lambda fuse: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], bloch_boundaries=bl,
                                  initial_fields=init, bloch_mode=mode, fuse_steps=fuse, defer_h=False)
     a, b = mk(True), mk(False)
