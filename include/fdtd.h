@@ -179,9 +179,10 @@ int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t
  * not NULL, E_out[src_index] += src_amp[0].  E is left unchanged (E_out holds
  * the new E), psi_e is updated in place, psi_h -> psi_h_out.  Bit-identical to
  * fdtd_update_h, then fdtd_update_e on E_out := E, then fdtd_add_point.
- * Needs fdtd_update_he_supported(plan) (fused Poynting, rows of at most 512
- * cells) and distinct E/E_out, H/H_out, psi_h/psi_h_out.  With Bloch
- * boundaries (FP32, uniform dx) the pass advances the pair: F -> F_out and
+ * Needs fdtd_update_he_supported(plan) (fused Poynting; long rows are cut
+ * into z segments) and distinct E/E_out, H/H_out, psi_h/psi_h_out.  With
+ * Bloch boundaries (FP32, uniform dx, rows of at most 512 cells, each tile
+ * whole rows) the pass advances the pair: F -> F_out and
  * G -> G_out, psi_g -> psi_g_out, psi_f in place, all distinct; it is then
  * bit-identical to the paired fdtd_update_h and fdtd_update_e (the four
  * update_H/G/E/F launches of simulation.py:252-264).
@@ -207,14 +208,14 @@ int fdtd_update_he_supported(const void* plan);
 int fdtd_update_he2(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,
                     void* stream);
 /* 1 when fdtd_update_he2 can run with this plan (FP32, fused Poynting, no
-   Bloch boundaries, rows of at most 256 cells, CPML slabs of one axis
-   apart), else 0. */
+   Bloch boundaries, CPML slabs of one axis apart), else 0. */
 int fdtd_update_he2_supported(const void* plan);
 /* 1 when fdtd_step_n runs two steps per pass with this plan (given E_out and
    psi_e_out): fdtd_update_he2 is supported and its time model (rounds of
    workgroups x planes per chunk) beats the one-step fused pass's, else 0.
-   Long rows leave the two-step tiles few owned rows; the one-step pass is
-   then kept. */
+   Where the two-step tiles (three halo rows, and three halo columns when
+   rows are cut into z segments) own few of their cells, the one-step pass
+   is kept. */
 int fdtd_update_he2_preferred(const void* plan);
 
 /* ------------------------------------------------------------------------

@@ -122,6 +122,9 @@ namespace {
                                      // (measured on the 64x64x256 Bloch supercell: 45.9 us per step against
                                      // 36.6 us with every load of the plane up front)
 #endif
+#ifndef FDTD_SEG_MIN
+#define FDTD_SEG_MIN 64              // he_march / he2_march: narrowest z segment of owned cells considered
+#endif
 #ifndef FDTD_ROW_MAX
 #define FDTD_ROW_MAX 512             // fused-S H kernel and fused H+E pass: longest rows (a 1024-thread tile
                                      // holds at least one owned row and its halo row)
@@ -163,6 +166,10 @@ template <class T>
 struct Geom {
     int sx, sy, sz, plane;
     int xchunk, nchunk, pblocks, tile;   // launch decomposition (tile = plane cells per workgroup)
+    // he_march / he2_march tiles: tyo owned rows by tzo owned z cells, tzb
+    // tiles per row of the grid, tw threads per workgroup row (tw = tzo = sz:
+    // whole rows, z periodic inside a row; else z segments with halo columns)
+    int tyo, tzo, tzb, tw;
     int seam_mask;       // Bloch axes with the phase applied (0 when decoupled)
     T dt;
     T inv_u;
@@ -462,6 +469,45 @@ __device__ __forceinline__ Cell make_cell(const Geom<T>& g, int p) {
     c.op_y = (uint32_t)pyp * B;
     c.op_z = (uint32_t)pzp * B;
     return c;
+}
+
+// A thread's place in a he_march / he2_march tile (Geom's tyo, tzo, tzb,
+// tw).  Workgroup tile pb holds the tile's owned rows with YL halo rows
+// below and YH above, tw threads a row, thread t in row r = t / tw, column
+// cz = t % tw.  Whole rows (tzb = 1): column cz is z, z periodic inside the
+// row.  Z segments (tzb > 1): the owned cells z0 .. z0+tzo-1 with ZL halo
+// columns before and ZH after.  nr, ncz: the rows and columns that hold
+// cells (the last tile of a column or row is cut short); in: the thread
+// holds one; p: its plane cell, y and z wrapped periodically (halo cells
+// may repeat cells of the tile on grids thinner than the tile).
+struct TileCell {
+    int p, r, cz, nr, ncz;
+    bool in, seg;
+};
+
+template <int YL, int YH, int ZL, int ZH, class T>
+__device__ __forceinline__ TileCell tile_cell(const Geom<T>& g, int pb, int t) {
+    TileCell k;
+    const int iy = pb / g.tzb;
+    const int iz = pb - iy * g.tzb;
+    const int y0 = iy * g.tyo;
+    k.seg = g.tzb > 1;
+    k.r = t / g.tw;
+    k.cz = t - k.r * g.tw;
+    k.nr = min(g.tyo, g.sy - y0) + YL + YH;
+    int z = k.cz;
+    k.ncz = g.sz;
+    if (k.seg) {
+        const int z0 = iz * g.tzo;
+        k.ncz = min(g.tzo, g.sz - z0) + ZL + ZH;
+        z = (z0 - ZL + k.cz) % g.sz;
+        if (z < 0) z += g.sz;
+    }
+    int y = (y0 - YL + k.r) % g.sy;
+    if (y < 0) y += g.sy;
+    k.in = k.r < k.nr && k.cz < k.ncz;
+    k.p = y * g.sz + z;
+    return k;
 }
 
 // ---------------------------------------------------------------------------
@@ -1290,11 +1336,14 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
     const int t = threadIdx.x;
-    const int halo = g.sz;
-    const int praw = pb * g.tile - halo + t;
-    const bool valid = (t < g.tile + halo) && praw < g.plane;
-    const int p = praw < 0 ? praw + g.plane : praw;
-    const bool own = valid && t >= halo;
+    // one halo row below the owned rows (and, with z segments, one halo
+    // column before the owned cells): the -y / -z neighbours of the E update
+    const TileCell tc = tile_cell<1, 0, 1, 0>(g, pb, t);
+    const bool valid = tc.in;
+    const int p = tc.p;
+    const bool own = valid && tc.r >= 1 && (!tc.seg || tc.cz >= 1);
+    const int ty = t - g.tw;
+    const int tz = (tc.cz > 0) ? t - 1 : t + g.tw - 1;
     const Cell c = make_cell<T>(g, valid ? p : 0);
     const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
     const int x0 = ck * g.xchunk;
@@ -1429,8 +1478,6 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
         lds[b][6][t] = hz1;
         lds_barrier();
         if (own && !dry) {
-            const int ty = t - g.sz;
-            const int tz = (c.z > 0) ? t - 1 : t + g.sz - 1;
             const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
             const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
             stc<FDTD_NT_H>(a.S, 0, oi, (oSxy + oSxz + nxy + nxz) * sf);
@@ -1722,16 +1769,14 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
     const int t = threadIdx.x;
-    const int sz = g.sz;
-    const int tbeg = pb * g.tile;
-    const int tend = min(tbeg + g.tile, g.plane);
-    const int ncell = (tend - tbeg) + 3 * sz;          // rows y0-2 .. y1
-    const bool valid = t < ncell;                       // H^{n+1/2}
-    const bool rowe = valid && t >= sz;                 // E^{n+1}: rows y0-1 .. y1
-    const bool rowh = rowe && t < ncell - sz;           // H^{n+3/2}: rows y0-1 .. y1-1
-    const bool own = rowh && t >= 2 * sz;               // rows y0 .. y1-1
-    int p = (tbeg - 2 * sz + t) % g.plane;
-    if (p < 0) p += g.plane;
+    // rows y0-2 .. y1 for the owned rows y0 .. y1-1 (with z segments also
+    // columns z0-2 .. z1 for the owned cells z0 .. z1-1)
+    const TileCell tc = tile_cell<2, 1, 2, 1>(g, pb, t);
+    const bool valid = tc.in;                                                   // H^{n+1/2}
+    const bool rowe = valid && tc.r >= 1 && (!tc.seg || tc.cz >= 1);           // E^{n+1}: y0-1 .. y1
+    const bool rowh = rowe && tc.r < tc.nr - 1 && (!tc.seg || tc.cz < tc.ncz - 1);   // H^{n+3/2}: .. y1-1
+    const bool own = rowh && tc.r >= 2 && (!tc.seg || tc.cz >= 2);             // y0 .. y1-1
+    const int p = tc.p;
     const Cell c = make_cell<T>(g, valid ? p : 0);
     const uint32_t pbytes = (uint32_t)g.plane * (uint32_t)sizeof(T);
     const int x0 = ck * g.xchunk;
@@ -1742,9 +1787,10 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
     const T iz = NU ? g.inv_e[2][c.z] : g.inv_u;
     const T jy = NU ? g.inv_h[1][c.y] : g.inv_u;     // E steps
     const T jz = NU ? g.inv_h[2][c.z] : g.inv_u;
-    const int tym = t - sz, typ = t + sz;
-    const int tzm = (c.z > 0) ? t - 1 : t + sz - 1;
-    const int tzp = (c.z < sz - 1) ? t + 1 : t - (sz - 1);
+    const int tw = g.tw;
+    const int tym = t - tw, typ = t + tw;
+    const int tzm = (tc.cz > 0) ? t - 1 : t + tw - 1;
+    const int tzp = (tc.cz < tw - 1) ? t + 1 : t - (tw - 1);
     const int src_comp = (valid && a.src_amp != nullptr && p == a.src_p) ? a.src_comp : -1;
     for (int i = t; i < a.coef_elems; i += HB) ctab[i] = g.coef[i];
     lds_barrier();
@@ -2035,8 +2081,12 @@ __global__ void add_point(T* f, const T* amp) {
 // Host side
 // ---------------------------------------------------------------------------
 
+// A marching launch's tiles of the (y, z) plane (TileShape in Geom): whole
+// rows (tzb = 1, tzo = tw = sz), or rows cut into z segments of tzo owned
+// cells, each workgroup row then tw = tzo + halo columns wide.
 struct Launch {
     int block, tile, pblocks, nchunk, xchunk, mode;
+    int tyo = 0, tzo = 0, tzb = 0, tw = 0;
 };
 
 struct Plan {
@@ -2076,8 +2126,9 @@ Launch make_launch(int plane, int sx, int block, int tile, int mode, int target)
 // the number of x chunks per tile column that minimises rounds x (planes
 // per chunk + 1), the time model of such a launch (each round runs its
 // workgroups side by side; a chunk takes a time proportional to its planes).
-Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks, int extra = 1) {
-    const int pblocks = (plane + tile - 1) / tile;
+// The chunks per tile column that minimise that cost for pblocks tiles per
+// plane; returns the cost, the chunk count in *best_n.
+int64_t round_chunks(int pblocks, int sx, int max_blocks, int extra, int* best_n) {
     int best = 1;
     int64_t best_cost = -1;
     for (int n = 1; n <= sx; ++n) {
@@ -2092,7 +2143,54 @@ Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks,
             best = n;
         }
     }
+    *best_n = best;
+    return best_cost;
+}
+
+Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks, int extra = 1) {
+    const int pblocks = (plane + tile - 1) / tile;
+    int best = 1;
+    (void)round_chunks(pblocks, sx, max_blocks, extra, &best);
     return make_launch(plane, sx, block, tile, 2, best * pblocks);
+}
+
+// Launch of he_march / he2_march: tiles of tyo rows by tzo z cells, the
+// workgroup holding yh halo rows besides the owned rows and, with z
+// segments, zh halo columns besides the owned cells.  Candidates: whole
+// rows where 1 + yh of them fit the workgroup (z periodic inside a row),
+// and unless whole_only, z segments of FDTD_SEG_MIN (64) up to 256 owned
+// cells, powers of two, on longer rows.  Each is chunked along x as make_round_launch does; the one
+// with the fewest rounds x planes wins, whole rows on a tie.  Cutting rows
+// keeps long rows from leaving a tile mostly halo (rows of 512 cells: one
+// owned row of two in a 1024-thread workgroup).  mode -1: no shape fits.
+Launch make_tile_launch(int sx, int sy, int sz, int block, int yh, int zh, int max_blocks, int extra,
+                        bool whole_only) {
+    Launch best{};
+    best.mode = -1;
+    int64_t best_cost = -1;
+    const auto consider = [&](int tyo, int tzo, int tzb, int tw) {
+        if (tyo < 1) return;
+        const int pblocks = ((sy + tyo - 1) / tyo) * tzb;
+        int n = 1;
+        const int64_t cost = round_chunks(pblocks, sx, max_blocks, extra, &n);
+        if (best_cost >= 0 && cost >= best_cost) return;
+        best_cost = cost;
+        best.block = block;
+        best.mode = 2;
+        best.tyo = tyo;
+        best.tzo = tzo;
+        best.tzb = tzb;
+        best.tw = tw;
+        best.tile = tyo * tzo;
+        best.pblocks = pblocks;
+        best.xchunk = (sx + n - 1) / n;
+        best.nchunk = (sx + best.xchunk - 1) / best.xchunk;
+    };
+    if ((int64_t)(1 + yh) * sz <= block) consider(block / sz - yh, sz, 1, sz);
+    if (!whole_only)
+        for (int tzo = FDTD_SEG_MIN; tzo <= 256; tzo *= 2)
+            if (tzo < sz && tzo + zh <= block) consider(block / (tzo + zh) - yh, tzo, (sz + tzo - 1) / tzo, tzo + zh);
+    return best;
 }
 
 template <class T>
@@ -2102,6 +2200,10 @@ void fill_geom(const Plan& P, const Launch& L, const fdtd_fields* f, Geom<T>& g)
     g.nchunk = L.nchunk;
     g.pblocks = L.pblocks;
     g.tile = L.tile;
+    g.tyo = L.tyo;
+    g.tzo = L.tzo;
+    g.tzb = L.tzb;
+    g.tw = L.tw;
     g.seam_mask = (P.d.bloch_mode == FDTD_BLOCH_PHASE) ? P.d.bloch_mask : 0;
     g.dt = (T)P.d.dt;
     g.inv_u = (T)P.d.inv_dx;
@@ -2268,9 +2370,9 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     return check_launch("fdtd_update_h");
 }
 
-// (lhe is set up only where the pass applies: fused Poynting, rows of at
-// most FDTD_ROW_MAX cells, 3N values within 32-bit byte offsets, and, with
-// Bloch boundaries, FP32, uniform dx and the coefficient block in LDS)
+// (lhe is set up only where the pass applies: fused Poynting, 3N values
+// within 32-bit byte offsets, and, with Bloch boundaries, rows of at most
+// FDTD_ROW_MAX cells, FP32, uniform dx and the coefficient block in LDS)
 bool he_supported(const Plan& P) {
     return P.d.poynting == FDTD_POYNTING_FUSED && P.lhe.mode == 2;
 }
@@ -2386,7 +2488,7 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
 
 int check_he_fields(const Plan& P, const fdtd_fields* f) {
     if (!he_supported(P))
-        return fail(3, "fdtd_update_he needs fused Poynting, rows of at most %d cells and, with Bloch boundaries, "
+        return fail(3, "fdtd_update_he needs fused Poynting and, with Bloch boundaries, rows of at most %d cells, "
                        "FP32, uniform dx and at most %d CPML coefficients", FDTD_ROW_MAX, kCoefTab);
     if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he needs E_out != E");
     if (P.d.bloch_mask) {
@@ -2550,30 +2652,32 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
     const bool pair_ok = d.bloch_mask == 0 || (d.dtype == FDTD_F32 && !d.nonuniform && P->coef_elems <= kCoefTab);
-    if (d.poynting == FDTD_POYNTING_FUSED && pair_ok && P->sz <= FDTD_ROW_MAX &&
+    if (d.poynting == FDTD_POYNTING_FUSED && pair_ok && (d.bloch_mask == 0 || P->sz <= FDTD_ROW_MAX) &&
         3 * N * esz <= (int64_t)UINT32_MAX) {
-        // one workgroup per CU at a time (1024 threads, <= 128 VGPRs)
+        // one workgroup per CU at a time (1024 threads, <= 128 VGPRs); one
+        // halo row (and column); the Bloch-pair pass takes whole rows only
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
-        P->lhe = make_round_launch(P->plane, P->sx, hb, ((hb - P->sz) / P->sz) * P->sz, FDTD_HE_TARGET);
+        P->lhe = make_tile_launch(P->sx, P->sy, P->sz, hb, 1, 1, FDTD_HE_TARGET, 1, d.bloch_mask != 0);
     }
-    // two-step pass: FP32, 1024-thread tiles of whole rows plus three halo
-    // rows, the coefficient block in LDS, slabs of one axis apart (their
-    // psi pairs are carried between the half-steps one slab per axis)
+    // two-step pass: FP32, 1024-thread tiles with three halo rows (and,
+    // with z segments, three halo columns), the coefficient block in LDS,
+    // slabs of one axis apart (their psi pairs are carried between the
+    // half-steps one slab per axis)
     P->lhe2.mode = -1;
     bool apart = true;
     for (int k = 0; k < d.n_face; ++k)
         for (int j = k + 1; j < d.n_face; ++j)
             if (d.face[k].axis == d.face[j].axis && P->face[k].lo < P->face[j].hi && P->face[j].lo < P->face[k].hi)
                 apart = false;
-    if (FDTD_HE2 && P->lhe.mode == 2 && d.bloch_mask == 0 && d.dtype == FDTD_F32 && 4 * P->sz <= FDTD_HE2_HB &&
-        P->coef_elems <= kCoefTab && apart)
-        P->lhe2 = make_round_launch(P->plane, P->sx, FDTD_HE2_HB, ((FDTD_HE2_HB - 3 * P->sz) / P->sz) * P->sz,
-                                    FDTD_HE_TARGET, 3);
+    if (FDTD_HE2 && P->lhe.mode == 2 && d.bloch_mask == 0 && d.dtype == FDTD_F32 && P->coef_elems <= kCoefTab &&
+        apart)
+        P->lhe2 = make_tile_launch(P->sx, P->sy, P->sz, FDTD_HE2_HB, 3, 3, FDTD_HE_TARGET, 3, false);
     // Time per step of each pass: rounds of workgroups x planes a chunk
     // marches, the two-step pass's plane iteration costing FDTD_HE2_COST/10
     // of the fused pass's (measured on MI355X: 1.15 with rows of 73 cells,
-    // 1.10 with rows of 128) and covering two steps.  Long rows leave the two-step tiles few owned rows
-    // of their 1024 threads; the cost model then keeps the fused pass.
+    // 1.10 with rows of 128) and covering two steps.  Where the two-step
+    // tiles own few of their cells (three halo rows and columns against
+    // one), the cost model keeps the fused pass.
     P->he2_pref = false;
     if (P->lhe2.mode == 2) {
         const auto cost = [&](const Launch& L, int extra) {
@@ -2826,9 +2930,16 @@ int fdtd_update_he2_supported(const void* plan) { return plan && he2_supported(*
 namespace {
 
 std::string launch_text(const Launch& L) {
-    char b[160];
-    snprintf(b, sizeof(b), "%d workgroups of %d threads (%d tile columns of %d plane cells x %d x-chunks of %d planes)",
-             L.pblocks * L.nchunk, L.block, L.pblocks, L.tile, L.nchunk, L.xchunk);
+    char b[200];
+    if (L.tzb > 1)
+        snprintf(b, sizeof(b),
+                 "%d workgroups of %d threads (%d tile columns of %d rows x %d-cell z segments x %d x-chunks of "
+                 "%d planes)",
+                 L.pblocks * L.nchunk, L.block, L.pblocks, L.tyo, L.tzo, L.nchunk, L.xchunk);
+    else
+        snprintf(b, sizeof(b),
+                 "%d workgroups of %d threads (%d tile columns of %d plane cells x %d x-chunks of %d planes)",
+                 L.pblocks * L.nchunk, L.block, L.pblocks, L.tile, L.nchunk, L.xchunk);
     return b;
 }
 

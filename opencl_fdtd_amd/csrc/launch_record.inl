@@ -26,6 +26,7 @@ struct fdtd_launch_record {
     int32_t has_geom;
     int32_t sx, sy, sz, plane, xchunk, nchunk, pblocks, tile;
     int32_t src_x, src_p, src_comp;
+    int32_t tyo, tzo, tzb, tw;      // he_march / he2_march tiles (Geom)
     const void* buf[FDTD_REC_SLOTS];
 };
 
@@ -41,6 +42,7 @@ void rec_geom(fdtd_launch_record& r, const Geom<T>& g) {
     r.has_geom = 1;
     r.sx = g.sx; r.sy = g.sy; r.sz = g.sz; r.plane = g.plane;
     r.xchunk = g.xchunk; r.nchunk = g.nchunk; r.pblocks = g.pblocks; r.tile = g.tile;
+    r.tyo = g.tyo; r.tzo = g.tzo; r.tzb = g.tzb; r.tw = g.tw;
 }
 
 template <class T>

@@ -1,7 +1,7 @@
 """Checks of the library's host logic through the launch-recording build
 (test infrastructure; no GPU): plan creation over many shapes, the launch
-decomposition of every kernel (tiles of whole rows with their halo rows,
-x chunks covering the grid), and the step_n schedule: the half-steps in
+decomposition of every kernel (tiles of whole rows or of z segments with
+their halo rows and columns, x chunks covering the grid), and the step_n schedule: the half-steps in
 the order of the caller loop (fdtd.py:162-175), each pass reading the
 buffers the previous one wrote, the source values in order, and the moved
 bits reporting where E, psi_E and H end.
@@ -15,6 +15,8 @@ import itertools
 import re
 import sys
 
+import numpy
+
 SLOTS = 16
 (E_IN, E_OUT, H_IN, H_OUT, PSIE_IN, PSIE_OUT, PSIH_IN, PSIH_OUT, F_IN, F_OUT, G_IN, G_OUT, S, SRC) = range(14)
 
@@ -23,7 +25,7 @@ class Record(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_char * 128), ("grid", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("has_geom", ctypes.c_int32)] + \
                [(n, ctypes.c_int32) for n in ("sx", "sy", "sz", "plane", "xchunk", "nchunk", "pblocks", "tile",
-                                              "src_x", "src_p", "src_comp")] + \
+                                              "src_x", "src_p", "src_comp", "tyo", "tzo", "tzb", "tw")] + \
                [("buf", ctypes.c_void_p * SLOTS)]
 
 
@@ -63,16 +65,74 @@ def _name(r):
 
 
 def _halo_rows(r):
-    """Halo rows a marching launch's workgroup holds besides its tile."""
+    """Halo rows a whole-row marching launch's workgroup holds besides its
+    tile."""
     name, text = _name(r), r.kernel.decode()
-    if name in ("he_march", "heb_march"):
+    if name == "heb_march":
         return 1
-    if name == "he2_march":
-        return 3
     if name == "h_march":
         mode = int(re.search(r"h_march<[^,]+,[^,]+,[^,]+,\s*(\d)", text).group(1))
         return 1 if mode == 2 else 0
     return 0
+
+
+# he_march / he2_march: halo rows below and above the owned rows, halo
+# columns before and after the owned cells of a z segment
+TILE_HALO = {"he_march": (1, 0, 1, 0), "he2_march": (2, 1, 2, 1)}
+_tiles_seen = set()
+
+
+def check_tiles(r):
+    """The 2-D tiles of he_march / he2_march (whole rows, or rows cut into z
+    segments): every plane cell is owned by exactly one workgroup, and for
+    each owned cell the workgroup holds its neighbours at every offset
+    -yl..yh in y and -zl..zh in z, at the periodically wrapped cell, as the
+    pass's stages need them.  The thread -> cell mapping restated from the
+    kernel is enumerated for planes up to 70 000 cells."""
+    name = _name(r)
+    yl, yh, zl, zh = TILE_HALO[name]
+    seg = r.tzb > 1
+    if seg:
+        assert r.tzo < r.sz and r.tzb == -(-r.sz // r.tzo) and r.tw == r.tzo + zl + zh, (name, r.tzo, r.tzb, r.tw)
+    else:
+        assert r.tzb == 1 and r.tzo == r.tw == r.sz, (name, r.tzo, r.tzb, r.tw, r.sz)
+    assert r.tyo >= 1 and (r.tyo + yl + yh) * r.tw <= r.block, (name, r.tyo, r.tw, r.block)
+    assert r.pblocks == -(-r.sy // r.tyo) * r.tzb, (name, r.pblocks, r.sy, r.tyo, r.tzb)
+    key = (name, r.sy, r.sz, r.tyo, r.tzo, r.tzb, r.tw, r.block)
+    if key in _tiles_seen or r.plane > 70000:
+        return
+    _tiles_seen.add(key)
+    t = numpy.arange(r.block)
+    rr, cz = t // r.tw, t % r.tw
+    owned = numpy.zeros(r.plane, dtype=numpy.int64)
+    for pb in range(r.pblocks):
+        iy, iz = divmod(pb, r.tzb)
+        y0 = iy * r.tyo
+        nr = min(r.tyo, r.sy - y0) + yl + yh
+        if seg:
+            z0 = iz * r.tzo
+            ncz = min(r.tzo, r.sz - z0) + zl + zh
+            z = (z0 - zl + cz) % r.sz
+        else:
+            ncz = r.sz
+            z = cz
+        y = (y0 - yl + rr) % r.sy
+        inn = (rr < nr) & (cz < ncz)
+        own = inn & (rr >= yl) & (rr < nr - yh)
+        if seg:
+            own &= (cz >= zl) & (cz < ncz - zh)
+        numpy.add.at(owned, (y * r.sz + z)[own], 1)
+        Y = numpy.full((nr + 1, r.tw + 1), -1)
+        Z = numpy.full((nr + 1, r.tw + 1), -1)
+        Y[rr[inn], cz[inn]] = y[inn]
+        Z[rr[inn], cz[inn]] = z[inn]
+        ro, co = rr[own], cz[own]
+        for dy in range(-yl, yh + 1):
+            for dz in range(-zl, zh + 1):
+                cn = co + dz if seg else (co + dz) % r.sz
+                ok = (Y[ro + dy, cn] == (y[own] + dy) % r.sy) & (Z[ro + dy, cn] == (z[own] + dz) % r.sz)
+                assert ok.all(), (name, r.sy, r.sz, r.tyo, r.tzo, pb, dy, dz)
+    assert (owned == 1).all(), (name, r.sy, r.sz, r.tyo, r.tzo, int(owned.min()), int(owned.max()))
 
 
 def check_decomposition(r):
@@ -83,9 +143,12 @@ def check_decomposition(r):
         assert r.grid == -(-r.plane // r.block) * r.sx, (r.grid, r.plane, r.block, r.sx)
         return
     assert r.pblocks * r.nchunk == r.grid, (_name(r), r.pblocks, r.nchunk, r.grid)
-    assert r.tile > 0 and r.tile * r.pblocks >= r.plane > r.tile * (r.pblocks - 1), (_name(r), r.tile, r.plane)
     assert r.xchunk > 0 and r.xchunk * r.nchunk >= r.sx > r.xchunk * (r.nchunk - 1), (_name(r), r.xchunk, r.sx)
     assert r.plane == r.sy * r.sz
+    if _name(r) in TILE_HALO:
+        check_tiles(r)
+        return
+    assert r.tile > 0 and r.tile * r.pblocks >= r.plane > r.tile * (r.pblocks - 1), (_name(r), r.tile, r.plane)
     halo = _halo_rows(r)
     if halo:
         assert r.tile % r.sz == 0, (_name(r), r.tile, r.sz)        # whole rows
@@ -176,7 +239,8 @@ def _desc(L, shape, faces, dtype, poynting, bloch_mask=0, bloch_mode=0):
 
 SHAPES = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (7, 5, 73), (5, 37, 3), (33, 4, 130), (2, 9, 10), (4, 25, 40),
           (120, 120, 73), (395, 345, 73), (64, 64, 256), (128, 128, 128), (24, 40, 512), (5, 6, 513),
-          (1000, 10, 3), (3, 700, 400), (264, 264, 136), (17, 1, 1), (1, 17, 1), (1, 1, 300)]
+          (1000, 10, 3), (3, 700, 400), (264, 264, 136), (17, 1, 1), (1, 17, 1), (1, 1, 300),
+          (4, 7, 700), (6, 30, 300), (3, 2, 270), (2, 3, 1500)]
 
 
 def run(lib_path: str, quick: bool = False) -> int:

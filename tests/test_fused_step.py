@@ -62,8 +62,11 @@ def _assert_same(a, b, what):
     ((6, 7, 5), 4, 5, numpy.float32, False),         # slabs of both faces overlap on every axis
     ((2, 9, 10), 1, 4, numpy.float32, False),        # two planes: every chunk is one plane
     ((3, 4, 341), 2, 3, numpy.float32, False),       # two owned rows per tile
-    ((3, 4, 512), 2, 4, numpy.float32, False),       # longest rows the fused pass takes: one owned row per tile
+    ((3, 4, 512), 2, 4, numpy.float32, False),       # whole rows of 512: one owned row per tile
     ((4, 3, 400), 3, 5, numpy.float64, True),        # one owned row per tile, FP64, nonuniform
+    ((3, 4, 700), 2, 4, numpy.float32, False),       # rows too long for whole-row tiles: 64-cell z segments
+    ((3, 2, 1500), 2, 3, numpy.float32, False),      # two rows: the halo rows repeat owned rows
+    ((4, 3, 1100), 3, 3, numpy.float64, True),       # FP64, nonuniform, a short last segment
     ((5, 6, 1), 1, 6, numpy.float32, True),          # one-cell rows
 ])
 def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
@@ -85,7 +88,8 @@ def test_step_n_fused_equals_unfused(shape, T, n, dtype, nonuniform):
 @pytest.mark.parametrize("shape, T, n", [
     ((160, 300, 250), 8, 6),     # 12 M cells, 1024-thread tiles of 3 rows, several rounds of workgroups
     ((40, 700, 100), 6, 5),      # 78 tile columns of 9 rows, short x
-    ((24, 40, 512), 8, 5),       # rows of 512: one owned row and its halo row per tile
+    ((24, 40, 512), 8, 5),       # rows of 512 in 64-cell z segments, two steps per pass
+    ((40, 30, 600), 6, 6),       # the two-step pass on 64-cell z segments
 ])
 def test_fused_large_grids_equal_unfused(shape, T, n):
     """Launch shapes the cost model gives larger grids (multiple rounds of
@@ -134,7 +138,9 @@ def test_update_he_equals_h_then_e(dtype):
 @pytest.mark.parametrize("shape, T, nonuniform", [
     ((10, 12, 14), 3, False),
     ((10, 12, 14), 3, True),
-    ((7, 3, 256), 1, False),     # longest rows the two-step pass takes: one owned row per tile
+    ((7, 3, 256), 1, False),     # whole rows of 256: one owned row per two-step tile
+    ((3, 2, 270), 1, False),     # two-step pass on 64-cell z segments, a short last one, two rows
+    ((4, 5, 700), 2, True),      # nonuniform, z segments for both passes
     ((4, 2, 9), 1, False),       # two rows: the halo rows are owned rows of the same tile
     ((4, 25, 40), 2, False),     # four planes, several tiles per column
     ((2, 5, 9), 1, False),       # two planes: the chunk's extra planes wrap onto its own
@@ -328,6 +334,8 @@ def test_per_call_loop_mixed_with_other_calls():
     ((6, 7, 5), 4, False),           # overlapping slabs on every axis
     ((23, 30, 73), 8, False),        # README row length, CPML-8
     ((120, 120, 73), 8, True),       # the same rows, a grid the two-step pass is chosen for
+    ((40, 30, 600), 8, True),        # the two-step pass on 64-cell z segments
+    ((6, 7, 700), 3, False),         # the one-step pass on 64-cell z segments
 ])
 def test_fused_step_n_matches_oracle(shape, T, two):
     """step_n through the fused pass (and the two-step pass) against the
@@ -365,7 +373,8 @@ def test_fusion_is_off_where_unsupported():
         s.update_HE([])
     assert Simulation(eps, [])._psi_e_out is not None
     assert Simulation(eps, [], float_type=numpy.float64)._psi_e_out is None
-    assert Simulation([numpy.ones((4, 4, 300))] * 3, [])._psi_e_out is None
+    thick = [{"axis": "z", "polarity": p, "thickness": 3} for p in "np"]   # 3 + 3 of 5 z cells overlap
+    assert Simulation([numpy.ones((6, 7, 5))] * 3, thick)._psi_e_out is None
     assert not Simulation(eps, [], float_type=numpy.float64).two_step_passes
     with pytest.raises(Exception):
         Simulation(eps, [], float_type=numpy.float64).update_HE2([])

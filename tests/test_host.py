@@ -162,8 +162,9 @@ def test_workloads():
 
 
 def test_update_he_support_rules():
-    """The fused H+E pass needs fused Poynting and rows of at most 512 cells,
-    and with Bloch boundaries FP32 (plan-level query, no GPU)."""
+    """The fused H+E pass needs fused Poynting (long rows are cut into z
+    segments), and with Bloch boundaries FP32 and rows of at most 512 cells
+    (plan-level query, no GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -177,7 +178,10 @@ def test_update_he_support_rules():
         return r
     assert supported(poynting=1) == 1
     assert supported(poynting=1, shape=(4, 4, 512)) == 1
-    assert supported(poynting=1, shape=(4, 4, 513)) == 0
+    assert supported(poynting=1, shape=(4, 4, 513)) == 1
+    assert supported(poynting=1, shape=(4, 4, 5000)) == 1
+    assert supported(poynting=1, shape=(4, 4, 512), bloch_mask=1) == 1
+    assert supported(poynting=1, shape=(4, 4, 513), bloch_mask=1) == 0
     assert supported(poynting=0) == 0
     assert supported(poynting=2) == 0
     # a Bloch pair runs the pass in FP32 on uniform grids
@@ -217,9 +221,9 @@ def test_update_he_rejects_bad_calls():
 
 
 def test_update_he2_support_rules():
-    """The two-step pass needs FP32, fused Poynting, no Bloch boundaries,
-    rows of at most 256 cells, a coefficient block that fits its LDS copy
-    and the CPML slabs of one axis apart (plan-level query, no GPU)."""
+    """The two-step pass needs FP32, fused Poynting, no Bloch boundaries, a
+    coefficient block that fits its LDS copy and the CPML slabs of one axis
+    apart; rows of any length (plan-level query, no GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -235,7 +239,8 @@ def test_update_he2_support_rules():
     assert supported(faces=six) == 1
     assert supported(faces=()) == 1
     assert supported(shape=(4, 4, 256)) == 1
-    assert supported(shape=(4, 4, 257)) == 0
+    assert supported(shape=(4, 4, 257)) == 1
+    assert supported(shape=(4, 4, 5000)) == 1
     assert supported(dtype=1) == 0
     assert supported(poynting=0) == 0
     assert supported(poynting=2) == 0
@@ -248,8 +253,10 @@ def test_update_he2_support_rules():
 
 def test_update_he2_preference():
     """step_n takes the two-step pass where its time model beats the
-    one-step pass: rows of 73 cells (11 of a tile's 14 rows owned), not
-    rows of 250 (1 of 4)."""
+    one-step pass: rows of 73 cells (11 of a tile's 14 rows owned), rows of
+    250 and 300 (12 of 15 rows of 64-cell z segments), but not a grid so
+    small that every x chunk is one plane (6 x 7 x 700: the two-step pass
+    would march three dry planes per chunk against one)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -261,8 +268,9 @@ def test_update_he2_preference():
         lib.fdtd_plan_destroy(plan)
         return r
     assert pref((395, 345, 73), 8) == (1, 1)
-    assert pref((64, 64, 250), 8) == (1, 0)
-    assert pref((64, 64, 300), 8) == (0, 0)
+    assert pref((64, 64, 250), 8) == (1, 1)
+    assert pref((64, 64, 300), 8) == (1, 1)
+    assert pref((6, 7, 700), 2) == (1, 0)
     assert lib.fdtd_update_he2_preferred(None) == 0
 
 
