@@ -25,6 +25,7 @@
  *                         (fdtd.py:168-169), with f(t) read from device memory
  *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
  *   fdtd_step_n_moved     (E, F, source, H, G, S) without host round trips
+ *   fdtd_step_n_observed  (the same with a plane flux and field copies per step)
  *   fdtd_update_he     <- sim.update_H + sim.update_S of one iteration of that
  *                         loop followed by sim.update_E and the soft source of
  *                         the next, in one pass over the grid
@@ -173,6 +174,28 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
  * buffer roles accordingly. */
 int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
                       const void* src_wave, void* stream, int32_t* moved);
+
+/* What fdtd_step_n_observed records after every step k = 0 .. n-1.  No
+ * counterpart in the reference: a caller there reads sim.S (or E, H) on the
+ * host between steps; this keeps the stepping on the device for the
+ * objective and checkpointed recompute of the adjoint. */
+typedef struct fdtd_observe {
+    double* flux_acc;        /* NULL: none; else flux_acc[0] += the sum of S component flux_comp over the
+                                plane {axis flux_axis coordinate == flux_index} (as fdtd_plane_sum), per step */
+    int32_t flux_comp, flux_axis;
+    int64_t flux_index;
+    double* flux_partials;   /* NULL: each step's sum as fdtd_plane_sum (one workgroup); else
+                                fdtd_reduce_partials(plan) doubles of device scratch for the same sum over
+                                many workgroups (another fixed order; faster on planes normal to z) */
+    void* const* E_keep;     /* NULL, or n device buffers of 3N values: E after step k is copied to E_keep[k] */
+    void* const* H_keep;     /* NULL, or the same for H */
+} fdtd_observe;
+
+/* fdtd_step_n_moved (same loop, same results, same *moved bits) with the
+ * observations of *obs after every step.  It runs one step per pass (the
+ * two-step pass does not keep the first step's S); obs may be NULL. */
+int fdtd_step_n_observed(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
+                         const void* src_wave, const fdtd_observe* obs, void* stream, int32_t* moved);
 
 /* The H half-step with S followed by the next E half-step in one pass:
  * H_out = H-step(E, H); S; E_out = E-step(E, H_out); then, when src_amp is

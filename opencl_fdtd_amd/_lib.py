@@ -22,7 +22,7 @@ BLOCH_PHASE, BLOCH_DECOUPLED = 0, 1
 EXPORTS = (
     "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy", "fdtd_plan_describe",
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
-    "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n", "fdtd_step_n_moved",
+    "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n", "fdtd_step_n_moved", "fdtd_step_n_observed",
     "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
     "fdtd_update_he2_preferred",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy", "fdtd_energy_noalloc",
@@ -63,6 +63,12 @@ class AdjointBufs(ctypes.Structure):
         ("parts", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Observe(ctypes.Structure):
+    _fields_ = [("flux_acc", ctypes.c_void_p), ("flux_comp", ctypes.c_int32), ("flux_axis", ctypes.c_int32),
+                ("flux_index", ctypes.c_int64), ("flux_partials", ctypes.c_void_p),
+                ("E_keep", ctypes.c_void_p), ("H_keep", ctypes.c_void_p)]
+
+
 class FdtdError(RuntimeError):
     pass
 
@@ -101,6 +107,8 @@ def load() -> ctypes.CDLL:
     lib.fdtd_add_point.argtypes = [P, P, ctypes.c_int64, P, P]
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_step_n_moved.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.POINTER(ctypes.c_int32)]
+    lib.fdtd_step_n_observed.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.POINTER(Observe), P,
+                                         ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_update_he.argtypes = [P, F, ctypes.c_int64, P, P]
     lib.fdtd_update_he_supported.argtypes = [P]
     lib.fdtd_update_he2.argtypes = [P, F, ctypes.c_int64, P, P]

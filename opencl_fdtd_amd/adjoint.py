@@ -64,25 +64,20 @@ class FluxAdjoint:
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
 
-    def _step(self, n: int) -> None:
-        self.sim.step_n(1, self.p.src_index, self.wave[n:n + 1])
-
-    def _flux(self, acc: torch.Tensor) -> None:
-        rc = self.lib.fdtd_plane_sum(self.sim.plan, ctypes.c_void_p(self.sim._S.data_ptr()), self.p.axis,
-                                     self.p.axis, self.p.index, ctypes.c_void_p(acc.data_ptr()), self._stream())
-        _lib.check(rc, "fdtd_plane_sum")
-
     def forward(self, keep_checkpoints: bool = True) -> Tuple[float, List[Dict[str, torch.Tensor]]]:
         """Runs the N steps; returns J and the checkpoints (state before
-        steps 0, K, 2K, ...)."""
-        K = self.p.checkpoint_every
+        steps 0, K, 2K, ...).  Each segment is one call that steps on the
+        device and adds the plane's flux after every step."""
+        N = self.p.steps
+        K = self.p.checkpoint_every if keep_checkpoints else max(N, 1)
         acc = torch.zeros(1, dtype=torch.float64, device=self.dev)
         cps = []
-        for n in range(self.p.steps):
-            if keep_checkpoints and n % K == 0:
+        for n0 in range(0, N, K):
+            n1 = min(n0 + K, N)
+            if keep_checkpoints:
                 cps.append(self.sim.checkpoint())
-            self._step(n)
-            self._flux(acc)
+            self.sim.step_n_observed(n1 - n0, self.p.src_index, self.wave[n0:n1],
+                                     flux=(self.p.axis, self.p.axis, self.p.index, acc))
         return float(acc.item()), cps
 
     def gradient(self) -> Tuple[float, torch.Tensor]:
@@ -111,12 +106,10 @@ class FluxAdjoint:
             n0 = s * K
             n1 = min(n0 + K, self.p.steps)
             sim.restore(cps[s])
-            Es = [sim._E.clone()]
-            Hs = [sim.H.clone()]
-            for n in range(n0, n1):
-                self._step(n)
-                Es.append(sim._E.clone())
-                Hs.append(sim.H.clone())
+            # the segment's fields: before its first step, then after each
+            Es = [sim._E.clone()] + [torch.empty_like(sim._E) for _ in range(n1 - n0)]
+            Hs = [sim.H.clone()] + [torch.empty_like(sim._E) for _ in range(n1 - n0)]
+            sim.step_n_observed(n1 - n0, self.p.src_index, self.wave[n0:n1], keep_E=Es[1:], keep_H=Hs[1:])
             for n in reversed(range(n0, n1)):
                 k = n - n0
                 b.E_prev, b.E_next = Es[k].data_ptr(), Es[k + 1].data_ptr()

@@ -34,6 +34,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -2783,8 +2784,24 @@ namespace {
 // n steps of the caller loop.  home: E and psi_e end in f->E and f->psi_e
 // (pass counts kept even); otherwise they may end in f->E_out / f->psi_e_out
 // and *moved says where (bit 0: E, bit 1: psi_e).
+int plane_sum_many(const Plan& P, const void* field, int comp, int axis, int64_t index, double* partials,
+                   double* acc, hipStream_t st);   // adjoint.inl
+
+// Device-to-device copy on the stream (a recorded pseudo-launch in the
+// launch-recording build).
+int copy_field(void* dst, const void* src, size_t bytes, hipStream_t st) {
+#ifdef FDTD_RECORD_LAUNCHES
+    fdtd_record_copy(dst, src, bytes);
+    (void)st;
+#else
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return fail(5, "hipMemcpyAsync of %zu bytes failed", bytes);
+#endif
+    return 0;
+}
+
 int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
-                void* stream, bool home, int32_t* moved) {
+                void* stream, bool home, int32_t* moved, const fdtd_observe* obs = nullptr) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
     if (moved) *moved = 0;
@@ -2792,12 +2809,37 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
     if (int rc = check_fields(P, f, true)) return rc;
+    if (obs && obs->flux_acc) {
+        if (P.d.poynting == FDTD_POYNTING_OFF || !f->S) return fail(3, "the flux needs S (Poynting on)");
+        if (obs->flux_comp < 0 || obs->flux_comp > 2 || obs->flux_axis < 0 || obs->flux_axis > 2)
+            return fail(3, "flux component %d / axis %d out of [0, 3)", obs->flux_comp, obs->flux_axis);
+        const int64_t s = obs->flux_axis == 0 ? P.sx : obs->flux_axis == 1 ? P.sy : P.sz;
+        if (obs->flux_index < 0 || obs->flux_index >= s)
+            return fail(3, "flux plane %lld out of [0, %lld)", (long long)obs->flux_index, (long long)s);
+    }
     hipStream_t st = (hipStream_t)stream;
     fdtd_fields cur = *f;
     const size_t esz = P.d.dtype == FDTD_F32 ? 4 : 8;
     const bool swap = (f->H_out && f->H_out != f->H);
     const auto amp_of = [&](int k) -> const void* {
         return src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
+    };
+    // observations of step k: E once its E half-step (and source) is done,
+    // the flux and H once its H half-step (and S) is
+    const size_t fbytes = (size_t)(3 * P.N) * esz;
+    const auto seen_e = [&](int k) -> int {
+        return (obs && obs->E_keep) ? copy_field(obs->E_keep[k], cur.E, fbytes, st) : 0;
+    };
+    const auto seen_h = [&](int k) -> int {
+        if (obs && obs->flux_acc) {
+            const int rc = obs->flux_partials
+                               ? plane_sum_many(P, cur.S, obs->flux_comp, obs->flux_axis, obs->flux_index,
+                                                obs->flux_partials, obs->flux_acc, st)
+                               : fdtd_plane_sum(plan, cur.S, obs->flux_comp, obs->flux_axis, obs->flux_index,
+                                                obs->flux_acc, stream);
+            if (rc) return rc;
+        }
+        return (obs && obs->H_keep) ? copy_field(obs->H_keep[k], cur.H, fbytes, st) : 0;
     };
     const auto half_e = [&](int k) {
         return (P.d.dtype == FDTD_F32) ? launch_e<float>(P, &cur, st, (const float*)amp_of(k), src_index)
@@ -2828,11 +2870,12 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
                            (f->F_out && f->F_out != f->F && f->G_out && f->G_out != f->G &&
                             (P.d.n_face == 0 || (f->psi_g_out && f->psi_g_out != f->psi_g)));
     const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P) && pair_bufs;
-    const bool fuse2 = fuse && he2_supported(P) && P.he2_pref &&
+    const bool fuse2 = fuse && !obs && he2_supported(P) && P.he2_pref &&
                        (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
     if (fuse) {
         if (int rc = half_e(0)) return rc;
+        if (int rc = seen_e(0)) return rc;
         k = 1;
         const int m = n - 1;
         const int n2 = fuse2 ? (home ? 2 * (m / 4) : m / 2) : 0;
@@ -2842,7 +2885,9 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
         const int r = m - 2 * n2 - (heh ? 1 : 0);
         if (home && r % 2 == 1) {
             if (int rc = half_h()) return rc;
+            if (int rc = seen_h(0)) return rc;
             if (int rc = half_e(1)) return rc;
+            if (int rc = seen_e(1)) return rc;
             k = 2;
         }
         for (int j = 0; j < (home ? r / 2 * 2 : r); ++j, ++k) {
@@ -2857,6 +2902,8 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
                 std::swap(cur.G, cur.G_out);
                 std::swap(cur.psi_g, cur.psi_g_out);
             }
+            if (int rc = seen_h(k - 1)) return rc;
+            if (int rc = seen_e(k)) return rc;
         }
         for (int j = 0; j < n2 + (heh ? 1 : 0); ++j, k += 2) {
             const bool last_e = !(heh && j == n2);
@@ -2866,12 +2913,16 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
             std::swap(cur.E, cur.E_out);
             std::swap(cur.psi_e, cur.psi_e_out);
         }
-        if (!heh)
+        if (!heh) {
             if (int rc = half_h()) return rc;
+            if (int rc = seen_h(n - 1)) return rc;
+        }
     } else {
         for (k = 0; k < n; ++k) {
             if (int rc = half_e(k)) return rc;
+            if (int rc = seen_e(k)) return rc;
             if (int rc = half_h()) return rc;
+            if (int rc = seen_h(k)) return rc;
         }
     }
     // a two-step pass swaps H once for two H half-steps, so with an odd
@@ -2894,6 +2945,16 @@ int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t
                       void* stream, int32_t* moved) {
     if (!moved) return fail(2, "moved is NULL");
     return step_n_impl(plan, f, n, src_index, src_wave, stream, false, moved);
+}
+
+int fdtd_step_n_observed(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
+                         const void* src_wave, const fdtd_observe* obs, void* stream, int32_t* moved) {
+    if (!moved) return fail(2, "moved is NULL");
+    if (obs)
+        for (int k = 0; k < n; ++k)
+            if ((obs->E_keep && !obs->E_keep[k]) || (obs->H_keep && !obs->H_keep[k]))
+                return fail(2, "E_keep / H_keep entry %d is NULL", k);
+    return step_n_impl(plan, f, n, src_index, src_wave, stream, false, moved, obs);
 }
 
 int fdtd_update_he(const void* plan, const fdtd_fields* f, int64_t src_index, const void* src_amp,

@@ -17,7 +17,7 @@ extern "C" {
 enum {
     FDTD_REC_E_IN = 0, FDTD_REC_E_OUT, FDTD_REC_H_IN, FDTD_REC_H_OUT, FDTD_REC_PSIE_IN, FDTD_REC_PSIE_OUT,
     FDTD_REC_PSIH_IN, FDTD_REC_PSIH_OUT, FDTD_REC_F_IN, FDTD_REC_F_OUT, FDTD_REC_G_IN, FDTD_REC_G_OUT,
-    FDTD_REC_S, FDTD_REC_SRC, FDTD_REC_SLOTS = 16
+    FDTD_REC_S, FDTD_REC_SRC, FDTD_REC_PTR0, FDTD_REC_SLOTS = 16
 };
 
 struct fdtd_launch_record {
@@ -125,6 +125,14 @@ void rec_fill(fdtd_launch_record& r, const HEBArgs<T>& a) {
 template <class A>
 void rec_fill(fdtd_launch_record& r, const A&) { r.src_comp = -1; }
 
+// kernels with plain arguments (plane_sum, the reductions, add_point, ...):
+// the first argument when it is a pointer
+template <class T>
+void rec_fill(fdtd_launch_record& r, T* const& p) {
+    r.src_comp = -1;
+    r.buf[FDTD_REC_PTR0] = p;
+}
+
 template <class A, class... Rest>
 void fdtd_record_launch(const char* kernel, dim3 grid, dim3 block, const A& a, const Rest&...) {
     fdtd_launch_record r;
@@ -133,6 +141,20 @@ void fdtd_record_launch(const char* kernel, dim3 grid, dim3 block, const A& a, c
     r.grid = (int32_t)(grid.x * grid.y * grid.z);
     r.block = (int32_t)(block.x * block.y * block.z);
     rec_fill(r, a);
+    std::lock_guard<std::mutex> lock(g_records_mu);
+    g_records.push_back(r);
+}
+
+// A device-to-device copy: kernel "copy", source in E_IN, destination in
+// E_OUT, the byte count in grid.
+void fdtd_record_copy(void* dst, const void* src, size_t bytes) {
+    fdtd_launch_record r;
+    memset(&r, 0, sizeof(r));
+    snprintf(r.kernel, sizeof(r.kernel), "copy");
+    r.grid = (int32_t)std::min(bytes, (size_t)INT32_MAX);
+    r.src_comp = -1;
+    r.buf[FDTD_REC_E_IN] = src;
+    r.buf[FDTD_REC_E_OUT] = dst;
     std::lock_guard<std::mutex> lock(g_records_mu);
     g_records.push_back(r);
 }

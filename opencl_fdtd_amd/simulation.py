@@ -269,6 +269,7 @@ class Simulation:
         self._plan = plan
         # second E buffer: step_n then runs each H half-step and the next E
         # half-step as one pass over the grid (fdtd_update_he)
+        self._flux_partials = None      # step_n_observed's plane-sum partials, allocated on first use
         self._E_out = None
         if fuse_steps and self._lib.fdtd_update_he_supported(plan):
             self._E_out = torch.empty_like(self._E)
@@ -529,6 +530,49 @@ class Simulation:
         src_wave[k]``, H (+G) with S.  ``src_wave`` is a device tensor (or
         array) of ``n`` source values in the field dtype.  E and psi_E may end
         in either of their two buffers (``sim.E`` follows)."""
+        return self._step_n(n, src_index, src_wave, events, None)
+
+    def step_n_observed(self, n: int, src_index: Optional[int] = None, src_wave=None, flux=None,
+                        keep_E: Optional[Sequence[torch.Tensor]] = None,
+                        keep_H: Optional[Sequence[torch.Tensor]] = None, events: Sequence = ()) -> Event:
+        """``step_n`` with observations after every step k, on the device
+        (fdtd_step_n_observed; no counterpart in the reference, whose caller
+        reads fields on the host between steps):
+        ``flux = (comp, axis, index, acc)``: ``acc[0] +=`` the FP64 sum of
+        S_comp over the plane {axis coordinate == index} (``acc``: a float64
+        device tensor), summed over many workgroups in a fixed order;
+        ``keep_E`` / ``keep_H``: n device tensors of 3N values in the field
+        dtype that receive E / H after step k.
+        It runs one step per pass (the two-step pass does not keep the first
+        step's S); the fields and S are those of ``step_n``."""
+        obs = _lib.Observe()
+        alive = []
+        if flux is not None:
+            comp, axis, index, acc = flux
+            if not (isinstance(acc, torch.Tensor) and acc.dtype == torch.float64 and acc.device == self.device):
+                raise Exception("the flux accumulator must be a float64 tensor on {}".format(self.device))
+            obs.flux_acc = acc.data_ptr()
+            obs.flux_comp, obs.flux_axis, obs.flux_index = int(comp), int(axis), int(index)
+            if self._flux_partials is None:
+                self._flux_partials = torch.empty(int(self._lib.fdtd_reduce_partials(self._plan)),
+                                                  dtype=torch.float64, device=self.device)
+            obs.flux_partials = self._flux_partials.data_ptr()
+        for name, keep in (("E_keep", keep_E), ("H_keep", keep_H)):
+            if keep is None:
+                continue
+            if len(keep) != n:
+                raise Exception("{} holds {} tensors, need {}".format(name, len(keep), n))
+            for t in keep:
+                if (t.device != self.device or t.dtype != self._tdtype or t.numel() != 3 * self.N
+                        or not t.is_contiguous()):
+                    raise Exception("{}: each tensor must be a contiguous {} tensor of {} values on {}".format(
+                        name, self._tdtype, 3 * self.N, self.device))
+            arr = (ctypes.c_void_p * max(n, 1))(*[t.data_ptr() for t in keep])
+            alive.append(arr)
+            setattr(obs, name, ctypes.cast(arr, ctypes.c_void_p))
+        return self._step_n(n, src_index, src_wave, events, obs)
+
+    def _step_n(self, n, src_index, src_wave, events, obs) -> Event:
         self.flush()
         stream = self._stream()
         for e in events or ():
@@ -545,11 +589,16 @@ class Simulation:
             wave_ptr = wave.data_ptr()
         f = self._fields()
         moved = ctypes.c_int32(0)
+        idx = int(src_index if src_index is not None else -1)
         with torch.cuda.device(self.device):
-            _lib.check(self._lib.fdtd_step_n_moved(self._plan, ctypes.byref(f), int(n),
-                                                   int(src_index if src_index is not None else -1),
-                                                   wave_ptr, ctypes.c_void_p(stream.cuda_stream),
-                                                   ctypes.byref(moved)), "fdtd_step_n_moved")
+            if obs is None:
+                _lib.check(self._lib.fdtd_step_n_moved(self._plan, ctypes.byref(f), int(n), idx, wave_ptr,
+                                                       ctypes.c_void_p(stream.cuda_stream), ctypes.byref(moved)),
+                           "fdtd_step_n_moved")
+            else:
+                _lib.check(self._lib.fdtd_step_n_observed(self._plan, ctypes.byref(f), int(n), idx, wave_ptr,
+                                                          ctypes.byref(obs), ctypes.c_void_p(stream.cuda_stream),
+                                                          ctypes.byref(moved)), "fdtd_step_n_observed")
         if len(self._H) == 2 and moved.value & 4:
             self._h = 1 - self._h
         if moved.value & 1:

@@ -18,7 +18,7 @@ import sys
 import numpy
 
 SLOTS = 16
-(E_IN, E_OUT, H_IN, H_OUT, PSIE_IN, PSIE_OUT, PSIH_IN, PSIH_OUT, F_IN, F_OUT, G_IN, G_OUT, S, SRC) = range(14)
+(E_IN, E_OUT, H_IN, H_OUT, PSIE_IN, PSIE_OUT, PSIH_IN, PSIH_OUT, F_IN, F_OUT, G_IN, G_OUT, S, SRC, PTR0) = range(15)
 
 
 class Record(ctypes.Structure):
@@ -41,6 +41,8 @@ def _load(path):
     lib.fdtd_psi_elems.restype = ctypes.c_int64
     lib.fdtd_step_n_moved.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_step_n_observed.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.POINTER(L.Observe), P,
+                                         ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_update_he2_preferred.argtypes = [P]
     lib.fdtd_update_he_supported.argtypes = [P]
     lib.fdtd_record_count.restype = ctypes.c_int64
@@ -169,14 +171,42 @@ def _halves(r):
     return HALF[name]
 
 
-def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong):
-    """Half-steps E H E H ..., each launch reading what the previous wrote."""
+def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None):
+    """Half-steps E H E H ..., each launch reading what the previous wrote.
+    obs (fdtd_step_n_observed): dict with "flux" (S + comp * N * esz, the
+    plane_sum input, or None) and "E_keep" / "H_keep" (destination lists or
+    None); after the k-th E half-step E is copied to E_keep[k], after the
+    k-th H half-step the flux is summed from S and H copied to H_keep[k]."""
     loc = {"E": f.E, "H": f.H, "psi_e": f.psi_e, "psi_h": f.psi_h, "F": f.F, "G": f.G}
     seq = []
     k_src = 0
+    seen = {"flux": 0, "E": 0, "H": 0}
+    prev = None
     for r in recs:
         name = _name(r)
+        last, prev = prev, name
         if name in ("s_from_os", "add_point"):
+            continue
+        if name in ("plane_sum", "plane_partial"):
+            assert obs is not None and obs["flux"] is not None, "plane sum without a flux"
+            assert (name == "plane_partial") == (obs["partials"] is not None), name
+            assert r.buf[PTR0] == obs["flux"], ("flux reads", r.buf[PTR0], obs["flux"])
+            assert seq.count("H") == seen["flux"] + 1, ("flux after H half-step", seq, seen)
+            seen["flux"] += 1
+            continue
+        if name == "reduce_accumulate":
+            assert last == "plane_partial" and r.buf[PTR0] == obs["partials"], (last, r.buf[PTR0])
+            continue
+        if name == "copy":
+            assert obs is not None
+            dst, src = r.buf[E_OUT], r.buf[E_IN]
+            which = "E" if obs["E_keep"] and dst in obs["E_keep"] else "H"
+            k = seen[which]
+            assert obs[which + "_keep"] and dst == obs[which + "_keep"][k], (which, k)
+            assert src == loc[which], ("copy of the newest", which, k)
+            assert seq.count(which) == k + 1, ("copy after the half-step", which, k, seq)
+            assert r.grid == min(obs["bytes"], 2 ** 31 - 1), (r.grid, obs["bytes"])
+            seen[which] += 1
             continue
         assert name in HALF, name
         b = r.buf
@@ -215,6 +245,9 @@ def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong):
     assert "".join(seq) == "EH" * n, ("".join(seq), n)
     if wave and n:
         assert k_src == n, (k_src, n)
+    if obs is not None:
+        assert seen["flux"] == (n if obs["flux"] is not None else 0), seen
+        assert seen["E"] == (n if obs["E_keep"] else 0) and seen["H"] == (n if obs["H_keep"] else 0), seen
     if moved is not None:
         assert bool(moved & 1) == (loc["E"] == f.E_out) and (loc["E"] in (f.E, f.E_out)), moved
         assert bool(moved & 2) == (loc["psi_e"] == f.psi_e_out) or f.psi_e_out is None
@@ -241,6 +274,54 @@ SHAPES = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (7, 5, 73), (5, 37, 3), (33, 4, 130),
           (120, 120, 73), (395, 345, 73), (64, 64, 256), (128, 128, 128), (24, 40, 512), (5, 6, 513),
           (1000, 10, 3), (3, 700, 400), (264, 264, 136), (17, 1, 1), (1, 17, 1), (1, 1, 300),
           (4, 7, 700), (6, 30, 300), (3, 2, 270), (2, 3, 1500)]
+
+
+def _observed(lib, L, plan, shape, esz, poynting, bloch, pingpong):
+    """fdtd_step_n_observed: the flux after every H half-step, the copies of
+    E and H after every half-step, in the step_n schedule (one step per
+    pass), on fake device addresses."""
+    N = shape[0] * shape[1] * shape[2]
+    checked = 0
+    for n, flux, keep in itertools.product((0, 1, 2, 3, 5, 8), (False, "one", "many"), (False, True)):
+        if flux and poynting == 0:
+            continue
+        f = L.Fields()
+        addrs = iter(range(0x100000, 0x10000000, 0x100000))
+        for name, _ in L.Fields._fields_:
+            setattr(f, name, next(addrs))
+        if not pingpong:
+            f.H_out, f.psi_h_out, f.G_out, f.psi_g_out = f.H, f.psi_h, f.G, f.psi_g
+        if not bloch:
+            f.F = f.G = f.G_out = f.psi_f = f.psi_g = f.psi_g_out = f.F_out = None
+        if poynting != 2:
+            f.oS = None
+        ob = L.Observe()
+        info = {"flux": None, "partials": None, "E_keep": None, "H_keep": None, "bytes": 3 * N * esz}
+        arrays = []
+        if flux:
+            comp, axis = 2, 1
+            ob.flux_acc, ob.flux_comp, ob.flux_axis, ob.flux_index = 0x60000000, comp, axis, shape[1] // 2
+            info["flux"] = f.S + comp * N * esz
+            if flux == "many":
+                ob.flux_partials = info["partials"] = 0x68000000
+        if keep:
+            for name, base in (("E_keep", 0x20000000), ("H_keep", 0x40000000)):
+                ptrs = [base + k * 0x100000 for k in range(n)]
+                arr = (ctypes.c_void_p * max(n, 1))(*ptrs)
+                arrays.append(arr)
+                setattr(ob, name, ctypes.cast(arr, ctypes.c_void_p))
+                info[name] = ptrs
+        moved = ctypes.c_int32(-1)
+        wave = 0x70000000
+        rc = lib.fdtd_step_n_observed(plan, ctypes.byref(f), n, 1, wave, ctypes.byref(ob), None, ctypes.byref(moved))
+        assert rc == 0, (shape, n, flux, keep, lib.fdtd_last_error())
+        recs = _records(lib)
+        assert not any(_name(r) == "he2_march" for r in recs), "observed steps run one step per pass"
+        for r in recs:
+            check_decomposition(r)
+        check_schedule(recs, f, n, moved.value, wave, esz, bloch, pingpong, obs=info)
+        checked += 1
+    return checked
 
 
 def run(lib_path: str, quick: bool = False) -> int:
@@ -287,6 +368,7 @@ def run(lib_path: str, quick: bool = False) -> int:
                     check_decomposition(r)
                 check_schedule(recs, f, n, None if homekeep else moved.value, wave, esz, bool(bloch), pingpong)
                 checked += 1
+            checked += _observed(lib, L, plan, shape, esz, poynting, bool(bloch), pingpong)
         finally:
             lib.fdtd_plan_destroy(plan)
     return checked

@@ -452,3 +452,97 @@ def test_bloch_step_n_and_loop_match_oracle():
         g, o = fetch(sim, name), getattr(orc, name)
         bad = numpy.flatnonzero(g != o)
         assert bad.size == 0, (name, bad.size, bad[:4])
+
+
+@pytest.mark.parametrize("shape, T, dtype, mode", [
+    ((10, 12, 14), 3, numpy.float32, "fused"),
+    ((30, 40, 73), 4, numpy.float32, "fused"),       # step_n would take the two-step pass here
+    ((6, 7, 8), 2, numpy.float64, "fused"),
+    ((6, 7, 8), 2, numpy.float32, "two_pass"),       # S from oS after every H half-step
+    ((3, 4, 700), 2, numpy.float32, "fused"),        # 64-cell z segments
+])
+def test_step_n_observed_equals_single_steps(shape, T, dtype, mode):
+    """fdtd_step_n_observed (one call of n steps) against one step per call
+    with fdtd_plane_sum and clones after every step: every E and H copy and
+    the final fields and psi blocks bit for bit; the flux, an FP64 sum of
+    the same FP32 S values in another order, against the exactly rounded
+    sum (math.fsum) of those values to 1e-13."""
+    import ctypes
+    from opencl_fdtd_amd import Simulation, _lib
+    print("case step_n_observed", shape, T, dtype.__name__, mode, flush=True)
+    eps, pmls, init, _ = _case(shape, T, seed=sum(shape), pec=True)
+    mk = lambda: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], initial_fields=init,
+                            float_type=dtype, poynting_mode=mode)
+    a, b = mk(), mk()
+    n = 9
+    idx = 2 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 3) * shape[2] + shape[2] // 2
+    wave = numpy.sin(0.4 * numpy.arange(n)).astype(dtype)
+    comp, axis, index = 2, 1, shape[1] // 2
+    acc_a = torch.zeros(1, dtype=torch.float64, device="cuda")
+    acc_b = torch.zeros(1, dtype=torch.float64, device="cuda")
+    keep_E = [torch.empty_like(a._E) for _ in range(n)]
+    keep_H = [torch.empty_like(a._E) for _ in range(n)]
+    a.step_n_observed(n, idx, wave, flux=(comp, axis, index, acc_a), keep_E=keep_E, keep_H=keep_H)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ref_E, ref_H, values = [], [], []
+    for k in range(n):
+        b.step_n(1, idx, wave[k:k + 1])
+        _lib.check(b._lib.fdtd_plane_sum(b.plan, ctypes.c_void_p(b._S.data_ptr()), comp, axis, index,
+                                         ctypes.c_void_p(acc_b.data_ptr()), st), "fdtd_plane_sum")
+        Sk = fetch(b, "S").reshape(3, *shape)[comp]
+        values.extend(numpy.take(Sk, index, axis=axis).astype(numpy.float64).ravel().tolist())
+        ref_E.append(b.E.clone())
+        ref_H.append(b.H.clone())
+    torch.cuda.synchronize()
+    exact = math.fsum(values)
+    assert exact != 0.0
+    for got in (float(acc_a.item()), float(acc_b.item())):
+        assert abs(got - exact) <= 1e-13 * sum(abs(v) for v in values), (got, exact)
+    for k in range(n):
+        assert torch.equal(keep_E[k], ref_E[k]), ("E after step", k)
+        assert torch.equal(keep_H[k], ref_H[k]), ("H after step", k)
+    _assert_same(_state(a), _state(b), "observed vs single steps")
+
+
+def test_step_n_observed_rejects_bad_observations():
+    from opencl_fdtd_amd import Simulation
+    shape = (6, 7, 8)
+    eps = [numpy.ones(shape)] * 3
+    s = Simulation(eps, [])
+    acc = torch.zeros(1, dtype=torch.float64, device="cuda")
+    with pytest.raises(Exception):
+        s.step_n_observed(3, flux=(2, 3, 0, acc))                 # axis out of range
+    with pytest.raises(Exception):
+        s.step_n_observed(3, flux=(2, 1, 7, acc))                 # plane beyond sy
+    with pytest.raises(Exception):
+        s.step_n_observed(3, flux=(2, 1, 0, acc.float()))         # not FP64
+    with pytest.raises(Exception):
+        s.step_n_observed(3, keep_E=[torch.empty_like(s._E)] * 2)   # n tensors needed
+    with pytest.raises(Exception):
+        Simulation(eps, [], do_poynting=False).step_n_observed(2, flux=(2, 1, 0, acc))   # no S
+
+
+@pytest.mark.parametrize("shape, dtype", [((5, 600, 500), numpy.float32), ((4, 300, 200), numpy.float64)])
+def test_step_n_observed_flux_on_every_axis(shape, dtype):
+    """The observed flux summed over many workgroups (a 600 x 500 plane:
+    586 workgroups of 512 values) on planes normal to x, y and z, against
+    the exactly rounded sum (math.fsum) of the S values after each step."""
+    from opencl_fdtd_amd import Simulation
+    print("case observed flux", shape, dtype.__name__, flush=True)
+    rng = numpy.random.default_rng(7)
+    eps = [rng.uniform(1, 3, shape) for _ in range(3)]
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    for axis in range(3):
+        sim = Simulation(eps, [], initial_fields=init, float_type=dtype)
+        acc = torch.zeros(1, dtype=torch.float64, device="cuda")
+        index = shape[axis] // 3
+        values = []
+        for k in range(2):
+            sim.step_n_observed(1, flux=(axis, axis, index, acc))
+            S = fetch(sim, "S").reshape(3, *shape)[axis]
+            values.extend(numpy.take(S, index, axis=axis).astype(numpy.float64).ravel().tolist())
+        exact = math.fsum(values)
+        got = float(acc.item())
+        assert abs(got - exact) <= 1e-13 * sum(abs(v) for v in values), (axis, got, exact)
+        del sim
+        torch.cuda.empty_cache()
