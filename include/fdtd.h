@@ -55,7 +55,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 4
+#define FDTD_ABI_VERSION 5
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -274,6 +274,8 @@ typedef struct {
                              2: E step only (lets a caller insert its own
                              adjoint between the two, e.g. the shell net) */
     int32_t reserved;
+    int32_t grad_lo[3];   /* eps_grad is accumulated only inside the box grad_lo <= (x, y, z) < grad_hi */
+    int32_t grad_hi[3];   /* (a design region); grad_hi all zero: the whole grid */
 } fdtd_adjoint_bufs;
 
 /* Elements of one adjoint psi block (same as fdtd_psi_elems). */

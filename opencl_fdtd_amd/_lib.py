@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_FACES = 12
 
 F32, F64 = 0, 1
@@ -60,7 +60,8 @@ class AdjointBufs(ctypes.Structure):
         "psib_e", "psib_e_out", "psib_h", "psib_h_out", "coef", "eps_grad")] + [
         ("src_index", ctypes.c_int64), ("src_amp", ctypes.c_double),
         ("obj_axis", ctypes.c_int32), ("obj_index", ctypes.c_int32),
-        ("parts", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+        ("parts", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("grad_lo", ctypes.c_int32 * 3), ("grad_hi", ctypes.c_int32 * 3)]
 
 
 class Observe(ctypes.Structure):

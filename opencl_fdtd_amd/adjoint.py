@@ -81,7 +81,10 @@ class FluxAdjoint:
         return float(acc.item()), cps
 
     def gradient(self) -> Tuple[float, torch.Tensor]:
-        """J and dJ/d eps (3, sx, sy, sz) in FP64."""
+        """J and dJ/d eps (3, sx, sy, sz) in FP64: inside the problem's
+        design region when it has one (zero outside, where the adjoint step
+        then reads neither the forward E nor the gradient), else
+        everywhere."""
         J, cps = self.forward()
         sim = self.sim
         N3 = 3 * sim.N
@@ -101,6 +104,10 @@ class FluxAdjoint:
         b.eps_grad = grad.data_ptr()
         b.obj_axis = self.p.axis
         b.obj_index = self.p.index
+        if self.p.design is not None:
+            for k, sl in enumerate(self.p.design):
+                lo, hi, _ = sl.indices(sim.shape[k])
+                b.grad_lo[k], b.grad_hi[k] = lo, hi
         cur = 0
         for s in reversed(range(len(cps))):
             n0 = s * K

@@ -36,6 +36,7 @@ struct AdjArgs {
     int src_x, src_p, src_comp;
     T src_amp;
     int obj_axis, obj_index, obj_mode;
+    int glo[3], ghi[3];  // the box of cells whose eps gradient is accumulated
 };
 
 struct SlabHit {
@@ -122,13 +123,17 @@ __global__ __launch_bounds__(kBlock) void adj_e(AdjArgs<T> q) {
     q.Hb[1][i] += hy;
     q.Hb[2][i] += hz;
     // d kappa_a / d eps_a = -kappa_a / eps_a; the bracket is (E' - E) / kappa
+    // (only in the gradient's box: outside it E and the gradient are not read)
+    if (!q.grad[0] || x < q.glo[0] || x >= q.ghi[0] || y < q.glo[1] || y >= q.ghi[1] || z < q.glo[2] ||
+        z >= q.ghi[2])
+        return;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         double en = (double)q.En[a][i];
         if (a == q.src_comp && x == q.src_x && p == q.src_p) en -= (double)q.src_amp;
         const double de = en - (double)q.Ep[a][i];
         const double ep = (double)q.eps[a][i];
-        if (q.grad[a] && isfinite(ep)) q.grad[a][i] += -(double)q.Eb[a][i] * de / ep;
+        if (isfinite(ep)) q.grad[a][i] += -(double)q.Eb[a][i] * de / ep;
     }
 }
 
@@ -353,6 +358,12 @@ int launch_adjoint(const Plan& P, const fdtd_adjoint_bufs* b, hipStream_t st) {
     comp3<T>(b->Eb, N, q.Eb);
     comp3<T>(b->Hb, N, q.Hb);
     for (int k = 0; k < 3; ++k) q.grad[k] = b->eps_grad ? b->eps_grad + k * N : nullptr;
+    const bool whole = b->grad_hi[0] == 0 && b->grad_hi[1] == 0 && b->grad_hi[2] == 0;
+    const int64_t s[3] = {P.sx, P.sy, P.sz};
+    for (int k = 0; k < 3; ++k) {
+        q.glo[k] = whole ? 0 : b->grad_lo[k];
+        q.ghi[k] = whole ? (int)s[k] : b->grad_hi[k];
+    }
     q.src_comp = -1;
     if (b->src_index >= 0) {
         const int64_t comp = b->src_index / N;
@@ -427,6 +438,13 @@ int fdtd_adjoint_step(const void* plan, const fdtd_adjoint_bufs* b, void* stream
             return fail(3, "adjoint: obj_index %d out of range", (int)b->obj_index);
     }
     if (b->src_index >= 3 * P.N) return fail(3, "adjoint: src_index out of range");
+    if (b->grad_hi[0] != 0 || b->grad_hi[1] != 0 || b->grad_hi[2] != 0) {
+        const int64_t s[3] = {P.sx, P.sy, P.sz};
+        for (int k = 0; k < 3; ++k)
+            if (b->grad_lo[k] < 0 || b->grad_lo[k] >= b->grad_hi[k] || b->grad_hi[k] > s[k])
+                return fail(3, "adjoint: gradient box [%d, %d) on axis %d not within [0, %lld)", b->grad_lo[k],
+                            b->grad_hi[k], k, (long long)s[k]);
+    }
     hipStream_t st = (hipStream_t)stream;
     if (P.d.dtype == FDTD_F32) return launch_adjoint<float>(P, b, st);
     return launch_adjoint<double>(P, b, st);

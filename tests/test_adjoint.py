@@ -116,6 +116,25 @@ def test_fp32_adjoint_vs_central_differences():
         assert abs(g[c] - fd["f64"]) <= 1e-2 * numpy.abs(numpy.load(os.path.join(HERE, "adjoint_fd_f64.npz"))["grad"]).max()
 
 
+def test_gradient_box_equals_whole_grid_inside():
+    """With a design region the adjoint accumulates the eps gradient only
+    there: bit-identical to the whole-grid gradient inside, zero outside."""
+    torch = _torch()
+    import dataclasses
+    from opencl_fdtd_amd.adjoint import FluxAdjoint
+    pr, p, eps = _oracle_problem_gpu(numpy.float32)
+    J1, boxed = FluxAdjoint(p, eps).gradient()
+    J2, whole = FluxAdjoint(dataclasses.replace(p, design=None), eps).gradient()
+    assert J1 == J2
+    sel = (slice(None),) + tuple(pr["design"])
+    assert torch.equal(boxed[sel], whole[sel])
+    assert float(whole[sel].abs().max()) > 0
+    inside = torch.zeros_like(boxed, dtype=torch.bool)
+    inside[sel] = True
+    assert float(boxed[~inside].abs().max()) == 0.0
+    assert float(whole[~inside].abs().max()) > 0
+
+
 def test_autograd_function():
     torch = _torch()
     from opencl_fdtd_amd.adjoint import PoyntingFluxObjective

@@ -35,7 +35,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.Face) == 16
     assert ctypes.sizeof(_lib.Desc) == 24 + 8 + 12 * 16 + 16 + 16 + 48
     assert ctypes.sizeof(_lib.Fields) == 20 * 8
-    assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4
+    assert ctypes.sizeof(_lib.AdjointBufs) == 13 * 8 + 8 + 8 + 4 * 4 + 6 * 4
 
 
 def _desc(shape=(10, 11, 12), faces=(), **kw):
@@ -301,3 +301,26 @@ def test_update_he2_rejects_bad_calls():
     rc, msg = call(e_out_same=True)
     assert rc != 0 and "fdtd_update_he2 needs E_out" in msg
     assert lib.fdtd_update_he2(None, None, -1, None, None) != 0
+
+
+def test_adjoint_gradient_box_is_validated():
+    """fdtd_adjoint_step rejects a gradient box that is empty or leaves the
+    grid before launching anything (fake device pointers)."""
+    from opencl_fdtd_amd import _lib
+    lib = _lib.load()
+    d = _desc((10, 11, 12), [(0, 0, 2)], poynting=1)
+    plan = ctypes.c_void_p()
+    assert lib.fdtd_plan_create(ctypes.byref(d), ctypes.byref(plan)) == 0
+    try:
+        for lo, hi in (((0, 0, 0), (11, 11, 12)), ((2, 3, 4), (2, 5, 6)), ((-1, 0, 0), (4, 4, 4)),
+                       ((0, 0, 0), (4, 12, 4))):
+            b = _lib.AdjointBufs()
+            for name in ("E_prev", "E_next", "H_prev", "H_next", "eps", "Eb", "Hb", "psib_e", "psib_h", "coef"):
+                setattr(b, name, 0x1000)
+            b.psib_e_out, b.psib_h_out, b.eps_grad = 0x2000, 0x3000, 0x4000
+            b.src_index, b.obj_axis = -1, -1
+            b.grad_lo[:], b.grad_hi[:] = lo, hi
+            assert lib.fdtd_adjoint_step(plan, ctypes.byref(b), None) != 0, (lo, hi)
+            assert "gradient box" in lib.fdtd_last_error().decode()
+    finally:
+        lib.fdtd_plan_destroy(plan)
