@@ -294,6 +294,41 @@ def learned_summary(steps: int = 100, warmup: int = 20, seeds=(1000, 1001, 1002,
     return out
 
 
+def adjoint_report(repeats: int = 2):
+    """The adjoint configuration (128^3, CPML-8, 2000 steps, checkpoints
+    every 100, flux through z = 100, a 32^3 design region; FP32 fields):
+    wall time of the objective alone and of the objective with its eps
+    gradient (forward with checkpoints, segment recompute, reverse steps),
+    best of ``repeats`` after one warm-up run.  One JSON line."""
+    import torch
+    from opencl_fdtd_amd import workloads
+    from opencl_fdtd_amd.adjoint import FluxAdjoint, FluxProblem, flux
+    w = workloads.adjoint()
+    eps = torch.as_tensor(numpy.array(w.epsilon), device="cuda")
+    c = w.shape[0] // 2
+    design = (slice(c - 16, c + 16), slice(c - 16, c + 16), slice(40, 72))
+    p = FluxProblem(shape=w.shape, pmls=w.pmls, steps=w.steps, src_index=w.src_index, src_wave=w.src_wave,
+                    axis=2, index=100, dt=w.dt, checkpoint_every=100, float_type=numpy.float32, design=design)
+    flux(p, eps)
+    FluxAdjoint(p, eps).gradient()
+    torch.cuda.synchronize()
+    t_f, t_g = [], []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        J = flux(p, eps)
+        torch.cuda.synchronize()
+        t_f.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        J2, g = FluxAdjoint(p, eps).gradient()
+        torch.cuda.synchronize()
+        t_g.append(time.perf_counter() - t)
+    out = {"workload": w.name, "shape": list(w.shape), "steps": w.steps, "checkpoint_every": 100,
+           "objective_s": min(t_f), "objective_us_per_step": min(t_f) / w.steps * 1e6,
+           "gradient_s": min(t_g), "gradient_us_per_step": min(t_g) / w.steps * 1e6,
+           "J": J, "J_with_gradient": J2, "timing": "host wall clock, synchronised, FP32 fields"}
+    print(json.dumps(out), flush=True)
+
+
 def configs_report(seconds: float = 1.0):
     """Throughput of every workload (C1 small 64^3, C2 README slab, C3 Bloch
     supercell, C4 the adjoint's 128^3 forward grid, a synthetic 512^3 grid
@@ -368,6 +403,8 @@ def main():
                     help="with --learned: Poynting on, so every run uses the fused H+E pass")
     ap.add_argument("--all-configs", action="store_true",
                     help="throughput of every workload (one JSON line each) instead of the benchmark")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the adjoint configuration's objective and gradient instead of the benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X; "
@@ -382,6 +419,9 @@ def main():
         return
     if args.all_configs:
         configs_report()
+        return
+    if args.adjoint:
+        adjoint_report()
         return
 
     import torch
