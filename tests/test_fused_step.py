@@ -546,3 +546,37 @@ def test_step_n_observed_flux_on_every_axis(shape, dtype):
         assert abs(got - exact) <= 1e-13 * sum(abs(v) for v in values), (axis, got, exact)
         del sim
         torch.cuda.empty_cache()
+
+
+def test_random_shapes_fused_equals_unfused():
+    """A seeded sweep of 24 grids (rows of 1 to 1500 cells, 1 to 40 rows,
+    2 to 9 planes; CPML thickness 0 to 3): step_n through whichever passes
+    the plan picks (whole-row or z-segment tiles, one or two steps per pass)
+    against the unfused kernels, and update_HE2 against update_HE twice
+    where the two-step pass is supported, bit for bit."""
+    rng = numpy.random.default_rng(2024)
+    for case in range(24):
+        shape = (int(rng.integers(2, 10)), int(rng.integers(1, 41)),
+                 int(rng.choice([int(rng.integers(1, 80)), int(rng.integers(80, 600)), int(rng.integers(600, 1500))])))
+        T = int(rng.integers(0, 4))
+        T = min(T, min(shape) // 2)
+        n = int(rng.integers(2, 7))
+        print("case random", case, shape, T, n, flush=True)
+        eps, pmls, init, _ = _case(shape, T, seed=100 + case, pec=bool(case % 3 == 0))
+        a = _sim(eps, pmls, init, None, numpy.float32, True)
+        b = _sim(eps, pmls, init, None, numpy.float32, False)
+        idx = (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
+        wave = numpy.cos(0.3 * numpy.arange(n)).astype(numpy.float32)
+        a.step_n(n, idx, wave)
+        b.step_n(n, idx, wave)
+        torch.cuda.synchronize()
+        _assert_same(_state(a), _state(b), "random %s" % (shape,))
+        if a._psi_e_out is not None:
+            c = _sim(eps, pmls, init, None, numpy.float32, True)
+            d = _sim(eps, pmls, init, None, numpy.float32, True)
+            amps = torch.tensor([0.5, -0.25], dtype=torch.float32, device="cuda")
+            c.update_HE2([], idx, amps)
+            d.update_HE([], idx, amps[0:1])
+            d.update_HE([], idx, amps[1:2])
+            torch.cuda.synchronize()
+            _assert_same(_state(c), _state(d), "random HE2 %s" % (shape,))
