@@ -245,3 +245,49 @@ def test_sources_and_saved_fields(tmp_path):
     assert {"F", "G"} <= set(simb.sources)
     d = fieldio.field_dict(simb)
     assert set(d) == {"E", "H", "S", "F", "G"}
+
+
+def test_dt_rules_and_blow_up_detection():
+    """The reference's checks (simulation.py:133-153, 341-346): dt <= 0 and
+    epsilon grids of different shapes raise; a dt above 0.99 min(dx)/sqrt(3)
+    warns and is used.  Such a run grows without bound, which the FP64
+    energy reduction (fdtd_energy) shows as a non-finite or exploding value,
+    while the stable dt keeps that energy constant in a lossless periodic
+    box (the quantity the leapfrog step conserves)."""
+    import ctypes
+    import math
+    import warnings
+    import torch
+    from opencl_fdtd_amd import Simulation
+    shape = (12, 12, 12)
+    rng = numpy.random.default_rng(5)
+    eps = [numpy.ones(shape)] * 3
+    init = {"E": random_fields(rng, shape), "H": random_fields(rng, shape)}
+    with pytest.raises(Exception, match="Invalid dt"):
+        Simulation(eps, [], dt=-0.1)
+    with pytest.raises(Exception, match="same shape"):
+        Simulation([numpy.ones(shape), numpy.ones(shape), numpy.ones((12, 12, 11))], [])
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def energies(sim, steps):
+        parts = torch.zeros(int(sim._lib.fdtd_reduce_partials(sim.plan)), dtype=torch.float64, device="cuda")
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        vals = []
+        for _ in range(steps):
+            hp = sim.H.clone()
+            sim.step_n(1)
+            assert sim._lib.fdtd_energy(sim.plan, ctypes.c_void_p(sim._E.data_ptr()),
+                                        ctypes.c_void_p(sim.H.data_ptr()), ctypes.c_void_p(hp.data_ptr()),
+                                        ctypes.c_void_p(sim.eps.data_ptr()), ctypes.c_void_p(parts.data_ptr()),
+                                        ctypes.c_void_p(out.data_ptr()), st) == 0
+            vals.append(float(out.item()))
+        return vals
+
+    stable = energies(Simulation(eps, [], initial_fields=init), 100)
+    assert all(abs(v - stable[0]) <= 1e-4 * abs(stable[0]) for v in stable), (stable[0], stable[-1])
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        fast = Simulation(eps, [], initial_fields=init, dt=1.2 * 0.99 / math.sqrt(3))
+    assert any("unstable dt" in str(w.message) for w in caught)
+    grown = energies(fast, 100)
+    assert not math.isfinite(grown[-1]) or abs(grown[-1]) > 1e10 * abs(grown[0]), (grown[0], grown[-1])
