@@ -2100,9 +2100,11 @@ __global__ void add_point(T* f, const T* amp) {
 // Host side
 // ---------------------------------------------------------------------------
 
-// A marching launch's tiles of the (y, z) plane (TileShape in Geom): whole
-// rows (tzb = 1, tzo = tw = sz), or rows cut into z segments of tzo owned
-// cells, each workgroup row then tw = tzo + halo columns wide.
+// A marching launch: block threads per workgroup, pblocks tiles of the
+// plane times nchunk x chunks of xchunk planes.  tile: the plane cells a
+// workgroup owns.  he_march / he2_march tiles (tyo, tzo, tzb, tw, as in
+// Geom): whole rows (tzb = 1, tzo = tw = sz), or rows cut into z segments
+// of tzo owned cells, each workgroup row then tw = tzo + halo columns wide.
 struct Launch {
     int block, tile, pblocks, nchunk, xchunk, mode;
     int tyo = 0, tzo = 0, tzb = 0, tw = 0;
@@ -2140,13 +2142,12 @@ Launch make_launch(int plane, int sx, int block, int tile, int mode, int target)
     return L;
 }
 
-// Marching launch with one workgroup per CU at a time (1024-thread tiles of
-// whole rows plus a halo row, one plane of extra "dry" work per x chunk):
-// the number of x chunks per tile column that minimises rounds x (planes
-// per chunk + 1), the time model of such a launch (each round runs its
-// workgroups side by side; a chunk takes a time proportional to its planes).
-// The chunks per tile column that minimise that cost for pblocks tiles per
-// plane; returns the cost, the chunk count in *best_n.
+// The time model of a marching launch with one workgroup per CU at a time:
+// each round runs up to max_blocks workgroups side by side, and a chunk
+// takes a time proportional to its planes, dry ones included (extra).  For
+// pblocks tiles per plane, the number of x chunks per tile column that
+// minimises rounds x (planes per chunk + extra): returns that cost, the
+// chunk count in *best_n.
 int64_t round_chunks(int pblocks, int sx, int max_blocks, int extra, int* best_n) {
     int best = 1;
     int64_t best_cost = -1;
@@ -2166,6 +2167,8 @@ int64_t round_chunks(int pblocks, int sx, int max_blocks, int extra, int* best_n
     return best_cost;
 }
 
+// Marching launch of tiles of `tile` contiguous plane cells (whole rows
+// plus a halo row for the fused-S H kernel), chunked by that model.
 Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks, int extra = 1) {
     const int pblocks = (plane + tile - 1) / tile;
     int best = 1;
@@ -2178,8 +2181,9 @@ Launch make_round_launch(int plane, int sx, int block, int tile, int max_blocks,
 // segments, zh halo columns besides the owned cells.  Candidates: whole
 // rows where 1 + yh of them fit the workgroup (z periodic inside a row),
 // and unless whole_only, z segments of FDTD_SEG_MIN (64) up to 256 owned
-// cells, powers of two, on longer rows.  Each is chunked along x as make_round_launch does; the one
-// with the fewest rounds x planes wins, whole rows on a tie.  Cutting rows
+// cells, powers of two, on longer rows.  Each is chunked along x by
+// round_chunks; the one with the fewest rounds x planes wins, whole rows
+// on a tie.  Cutting rows
 // keeps long rows from leaving a tile mostly halo (rows of 512 cells: one
 // owned row of two in a 1024-thread workgroup).  mode -1: no shape fits.
 Launch make_tile_launch(int sx, int sy, int sz, int block, int yh, int zh, int max_blocks, int extra,
@@ -2799,9 +2803,6 @@ int fdtd_add_point(const void* plan, void* field, int64_t index, const void* amp
 
 namespace {
 
-// n steps of the caller loop.  home: E and psi_e end in f->E and f->psi_e
-// (pass counts kept even); otherwise they may end in f->E_out / f->psi_e_out
-// and *moved says where (bit 0: E, bit 1: psi_e).
 int plane_sum_many(const Plan& P, const void* field, int comp, int axis, int64_t index, double* partials,
                    double* acc, hipStream_t st);   // adjoint.inl
 
@@ -2818,6 +2819,10 @@ int copy_field(void* dst, const void* src, size_t bytes, hipStream_t st) {
     return 0;
 }
 
+// n steps of the caller loop.  home: E and psi_e end in f->E and f->psi_e
+// (pass counts kept even); otherwise they may end in f->E_out / f->psi_e_out
+// and *moved says where (bit 0: E, bit 1: psi_e).  obs: the observations of
+// fdtd_step_n_observed after every step (one step per pass then).
 int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
                 void* stream, bool home, int32_t* moved, const fdtd_observe* obs = nullptr) {
     if (!plan) return fail(1, "plan is NULL");
