@@ -1327,7 +1327,10 @@ struct HEArgs {
 
 // SHELL: add the pending learned-shell correction to E^n before the E
 // update (the H update and S use E^n as it is).
-template <class T, int HB, bool NU, bool CT, bool SHELL, bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
+// POY: fused Poynting (S from the oS products of H^{n+1/2}); without it the
+// pass skips the products and S (do_poynting off).
+template <class T, int HB, bool NU, bool CT, bool SHELL, bool POY = true,
+          bool PF = (FDTD_HE_PF != 0 && sizeof(T) == 4)>
 __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     __shared__ T lds[2][7][HB];
     __shared__ T ctab[CT ? kCoefTab : 1];
@@ -1473,20 +1476,24 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
         if (valid) {
             const HCell<T> h = h_update<T, false, true>(g, hs, c, x, pbytes, qx, qy, qz, cur.h, ix, iy, iz,
                                                         own && !dry, kcpml, kh, &cur.qh, tab);
-            const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
-            oSxy = h.aEyx * aHz;
-            oSxz = -h.aEzx * aHy;
-            oSyz = h.aEzy * aHx;
-            oSyx = -h.aExy * aHz;
-            oSzx = h.aExz * aHy;
-            oSzy = -h.aEyz * aHx;
+            if constexpr (POY) {
+                const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+                oSxy = h.aEyx * aHz;
+                oSxz = -h.aEzx * aHy;
+                oSyz = h.aEzy * aHx;
+                oSyx = -h.aExy * aHz;
+                oSzx = h.aExz * aHy;
+                oSzy = -h.aEyz * aHx;
+            }
             hx1 = h.hx1; hy1 = h.hy1; hz1 = h.hz1;
         }
         const int b = it & 1;
-        lds[b][0][t] = oSxy;
-        lds[b][1][t] = oSzy;
-        lds[b][2][t] = oSxz;
-        lds[b][3][t] = oSyz;
+        if constexpr (POY) {
+            lds[b][0][t] = oSxy;
+            lds[b][1][t] = oSzy;
+            lds[b][2][t] = oSxz;
+            lds[b][3][t] = oSyz;
+        }
         lds[b][4][t] = hx1;
         lds[b][5][t] = hy1;
         lds[b][6][t] = hz1;
@@ -1497,11 +1504,13 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
             // row wraps inside the row
             const int ty = t - g.tw;
             const int tz = (g.tzb > 1 || c.z > 0) ? t - 1 : t + g.tw - 1;
-            const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
-            const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
-            stc<FDTD_NT_H>(a.S, 0, oi, (oSxy + oSxz + nxy + nxz) * sf);
-            stc<FDTD_NT_H>(a.S, 1, oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
-            stc<FDTD_NT_H>(a.S, 2, oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            if constexpr (POY) {
+                const T nxy = lds[b][0][ty], nzy = lds[b][1][ty];
+                const T nxz = lds[b][2][tz], nyz = lds[b][3][tz];
+                stc<FDTD_NT_H>(a.S, 0, oi, (oSxy + oSxz + nxy + nxz) * sf);
+                stc<FDTD_NT_H>(a.S, 1, oi, (oSyz + oSyx + nyz + oSyx_m) * sf);
+                stc<FDTD_NT_H>(a.S, 2, oi, (oSzx + oSzy + oSzx_m + nzy) * sf);
+            }
             EIn<T> v;
             v.hx = hx1; v.hy = hy1; v.hz = hz1;
             v.hx_ym = lds[b][4][ty];
@@ -1775,9 +1784,12 @@ struct HE2Args {
 // plane parity) instead of being carried in registers (fewer spills).
 // EARLY: the loads of plane x+1 are issued after the second barrier, so
 // they are in flight while the second stage finishes plane x-1.
+// POY: fused Poynting (S of the second H half-step); without it (and then
+// without S2) the pass skips the oS products and S.
 template <class T, bool NU, bool S2, bool E2 = true, bool LC = (FDTD_HE2_LDSCARRY != 0),
-          bool EARLY = (FDTD_HE2_EARLY != 0)>
+          bool EARLY = (FDTD_HE2_EARLY != 0), bool POY = true>
 __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 3) void he2_march(HE2Args<T> a) {
+    static_assert(POY || !S2, "the first half-step's S needs Poynting");
     constexpr int HB = FDTD_HE2_HB;
     __shared__ T la[LC ? 2 : 1][S2 ? 7 : 3][HB];   // H^{n+1/2} (x, y, z), with S2 also its oS products
     __shared__ T lp[LC ? 2 : 1][3][LC ? HB : 1];   // dt / eps
@@ -1975,19 +1987,23 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
             const T ixb = NU ? g.inv_e[0][xb] : g.inv_u;
             const HCell<T> h = h_update<T, false, true>(g, hB, c, xb, pbytes, f1x, f1y, f1z, v, ixb, iy, iz,
                                                         own && wb, (const CpmlCols<T>*)&kb, nullptr, &kh, tab);
-            const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
-            oBxy = h.aEyx * aHz;
-            oBxz = -h.aEzx * aHy;
-            oByz = h.aEzy * aHx;
-            oByx = -h.aExy * aHz;
-            oBzx = h.aExz * aHy;
-            oBzy = -h.aEyz * aHx;
+            if constexpr (POY) {
+                const T aHx = h.hx1 + h.hx0, aHy = h.hy1 + h.hy0, aHz = h.hz1 + h.hz0;
+                oBxy = h.aEyx * aHz;
+                oBxz = -h.aEzx * aHy;
+                oByz = h.aEzy * aHx;
+                oByx = -h.aExy * aHz;
+                oBzx = h.aExz * aHy;
+                oBzy = -h.aEyz * aHx;
+            }
             g2x = h.hx1; g2y = h.hy1; g2z = h.hz1;
         }
-        lb[0][t] = oBxy;
-        lb[1][t] = oBzy;
-        lb[2][t] = oBxz;
-        lb[3][t] = oByz;
+        if constexpr (POY) {
+            lb[0][t] = oBxy;
+            lb[1][t] = oBzy;
+            lb[2][t] = oBxz;
+            lb[3][t] = oByz;
+        }
         lb[4][t] = g2x;
         lb[5][t] = g2y;
         lb[6][t] = g2z;
@@ -1995,7 +2011,7 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
         Loads nxt;
         if (EARLY && it + 1 < n_it) fetch(xn, true, nxt);
         // ---- second stage: S(xb) and E^{n+2}(xb) on the owned rows
-        if (own && wb) {
+        if (POY && own && wb) {
             const uint32_t oi = (uint32_t)xb * pbytes + c.op;
             stc<FDTD_NT_H>(a.S, 0, oi, (oBxy + oBxz + lb[0][tym] + lb[2][tzm]) * sf);
             stc<FDTD_NT_H>(a.S, 1, oi, (oByz + oByx + lb[3][tzm] + oByx_m) * sf);
@@ -2393,11 +2409,19 @@ int launch_h(const Plan& P, const fdtd_fields* f, hipStream_t st) {
     return check_launch("fdtd_update_h");
 }
 
-// (lhe is set up only where the pass applies: fused Poynting, 3N values
-// within 32-bit byte offsets, and, with Bloch boundaries, rows of at most
-// FDTD_ROW_MAX cells, FP32, uniform dx and the coefficient block in LDS)
+// (lhe is set up only where the pass applies: fused Poynting or Poynting
+// off, 3N values within 32-bit byte offsets, and, with Bloch boundaries,
+// fused Poynting, rows of at most FDTD_ROW_MAX cells, FP32, uniform dx and
+// the coefficient block in LDS)
 bool he_supported(const Plan& P) {
-    return P.d.poynting == FDTD_POYNTING_FUSED && P.lhe.mode == 2;
+    return P.d.poynting != FDTD_POYNTING_TWO_PASS && P.lhe.mode == 2;
+}
+
+// The second buffers the fused passes read from and write to: E_out, H_out
+// and (with CPML) psi_h_out.
+bool he_buffers(const Plan& P, const fdtd_fields* f) {
+    return f->E_out && f->E_out != f->E && f->H_out && f->H_out != f->H &&
+           (P.d.n_face == 0 || (f->psi_h_out && f->psi_h_out != f->psi_h));
 }
 
 int launch_heb(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index) {
@@ -2490,12 +2514,18 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
     const dim3 grid(L.pblocks * L.nchunk);
     const bool nu = P.d.nonuniform != 0;
     const bool ct = FDTD_HE_CTAB && P.coef_elems <= kCoefTab;
-#define FDTD_LAUNCH_HE2(HB_, SH_)                                                                     \
-    do {                                                                                              \
-        if (nu && ct) hipLaunchKernelGGL((he_march<T, HB_, true, true, SH_>), grid, dim3(HB_), 0, st, a);    \
-        else if (nu) hipLaunchKernelGGL((he_march<T, HB_, true, false, SH_>), grid, dim3(HB_), 0, st, a);    \
-        else if (ct) hipLaunchKernelGGL((he_march<T, HB_, false, true, SH_>), grid, dim3(HB_), 0, st, a);    \
-        else hipLaunchKernelGGL((he_march<T, HB_, false, false, SH_>), grid, dim3(HB_), 0, st, a);           \
+    const bool poy = P.d.poynting == FDTD_POYNTING_FUSED;
+#define FDTD_LAUNCH_HE3(HB_, SH_, POY_)                                                                      \
+    do {                                                                                                     \
+        if (nu && ct) hipLaunchKernelGGL((he_march<T, HB_, true, true, SH_, POY_>), grid, dim3(HB_), 0, st, a); \
+        else if (nu) hipLaunchKernelGGL((he_march<T, HB_, true, false, SH_, POY_>), grid, dim3(HB_), 0, st, a); \
+        else if (ct) hipLaunchKernelGGL((he_march<T, HB_, false, true, SH_, POY_>), grid, dim3(HB_), 0, st, a); \
+        else hipLaunchKernelGGL((he_march<T, HB_, false, false, SH_, POY_>), grid, dim3(HB_), 0, st, a);        \
+    } while (0)
+#define FDTD_LAUNCH_HE2(HB_, SH_)                      \
+    do {                                               \
+        if (poy) FDTD_LAUNCH_HE3(HB_, SH_, true);      \
+        else FDTD_LAUNCH_HE3(HB_, SH_, false);         \
     } while (0)
 #define FDTD_LAUNCH_HE(HB_)                      \
     do {                                         \
@@ -2506,14 +2536,19 @@ int launch_he(const Plan& P, const fdtd_fields* f, hipStream_t st, const T* src_
     else FDTD_LAUNCH_HE(1024);
 #undef FDTD_LAUNCH_HE
 #undef FDTD_LAUNCH_HE2
+#undef FDTD_LAUNCH_HE3
     return check_launch("fdtd_update_he");
 }
 
 int check_he_fields(const Plan& P, const fdtd_fields* f) {
     if (!he_supported(P))
-        return fail(3, "fdtd_update_he needs fused Poynting and, with Bloch boundaries, rows of at most %d cells, "
-                       "FP32, uniform dx and at most %d CPML coefficients", FDTD_ROW_MAX, kCoefTab);
+        return fail(3, "fdtd_update_he needs fused Poynting or Poynting off and, with Bloch boundaries, fused "
+                       "Poynting, rows of at most %d cells, FP32, uniform dx and at most %d CPML coefficients",
+                    FDTD_ROW_MAX, kCoefTab);
     if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he needs E_out != E");
+    if (!f->H_out || f->H_out == f->H) return fail(2, "fdtd_update_he needs H_out != H");
+    if (P.d.n_face > 0 && (!f->psi_h_out || f->psi_h_out == f->psi_h))
+        return fail(2, "fdtd_update_he needs psi_h_out != psi_h");
     if (P.d.bloch_mask) {
         if (!f->F_out || f->F_out == f->F) return fail(2, "fdtd_update_he with Bloch boundaries needs F_out != F");
         if (!f->G_out || f->G_out == f->G) return fail(2, "fdtd_update_he with Bloch boundaries needs G_out != G");
@@ -2556,17 +2591,26 @@ int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
     }
     const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
     const bool nu = P.d.nonuniform != 0;
-    if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-    else if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-    else if (last_e) hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-    else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    if (P.d.poynting == FDTD_POYNTING_FUSED) {
+        if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    } else {
+        // Poynting off: no S (and no S of the first half-step)
+        constexpr bool LC = FDTD_HE2_LDSCARRY != 0, EA = FDTD_HE2_EARLY != 0;
+        if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, false, true, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (nu) hipLaunchKernelGGL((he2_march<T, true, false, false, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+    }
     return check_launch("fdtd_update_he2");
 }
 
 int check_he2_fields(const Plan& P, const fdtd_fields* f) {
     if (!he2_supported(P))
-        return fail(3, "fdtd_update_he2 needs FP32, fused Poynting, no Bloch boundaries, rows of at most 256 cells, "
-                       "at most %d CPML coefficients and no two overlapping slabs on one axis", kCoefTab);
+        return fail(3, "fdtd_update_he2 needs FP32, fused Poynting or Poynting off, no Bloch boundaries, at most %d "
+                       "CPML coefficients and no two overlapping slabs on one axis", kCoefTab);
     if (!f->E_out || f->E_out == f->E) return fail(2, "fdtd_update_he2 needs E_out != E");
     if (P.d.n_face > 0 && (!f->psi_e_out || f->psi_e_out == f->psi_e))
         return fail(2, "fdtd_update_he2 needs psi_e_out != psi_e");
@@ -2675,8 +2719,8 @@ int fdtd_plan_create(const fdtd_desc* desc, void** plan_out) {
     // fused H+E pass: the same tile shape (whole rows plus one halo row)
     P->lhe.mode = -1;
     const bool pair_ok = d.bloch_mask == 0 || (d.dtype == FDTD_F32 && !d.nonuniform && P->coef_elems <= kCoefTab);
-    if (d.poynting == FDTD_POYNTING_FUSED && pair_ok && (d.bloch_mask == 0 || P->sz <= FDTD_ROW_MAX) &&
-        3 * N * esz <= (int64_t)UINT32_MAX) {
+    const bool fused_s = d.poynting == FDTD_POYNTING_FUSED || (d.poynting == FDTD_POYNTING_OFF && !d.bloch_mask);
+    if (fused_s && pair_ok && (d.bloch_mask == 0 || P->sz <= FDTD_ROW_MAX) && 3 * N * esz <= (int64_t)UINT32_MAX) {
         // one workgroup per CU at a time (1024 threads, <= 128 VGPRs); one
         // halo row (and column); the Bloch-pair pass takes whole rows only
         const int hb = P->sz <= 128 ? FDTD_HE_HB_SMALL : 1024;
@@ -2892,7 +2936,7 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     const bool pair_bufs = !P.d.bloch_mask ||
                            (f->F_out && f->F_out != f->F && f->G_out && f->G_out != f->G &&
                             (P.d.n_face == 0 || (f->psi_g_out && f->psi_g_out != f->psi_g)));
-    const bool fuse = n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P) && pair_bufs;
+    const bool fuse = n >= 2 && he_buffers(P, f) && he_supported(P) && pair_bufs;
     const bool fuse2 = fuse && !obs && he2_supported(P) && P.he2_pref &&
                        (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
@@ -3084,7 +3128,9 @@ std::string describe(const Plan& P, int which) {
             break;
         default:
             if (he_supported(P)) {
-                out += "he_march<" + std::string(ty) + ">: H+S then E in one pass; " + launch_text(P.lhe) + "\n";
+                out += "he_march<" + std::string(ty) + ">: " +
+                       (d.poynting == FDTD_POYNTING_OFF ? "H then E" : "H+S then E") + " in one pass; " +
+                       launch_text(P.lhe) + "\n";
                 if (he2_supported(P)) {
                     out += "he2_march<float>: two steps per pass; " + launch_text(P.lhe2) +
                            (P.he2_pref ? " (chosen by the time model)\n" : " (not chosen: the one-step pass is faster)\n");

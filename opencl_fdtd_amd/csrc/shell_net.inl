@@ -911,7 +911,7 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
     // correction of step k-1) and the correction of step k, then H_{n-1}.
     // Same order of operations per cell as below; the count of passes is
     // kept even so that E ends in f->E.
-    if (n >= 2 && f->E_out && f->E_out != f->E && swap && he_supported(P)) {
+    if (n >= 2 && he_buffers(P, f) && he_supported(P)) {
         if (int rc = half_e(0)) return rc;
         launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
         int k = 1;

@@ -24,9 +24,10 @@ Differences from the reference, all documented behaviour:
   the seam; ``'decoupled'`` reproduces the reference as shipped.
 * ``step_n`` (n steps without host round trips) runs each H half-step and
   the next E half-step as one pass over the grid when the plan allows it
-  (fused Poynting; with Bloch boundaries FP32, uniform dx and rows of at
-  most 512 cells, and then the pass advances E/H and F/G together;
-  ``fuse_steps=False`` turns it off and saves the second E buffer it needs),
+  (fused Poynting, or Poynting off without Bloch boundaries; with Bloch
+  boundaries FP32, uniform dx and rows of at most 512 cells, and then the
+  pass advances E/H and F/G together; ``fuse_steps=False`` turns it off and
+  saves the second E buffer it needs, and with Poynting off the second H),
   and two steps per pass where that is faster.  E may end in either of its
   two buffers: read ``sim.E`` after each call.  Results are bit-identical.
   ``update_HE`` exposes that pass once; it moves E to the other buffer.
@@ -211,7 +212,11 @@ class Simulation:
         self.do_poynting = bool(do_poynting)
         self.poynting_mode = poynting_mode if do_poynting else "off"
         self.bloch_mode = bloch_mode
-        pingpong = self.poynting_mode == "fused"
+        # H (and psi_H) in two buffers: fused S reads the old H, and the fused
+        # passes (also with Poynting off, without Bloch boundaries) read H
+        # from one buffer while writing the other
+        pingpong = self.poynting_mode == "fused" or (self.poynting_mode == "off" and bool(fuse_steps)
+                                                      and not self.bloch_boundaries)
         self._E = self._create_field(initial_fields.get("E"))
         self._H = [self._create_field(initial_fields.get("H"))]
         if pingpong:
@@ -479,8 +484,8 @@ class Simulation:
         """``update_H`` (with S) of one step and ``update_E`` of the next,
         then ``E[src_index] += src_amp[0]`` (a one-value device tensor), in
         one pass over the grid.  Needs ``fuse_steps`` and a plan that
-        supports it (fused Poynting; with Bloch boundaries FP32, uniform dx
-        and sz <= 512, F moving with E).  The
+        supports it (fused Poynting or Poynting off; with Bloch boundaries
+        fused Poynting, FP32, uniform dx and sz <= 512, F moving with E).  The
         new E is in a second buffer afterwards: read ``sim.E`` again."""
         if self._E_out is None:
             raise _lib.FdtdError("update_HE is not available for this simulation (fdtd_update_he_supported)")
@@ -502,8 +507,8 @@ class Simulation:
         """``update_HE`` twice in one pass over the grid: H (with S), E,
         ``E[src_index] += src_amp[0]``, H (with S), E, ``E[src_index] +=
         src_amp[1]`` (``src_amp``: a two-value device tensor).  Needs a plan
-        that supports it (FP32, fused Poynting, no Bloch boundaries, CPML
-        slabs of one axis apart).  The new E is in a second buffer
+        that supports it (FP32, fused Poynting or Poynting off, no Bloch
+        boundaries, CPML slabs of one axis apart).  The new E is in a second buffer
         afterwards: read ``sim.E`` again."""
         if self._psi_e_out is None:
             raise _lib.FdtdError("update_HE2 is not available for this simulation (fdtd_update_he2_supported)")

@@ -341,9 +341,12 @@ def run(lib_path: str, quick: bool = False) -> int:
                 nch = lib.fdtd_plan_describe(plan, which, None, 0)
                 buf = ctypes.create_string_buffer(nch + 1)
                 assert lib.fdtd_plan_describe(plan, which, buf, nch + 1) == nch > 0
-            pingpong = poynting == 1
             esz = 4 if dtype == 0 else 8
-            for n, with_out, homekeep in itertools.product((0, 1, 2, 3, 4, 5, 8, 11), (False, True), (False, True)):
+            # H ping-pong: always with fused S; with Poynting off both ways
+            # (in place, or two buffers for the fused passes without S)
+            for n, with_out, homekeep, pingpong in itertools.product(
+                    (0, 1, 2, 3, 4, 5, 8, 11), (False, True), (False, True),
+                    (False, True) if poynting == 0 else (poynting == 1,)):
                 f = L.Fields()
                 addrs = iter(range(0x100000, 0x10000000, 0x100000))
                 for name, _ in L.Fields._fields_:
@@ -368,7 +371,7 @@ def run(lib_path: str, quick: bool = False) -> int:
                     check_decomposition(r)
                 check_schedule(recs, f, n, None if homekeep else moved.value, wave, esz, bool(bloch), pingpong)
                 checked += 1
-            checked += _observed(lib, L, plan, shape, esz, poynting, bool(bloch), pingpong)
+            checked += _observed(lib, L, plan, shape, esz, poynting, bool(bloch), poynting in (0, 1))
         finally:
             lib.fdtd_plan_destroy(plan)
     return checked

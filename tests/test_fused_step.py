@@ -361,8 +361,11 @@ def test_fusion_is_off_where_unsupported():
     eps = [numpy.ones(shape)] * 3
     assert Simulation(eps, [])._E_out is not None
     assert Simulation(eps, [], poynting_mode="two_pass")._E_out is None
-    assert Simulation(eps, [], do_poynting=False)._E_out is None
+    assert Simulation(eps, [], do_poynting=False)._E_out is not None            # Poynting off: fused too
+    assert Simulation(eps, [], do_poynting=False, fuse_steps=False)._E_out is None
+    assert len(Simulation(eps, [], do_poynting=False, fuse_steps=False)._H) == 1   # no second H then
     bloch = [{"axis": "x", "real": 1.0, "imag": 0.0}]
+    assert Simulation(eps, [], do_poynting=False, bloch_boundaries=bloch)._E_out is None
     sb = Simulation(eps, [], bloch_boundaries=bloch)
     assert sb._E_out is not None and sb._F_out is not None and not sb.two_step_passes
     assert Simulation(eps, [], bloch_boundaries=bloch, float_type=numpy.float64)._E_out is None
@@ -580,3 +583,62 @@ def test_random_shapes_fused_equals_unfused():
             d.update_HE([], idx, amps[1:2])
             torch.cuda.synchronize()
             _assert_same(_state(c), _state(d), "random HE2 %s" % (shape,))
+
+
+@pytest.mark.parametrize("shape, T, n, dtype, nonuniform", [
+    ((11, 13, 9), 3, 5, numpy.float32, False),
+    ((30, 40, 73), 4, 6, numpy.float32, False),      # the two-step pass
+    ((3, 4, 700), 2, 4, numpy.float32, False),       # z segments
+    ((33, 8, 17), 3, 5, numpy.float64, True),
+    ((5, 6, 160), 2, 4, numpy.float32, True),
+])
+def test_poynting_off_fused_equals_unfused(shape, T, n, dtype, nonuniform):
+    """do_poynting=False: step_n through the fused passes (without S) against
+    the separate kernels, bit for bit (E, H, both psi blocks)."""
+    from opencl_fdtd_amd import Simulation
+    print("case poynting off", shape, T, n, dtype.__name__, nonuniform, flush=True)
+    eps, pmls, init, dxes = _case(shape, T, seed=sum(shape) + 3, nonuniform=nonuniform, pec=True)
+    mk = lambda fuse: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], dxes=dxes,
+                                 initial_fields=init, float_type=dtype, do_poynting=False, fuse_steps=fuse)
+    a, b = mk(True), mk(False)
+    assert a._E_out is not None and b._E_out is None and a.S is None
+    idx = 2 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 3) * shape[2] + shape[2] // 2
+    wave = numpy.sin(0.3 * numpy.arange(3 * n)).astype(dtype)
+    for k in range(3):
+        a.step_n(n, idx, wave[k * n:(k + 1) * n])
+        b.step_n(n, idx, wave[k * n:(k + 1) * n])
+    torch.cuda.synchronize()
+    for name in ("E", "H"):
+        assert numpy.array_equal(fetch(a, name), fetch(b, name)), name
+    assert torch.equal(a._psi_e, b._psi_e) and torch.equal(a._psi_h[a._h], b._psi_h[b._h])
+    if a._psi_e_out is not None:                      # the two-step pass itself, without S
+        c, d = mk(True), mk(True)
+        amps = torch.tensor([0.5, -0.25], dtype=torch.float32, device="cuda")
+        c.update_HE2([], idx, amps)
+        d.update_HE([], idx, amps[0:1])
+        d.update_HE([], idx, amps[1:2])
+        torch.cuda.synchronize()
+        for name in ("E", "H"):
+            assert numpy.array_equal(fetch(c, name), fetch(d, name)), ("HE2", name)
+
+
+def test_poynting_off_matches_oracle_and_loop():
+    """do_poynting=False against the oracle: step_n (two-step pass where
+    chosen) and the per-call loop, whose update_H is deferred into the next
+    update_E; update_S is None as in the reference."""
+    shape, T = (40, 30, 73), 4
+    eps, pmls, init, _ = _case(shape, T, seed=21, pec=True)
+    sim, orc = build_pair(eps, pmls, init=init, do_poynting=False)
+    assert sim.update_S is None and sim._defer
+    idx = 1 * sim.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 2) * shape[2] + shape[2] // 2
+    wave = numpy.cos(0.2 * numpy.arange(30))
+    sim.step_n(17, idx, wave[:17].astype(numpy.float32))
+    for t in range(17, 30):
+        sim.update_E([]).wait()
+        sim.E[idx] += float(numpy.float32(wave[t]))
+        sim.update_H([]).wait()
+    step_oracle(orc, 30, idx, wave)
+    for name in ("E", "H"):
+        g, o = fetch(sim, name), getattr(orc, name)
+        bad = numpy.flatnonzero(g != o)
+        assert bad.size == 0, (name, bad.size, bad[:4])

@@ -162,9 +162,9 @@ def test_workloads():
 
 
 def test_update_he_support_rules():
-    """The fused H+E pass needs fused Poynting (long rows are cut into z
-    segments), and with Bloch boundaries FP32 and rows of at most 512 cells
-    (plan-level query, no GPU)."""
+    """The fused H+E pass needs fused Poynting or Poynting off (long rows
+    are cut into z segments), and with Bloch boundaries fused Poynting, FP32
+    and rows of at most 512 cells (plan-level query, no GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -182,8 +182,9 @@ def test_update_he_support_rules():
     assert supported(poynting=1, shape=(4, 4, 5000)) == 1
     assert supported(poynting=1, shape=(4, 4, 512), bloch_mask=1) == 1
     assert supported(poynting=1, shape=(4, 4, 513), bloch_mask=1) == 0
-    assert supported(poynting=0) == 0
+    assert supported(poynting=0) == 1        # Poynting off: the pass without S
     assert supported(poynting=2) == 0
+    assert supported(poynting=0, bloch_mask=1) == 0
     # a Bloch pair runs the pass in FP32 on uniform grids
     assert supported(poynting=1, bloch_mask=1) == 1
     assert supported(poynting=1, bloch_mask=3, bloch_mode=1) == 1
@@ -221,9 +222,10 @@ def test_update_he_rejects_bad_calls():
 
 
 def test_update_he2_support_rules():
-    """The two-step pass needs FP32, fused Poynting, no Bloch boundaries, a
-    coefficient block that fits its LDS copy and the CPML slabs of one axis
-    apart; rows of any length (plan-level query, no GPU)."""
+    """The two-step pass needs FP32, fused Poynting or Poynting off, no
+    Bloch boundaries, a coefficient block that fits its LDS copy and the
+    CPML slabs of one axis apart; rows of any length (plan-level query, no
+    GPU)."""
     from opencl_fdtd_amd import _lib
     lib = _lib.load()
 
@@ -242,7 +244,7 @@ def test_update_he2_support_rules():
     assert supported(shape=(4, 4, 257)) == 1
     assert supported(shape=(4, 4, 5000)) == 1
     assert supported(dtype=1) == 0
-    assert supported(poynting=0) == 0
+    assert supported(poynting=0) == 1
     assert supported(poynting=2) == 0
     assert supported(bloch_mask=2) == 0
     assert supported(shape=(6, 7, 5), faces=six) == 0          # both z slabs (3 of 5 cells) overlap
