@@ -33,12 +33,13 @@ def kernel_bytes(shape, pmls, kind, dtype_bytes=4, poynting=True):
     """Algorithmic bytes of one launch: E kernel = E r+w, H r, eps r (12
     words/cell); H kernel = E r, H r, H w, S w (12 words/cell with Poynting,
     9 without); plus psi r+w (4 words) per CPML slab cell and face.  The
-    fused H+E pass ("HE") = E r, H r, eps r, H w, S w, E w (18 words/cell)
-    plus both psi blocks (8 words per slab cell and face).  The two-step
-    pass ("HE2") moves the same per launch, for two steps."""
+    fused H+E pass ("HE") = E r, H r, eps r, H w, S w, E w (18 words/cell;
+    15 without Poynting) plus both psi blocks (8 words per slab cell and
+    face).  The two-step pass ("HE2") moves the same per launch, for two
+    steps."""
     n = float(numpy.prod(shape))
     if kind in ("HE", "HE2"):
-        words, psi_words = 18, 8
+        words, psi_words = (18 if poynting else 15), 8
     else:
         words, psi_words = (12 if (kind == "E" or poynting) else 9), 4
     slab = 0.0
