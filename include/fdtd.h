@@ -55,7 +55,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 5
+#define FDTD_ABI_VERSION 6
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -306,12 +306,15 @@ int fdtd_energy_noalloc(const void* plan, const void* E, const void* H, const vo
 
 /* ------------------------------------------------------------------------
  * Learned absorbing shell.  No counterpart in the reference: a thin shell of
- * T cells on every face runs the CPML-T update, and each step a linear
- * 2-layer 3x3x3 convolution (6 -> 16 -> 3 channels: E, H in, a correction
- * to E out) of the fields after the E update (and source) is added to E on
- * the shell cells.  Six weight sets, one per face box, each with one
- * layer-2 head per depth (index along the box's thin axis); BF16 weights
- * and activations, FP32 accumulation.
+ * T cells on every face runs the CPML-T update, and a linear 2-layer 3x3x3
+ * convolution (6 -> 16 -> 3 channels: E, H in, a correction to E out) of
+ * the fields after the E update (and source) is added to E on the shell
+ * cells every step, every second or every fourth step (fdtd_shell_step_n's
+ * period).  Six
+ * weight sets, one per face box, each with one layer-2 head per depth
+ * (index along the box's thin axis); BF16 weights.  Inference evaluates the
+ * two layers as their composed 5x5x5 convolution (composed in FP32, rounded
+ * once to BF16) on BF16 inputs with FP32 accumulation.
  * Weights: fdtd_shell_weight_elems() BF16 values, W1[box][16][6][27] for the
  * six boxes, then W2[box][depth 0..3][3][16][27]; taps
  * t = 9(dx+1) + 3(dy+1) + (dz+1).
@@ -326,13 +329,22 @@ int64_t fdtd_shell_partials_elems(void);
 /* E += correction(E, H) on the shell cells (E read before it is corrected). */
 int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, void* E, const void* H,
                      float* hidden, void* stream);
-/* n steps of the shell solver, each: E update and source (as fdtd_step_n);
- * H update from that E; E += correction(E, H) with E and H as the H update
- * read them (the addition is made by the next step's E update, and by a
- * final pass after step n).  Swap rules as fdtd_step_n.  Real fields only
+/* n steps of the shell solver.  Step k: E update (with the pending
+ * correction added to E first) and source, as fdtd_step_n; H update from
+ * that E; then, when the step is corrected, E += correction(E, H) with E
+ * and H as the H update read them.  period 1: every step is corrected (the
+ * addition is made by the next step's E update, and by a final pass after
+ * step n); period 2 or 4: steps 0, period, 2 period, ... of the call (n a
+ * multiple of the period; each correction is added by the next step's E
+ * update).  With period 2 or 4, FP32, a uniform grid, the fused passes'
+ * buffers and moved non-NULL, each pair of steps runs as one two-step pass
+ * with the corrections between passes, and E, psi_E and H may end in the
+ * other buffers: *moved gets the bits of fdtd_step_n_moved.  Otherwise E and psi_E end in f->E and f->psi_e (H as
+ * fdtd_step_n; *moved reports it when moved is not NULL).  Real fields only
  * (no Bloch boundaries), two-pass Poynting excluded. */
 int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness, const uint16_t* w,
-                      float* hidden, int32_t n, int64_t src_index, const void* src_wave, void* stream);
+                      float* hidden, int32_t n, int32_t period, int64_t src_index, const void* src_wave,
+                      void* stream, int32_t* moved);
 /* Adjoint of E_out = E + correction(E, H): given Ebar (adjoint of E_out),
  * Ebar_out += Ebar-contribution through the correction's E input (callers
  * initialise Ebar_out with Ebar for the identity part), Hbar_out += the H

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_FACES = 12
 
 F32, F64 = 0, 1
@@ -128,7 +128,8 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_int64
     lib.fdtd_shell_apply.argtypes = [P, ctypes.c_int32, P, P, P, P, P]
     lib.fdtd_shell_backward.argtypes = [P, ctypes.c_int32] + [P] * 10 + [P]
-    lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P,
+                                      ctypes.POINTER(ctypes.c_int32)]
     sized = ("fdtd_last_error", "fdtd_plan_describe", "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_reduce_partials",
              "fdtd_shell_weight_elems", "fdtd_shell_hidden_elems", "fdtd_shell_partials_elems")
     for name in EXPORTS:

@@ -209,8 +209,7 @@ struct EArgs {
     const T* src_amp;
     int src_x, src_p, src_comp;
     // pending learned-shell correction: E += ycorr before the update on the
-    // cells within ycT of a face, stored per component in the shell's box
-    // order (shell_index); null: none
+    // cells within ycT of a face, per component on the grid; null: none
     const float* ycorr[3];
     int ycT;
     int coef_elems;
@@ -839,31 +838,12 @@ __device__ __forceinline__ void e_update(const Geom<T>& g, const S& s, const Cel
     stc<FDTD_NT_E>(s.E, 2, oi, ez);
 }
 
-// The learned shell's cell order: the x faces (T x sy x sz each), then the
-// y faces ((sx-2T) x T x sz), then the z faces ((sx-2T) x (sy-2T) x T), C
-// order inside a box (the boxes of shell_net.inl).
-// Per-thread part of that index for a fixed plane cell (y, z): index =
-// xf + x * sy * sz on the x faces, base + (x - T) * step elsewhere (when
-// (y, z) is within T of a y or z face; in_yz false otherwise).
-struct ShellCol {
-    uint32_t xf, base, step;
-    bool in_yz;
-};
-
-__device__ __forceinline__ ShellCol shell_col(int y, int z, int T, int sx, int sy, int sz) {
-    const uint32_t bx = (uint32_t)T * sy * sz;
-    const int mx = sx - 2 * T, my = sy - 2 * T;
-    const uint32_t by = (uint32_t)mx * T * sz;
-    const uint32_t bz = (uint32_t)mx * my * T;
-    ShellCol c;
-    c.xf = (uint32_t)y * sz + z;
-    c.in_yz = true;
-    if (y < T) { c.base = 2 * bx + (uint32_t)y * sz + z; c.step = (uint32_t)T * sz; }
-    else if (y >= sy - T) { c.base = 2 * bx + by + (uint32_t)(y - (sy - T)) * sz + z; c.step = (uint32_t)T * sz; }
-    else if (z < T) { c.base = 2 * bx + 2 * by + (uint32_t)(y - T) * T + z; c.step = (uint32_t)my * T; }
-    else if (z >= sz - T) { c.base = 2 * bx + 2 * by + bz + (uint32_t)(y - T) * T + (z - (sz - T)); c.step = (uint32_t)my * T; }
-    else { c.base = 0; c.step = 0; c.in_yz = false; }
-    return c;
+// The learned shell's correction is kept per component on the grid (3 N
+// values, read at the cell's own offset; only shell cells are written).  A
+// cell is in the shell when x, y or z is within T of a face: in_yz is the
+// per-thread part of that test for a fixed plane cell (y, z).
+__device__ __forceinline__ bool shell_yz(int y, int z, int T, int sy, int sz) {
+    return y < T || y >= sy - T || z < T || z >= sz - T;
 }
 
 // SHELL: add the pending learned-shell correction (EArgs::ycorr) to E
@@ -909,8 +889,7 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
         cols.y = psi_col<1>(g, c, slab_hit(g, 1, c.y, g.sy));
         cols.z = psi_col<2>(g, c, slab_hit(g, 2, c.z, g.sz));
     }
-    ShellCol sc{};
-    if (SHELL) sc = shell_col(c.y, c.z, a.ycT, g.sx, g.sy, g.sz);
+    const bool in_yz = SHELL && shell_yz(c.y, c.z, a.ycT, g.sy, g.sz);
 
     EIn<T> cur[NS], nxt[NS];
     T e0, e1, e2, n0 = (T)0, n1 = (T)0, n2 = (T)0;
@@ -935,12 +914,9 @@ __global__ __launch_bounds__(kBlock, FDTD_E_WAVES) void e_march(EArgs<T> a) {
         bool in_shell = false;
         float y0 = 0.f, y1 = 0.f, y2 = 0.f;
         if (SHELL) {
-            const bool xlo = x < a.ycT, xhi = x >= g.sx - a.ycT;
-            in_shell = xlo || xhi || sc.in_yz;
+            in_shell = x < a.ycT || x >= g.sx - a.ycT || in_yz;
             if (in_shell) {
-                const uint32_t i = xlo   ? sc.xf + (uint32_t)x * g.plane
-                                   : xhi ? (uint32_t)a.ycT * g.plane + sc.xf + (uint32_t)(x - (g.sx - a.ycT)) * g.plane
-                                         : sc.base + (uint32_t)(x - a.ycT) * sc.step;
+                const uint32_t i = (uint32_t)x * g.plane + (uint32_t)p;
                 y0 = a.ycorr[0][i];
                 y1 = a.ycorr[1][i];
                 y2 = a.ycorr[2][i];
@@ -1315,7 +1291,7 @@ struct HEArgs {
     int src_x, src_p, src_comp;
     int coef_elems;
     // learned shell (SHELL): the correction added to E^n before its E update
-    // on the cells within ycT of a face, in the shell's box order
+    // on the cells within ycT of a face, per component on the grid
     Packed3<const float> ycorr;
     int ycT;
 };
@@ -1398,8 +1374,7 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
     // Everything a plane iteration loads from memory: the H-step inputs,
     // the psi pairs of both half-steps and eps.  With PF the loads of plane
     // x+1 are issued before plane x is computed.
-    ShellCol sc{};
-    if (SHELL) sc = shell_col(c.y, c.z, a.ycT, g.sx, g.sy, g.sz);
+    const bool in_yz = SHELL && shell_yz(c.y, c.z, a.ycT, g.sy, g.sz);
     struct PlaneIn {
         HIn<T> h;
         PsiPair<T> qh, qe;    // (left unset where the cell is in no slab: only read where it is)
@@ -1428,15 +1403,11 @@ __global__ __launch_bounds__(HB, FDTD_HE_WAVES) void he_march(HEArgs<T> a) {
         }
         in.ins = false;
         if (SHELL && own && !dryf) {
-            const bool xlo = xf < a.ycT, xhi = xf >= g.sx - a.ycT;
-            in.ins = xlo || xhi || sc.in_yz;
+            in.ins = xf < a.ycT || xf >= g.sx - a.ycT || in_yz;
             if (in.ins) {
-                const uint32_t i = xlo   ? sc.xf + (uint32_t)xf * g.plane
-                                   : xhi ? (uint32_t)a.ycT * g.plane + sc.xf + (uint32_t)(xf - (g.sx - a.ycT)) * g.plane
-                                         : sc.base + (uint32_t)(xf - a.ycT) * sc.step;
-                in.y0 = ldc<false>(a.ycorr, 0, i * 4u);
-                in.y1 = ldc<false>(a.ycorr, 1, i * 4u);
-                in.y2 = ldc<false>(a.ycorr, 2, i * 4u);
+                in.y0 = ldc<false>(a.ycorr, 0, of);
+                in.y1 = ldc<false>(a.ycorr, 1, of);
+                in.y2 = ldc<false>(a.ycorr, 2, of);
             }
         }
         h_fetch<T, false>(g, hs, c, xf, pbytes, in.h);
@@ -1778,6 +1749,10 @@ struct HE2Args {
     const T* src_amp;               // two values: added to E^{n+1} and E^{n+2} at the source; null: none
     int src_x, src_p, src_comp;
     int coef_elems;
+    // learned shell (SHELL): the correction added to E^n before the first
+    // E update on the cells within ycT of a face, per component on the grid
+    Packed3<const float> ycorr;
+    int ycT;
 };
 
 // LC: H^{n+1/2}, E^{n+1} and eps of plane x-1 are read back from LDS (by
@@ -1786,8 +1761,11 @@ struct HE2Args {
 // they are in flight while the second stage finishes plane x-1.
 // POY: fused Poynting (S of the second H half-step); without it (and then
 // without S2) the pass skips the oS products and S.
+// SHELL: add the pending learned-shell correction to E^n before the first
+// E update, on every cell that update evaluates (the halo rows' E^{n+1}
+// feeds the second H half-step); the first H update and S use E^n as it is.
 template <class T, bool NU, bool S2, bool E2 = true, bool LC = (FDTD_HE2_LDSCARRY != 0),
-          bool EARLY = (FDTD_HE2_EARLY != 0), bool POY = true>
+          bool EARLY = (FDTD_HE2_EARLY != 0), bool POY = true, bool SHELL = false>
 __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 3) void he2_march(HE2Args<T> a) {
     static_assert(POY || !S2, "the first half-step's S needs Poynting");
     constexpr int HB = FDTD_HE2_HB;
@@ -1857,7 +1835,10 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
         HIn<T> h;
         PsiPair<T> qh, qe;    // (left unset where the cell is in no slab: only read where it is)
         T e0, e1, e2;
+        bool ins;             // the cell is in the shell (SHELL)
+        float y0, y1, y2;     // its pending correction
     };
+    const bool in_yz = SHELL && shell_yz(c.y, c.z, a.ycT, g.sy, g.sz);
     const auto fetch = [&](int xf, bool ef, Loads& L) {
         CpmlCols<T> kf = cols;
         kf.hx = slab_hit(g, 0, xf, g.sx);
@@ -1869,6 +1850,16 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
             L.e0 = ldc<FDTD_NT_E>(a.eps, 0, of);
             L.e1 = ldc<FDTD_NT_E>(a.eps, 1, of);
             L.e2 = ldc<FDTD_NT_E>(a.eps, 2, of);
+        }
+        L.ins = false;
+        if (SHELL && rowe && ef) {
+            L.ins = xf < a.ycT || xf >= g.sx - a.ycT || in_yz;
+            if (L.ins) {
+                const uint32_t of = (uint32_t)xf * pbytes + c.op;
+                L.y0 = ldc<false>(a.ycorr, 0, of);
+                L.y1 = ldc<false>(a.ycorr, 1, of);
+                L.y2 = ldc<false>(a.ycorr, 2, of);
+            }
         }
         h_fetch<T, false>(g, hA, c, xf, pbytes, L.h);
     };
@@ -1938,6 +1929,11 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
             v.hx_zm = la[ba][0][tzm];
             v.hy_zm = la[ba][1][tzm];
             v.ex = qx; v.ey = qy; v.ez = qz;
+            if (SHELL && cur.ins) {
+                v.ex = v.ex + (T)cur.y0;
+                v.ey = v.ey + (T)cur.y1;
+                v.ez = v.ez + (T)cur.y2;
+            }
             const T jxa = NU ? g.inv_h[0][xa] : g.inv_u;
             const bool src = src_comp >= 0 && xa == a.src_x;
             if constexpr (LC) {
@@ -2563,10 +2559,14 @@ bool he2_supported(const Plan& P) { return P.lhe2.mode == 2; }
 // last_e false: the pass stops after the second H half-step (H, E with the
 // source, H with S), which ends a step_n schedule with an odd pair count
 int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float* src_amp, int64_t src_index,
-               bool last_e = true) {
+               bool last_e = true, const float* ycorr = nullptr, int ycT = 0, int64_t ycells = 0) {
     using T = float;
     HE2Args<T> a;
     memset(&a, 0, sizeof(a));
+    if (ycorr) {
+        a.ycorr = {ycorr, (uint32_t)(ycells * 4)};
+        a.ycT = ycT;
+    }
     fill_geom<T>(P, P.lhe2, f, a.g);
     const uint32_t nb = (uint32_t)(P.N * (int64_t)sizeof(T));
     a.E = {(const T*)f->E, nb};
@@ -2591,6 +2591,17 @@ int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
     }
     const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
     const bool nu = P.d.nonuniform != 0;
+    if (ycorr) {
+        // the learned shell's passes (uniform grids), Poynting fused or off
+        constexpr bool LC = FDTD_HE2_LDSCARRY != 0, EA = FDTD_HE2_EARLY != 0;
+        if (nu) return fail(3, "the shell correction's two-step pass needs a uniform grid");
+        const bool poy = P.d.poynting == FDTD_POYNTING_FUSED;
+        if (poy && last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, true, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (poy) hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, true, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, false, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        else hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, false, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        return check_launch("fdtd_update_he2");
+    }
     if (P.d.poynting == FDTD_POYNTING_FUSED) {
         if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
         else if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);

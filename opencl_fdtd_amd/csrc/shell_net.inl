@@ -19,11 +19,8 @@
 // live per box on the box grown by one cell, in a caller-provided FP32
 // scratch [cell][16].
 
-#ifndef FDTD_SHELL_TILE_A1
-#define FDTD_SHELL_TILE_A1 8         // shell_mfma tile: output rows along a1 (even; 8 or 10)
-#endif
 #ifndef FDTD_SHELL_WAVES
-#define FDTD_SHELL_WAVES 4           // shell_mfma: minimum waves per SIMD (4 workgroups of 4 waves per CU)
+#define FDTD_SHELL_WAVES 3           // shell_conv: minimum waves per SIMD (3 workgroups of 4 waves per CU)
 #endif
 
 namespace {
@@ -118,48 +115,61 @@ __global__ __launch_bounds__(256) void shell_wgrad_sum(const float* part, int gr
     out[wi] += (float)s;
 }
 
-// ---- inference with MFMA (v_mfma_f32_16x16x32_bf16) ----------------------
+// ---- inference: the composed 5x5x5 convolution on the matrix cores -------
+//
+// The net has no activation, so per box b and depth d its two layers are one
+// 5x5x5 convolution from the 6 inputs to the 3 outputs:
+//   Wc[b][d][o][c][s] = sum_k sum_u W2[b][d][o][k][u] W1[b][k][c][s - u]
+// over the 3x3x3 offsets u with s - u also one (shell_compose: k outer, u
+// inner, FP32 products of BF16 values, which are exact, and FP32 sums, then
+// one rounding to BF16), and
+//   y_o(i) = sum_{c,s} Wc[b][d(i)][o][c][s] x_c(i + s),   x rounded to BF16.
+// That is 2250 multiply-adds per output cell against 3888 for the two layers
+// (whose hidden layer a tile must also evaluate on a halo).
 //
 // One workgroup (4 waves) per tile of a box.  A tile has three axes: a0 the
-// box's thin axis (T cells), a1 (8 cells) and a2 (14 cells), a2 innermost in
-// LDS; a2 is z for the x and y faces and y for the z faces (a1 is the
-// remaining axis).  The tile's input grown by 2 (T+4 x 12 x 18 cells) is
-// staged in LDS as BF16 [cell][8] (6 channels + 2 zero pads), its hidden
-// layer grown by 1 (T+2 x 10 x 16) as BF16 in two planes [cell][8 channels]
-// over the same LDS words.
+// box's thin axis (all T depths), a1 (16 rows) and a2 (16 columns, innermost
+// in LDS); a2 is z for the x and y faces and y for the z faces.  Its input,
+// planes -2 .. 5 along a0 (8 whatever T; weights of planes past T+1 are
+// zero) by 20 x 20 cells, is staged in LDS as BF16 [cell][8] (6 channels
+// and 2 zero pads), with the a0 stride padded by one cell so that writes in
+// grid order (z = a0 on the z faces) meet no bank conflicts.
 //
-// Each layer is a GEMM with the weights as the A operand (M = the layer's
-// output channels) and 16 cells of one a2 row as B (N; a hidden row is
-// exactly 16 cells, an output row 14 and two discarded), K running over
-// (tap, input channel).  A fragment (K = 32) covers one tap pair (d0, d1)
-// along a0, a1 with its three taps along a2 in the lane groups:
-//   layer 1: 9 fragments, group g < 3 = tap d2 = g x 8 input channels
-//            (6 used), group 3 zero;
-//   layer 2: 18 fragments, two per pair: (d2 = 0 channels 0-7, 8-15,
-//            d2 = 1 channels 0-7, 8-15) and (d2 = 2 channels 0-7, 8-15,
-//            zero, zero).
-// The fragment of pair (d0, d1) for row r reads the same cells as pair
-// (d0, d1 + 1) for row r - 1, so a wave walking a run of rows along a1 loads
-// each B fragment once and feeds it to up to 3 rows' MFMAs.
+// The convolution is a GEMM with the weights as the A operand (M = 16 rows
+// m = 4 d + o: depth d, output o < 3) and 16 cells of one a2 row as B (N).
+// K runs over (input plane j, row offset e1, column offset e2, channel):
+// a fragment (K = 32) holds four (j, e2) pairs p = 4 q + g (lane group g;
+// j = p / 5, e2 = p % 5) of one row offset e1, 8 channels each, and the
+// weight of depth d is the one of offset j - 2 - d along a0 (zero beyond
+// +-2).  So every MFMA yields all T depths of 16 cells, and the B fragment
+// of input row r, group q feeds the MFMAs of output rows r - e1 for the
+// five e1: a wave walking its rows reads each B fragment once for up to 5
+// MFMAs.  50 fragments (10 q x 5 e1) per box: waves 2 h + k take output
+// rows 8 h .. 8 h + 7 and the pairs q = 5 k .. 5 k + 4 (25 fragments,
+// 100 VGPRs, held for the whole tile); the two K halves of a row are summed
+// through LDS at the end.
 // MFMA operands (16x16x32): lane l holds A row l%16 and B column l%16 at
 // k = 8*(l/16) + [0..7]; D column l%16 at rows 4*(l/16) + [0..3], i.e. for
-// cell l%16 the output channels 4*(l/16) + [0..3].  The correction goes to a
-// scratch in box order [3][shell cells] and is added to E later (by the next
-// E update, or by shell_add), since neighbouring tiles read E around their
-// own cells.
+// cell l%16 the outputs 0..2 of depth l/16.  The correction goes to a
+// scratch laid out as the fields, [3][N] (shell cells only), and is added
+// to E later (by the next E update, or by shell_add), since neighbouring
+// tiles read E around their own cells.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxT = 4;
-constexpr int kTileA1 = FDTD_SHELL_TILE_A1, kTileA2 = 14;   // output cells along a1, a2
-constexpr int kRow = 16;                                  // MFMA columns = hidden a2 extent
-constexpr int kInA0 = (kTileA1 + 4) * (kTileA2 + 4) + 1;  // input a0 stride (padded: no bank conflicts)
-constexpr int kInMax = (kMaxT + 4) * kInA0;
-constexpr int kHidRows = (kMaxT + 2) * (kTileA1 + 2);     // hidden a2 rows (of 16 cells)
-constexpr int kHidPlane = kHidRows * kRow;                // cells per hidden plane
-constexpr int kTileLds = kInMax > 2 * kHidPlane ? kInMax : 2 * kHidPlane;   // uint4 words
-static_assert(kTileA2 + 2 == kRow, "hidden rows must be one MFMA wide");
-static_assert(kDepth >= kMaxT, "one layer-2 head per shell depth");
+constexpr int kCR = 8;                                 // output rows per wave
+constexpr int kTileA1 = 2 * kCR, kTileA2 = 16;         // output rows (a1) and columns (a2) per tile
+constexpr int kInR = kTileA1 + 4, kInC = kTileA2 + 4;  // input rows and columns
+constexpr int kInPlanes = kMaxT + 4;                   // input planes along a0
+constexpr int kInA0 = kInR * kInC + 1;                 // LDS cells per input plane (padded)
+constexpr int kTileLds = kInPlanes * kInA0;            // uint4 words (51.3 KB)
+constexpr int kPairs = kInPlanes * 5;                  // (plane, column offset) pairs
+constexpr int kQ = kPairs / 4, kQHalf = kQ / 2;        // pair groups, per K half
+constexpr int kFragBox = 5 * kQ;                       // A fragments per box
+static_assert(kPairs % 8 == 0, "two K halves of whole pair groups");
+static_assert(kDepth >= kMaxT && kDepth * 4 <= 16, "one MFMA row group per depth");
+static_assert(kTileA1 * 64 <= kTileLds, "the K-half partials fit the input tile's LDS");
 
 struct ShellTiles {
     int perm[6][3];    // grid axis of tile axes a0, a1, a2
@@ -178,40 +188,45 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, r);
 }
 
-// grid-axis displacement (0..2) of tap t along grid axis q, and the tap of
-// tile-axis displacements (d0, d1, d2) for a tile with axis permutation p
-__host__ __device__ __forceinline__ int tap_of(const int* p, int d0, int d1, int d2) {
-    int d[3];
-    d[p[0]] = d0; d[p[1]] = d1; d[p[2]] = d2;
-    return (d[0] * 3 + d[1]) * 3 + d[2];
-}
+// Composed weights, rounded to BF16, as A fragments in lane order: per box
+// [K half][e1][q - 5 K half][64 lanes][8 channels], so a lane loads each
+// fragment with one 16-byte read.  One thread per BF16 value.
+constexpr int kPackBox = kFragBox * 64 * 8;                // BF16 values per box
+constexpr int kPackFloats = 6 * kPackBox / 2;              // FP32 words of scratch for 6 boxes
 
-// A fragments in lane order, per box: [kFrag1 + kDepth * kFrag2][64
-// lanes][8] BF16 (layer 1, then layer 2 head by head), so a lane loads each
-// fragment with one 16-byte read (layouts above).
-constexpr int kFrag1 = 9, kFrag2 = 18;                    // per box; kFrag2 per layer-2 head
-constexpr int kPackBox = (kFrag1 + kDepth * kFrag2) * 64 * 8;   // BF16 values per box
-constexpr int kPackFloats = 6 * kPackBox / 2;             // FP32 words of scratch for 6 boxes
-
-__global__ __launch_bounds__(256) void shell_pack(const uint16_t* w, int nbox, ShellTiles tl, uint16_t* packed) {
+__global__ __launch_bounds__(256) void shell_compose(const uint16_t* w, int nbox, int T, ShellTiles tl,
+                                                     uint16_t* packed) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= nbox * kPackBox) return;
     const int b = idx / kPackBox, r = idx % kPackBox;
-    const int f = r / 512, lane = (r / 8) % 64, i = r % 8;
-    const int g = lane >> 4, n = lane & 15;
-    const int* p = tl.perm[b];
-    uint16_t v = 0;
-    if (f < kFrag1) {
-        if (g < 3 && i < kIn) v = w[(size_t)b * kW1 + (n * kIn + i) * kTaps + tap_of(p, f / 3, f % 3, g)];
-    } else {
-        const int dep = (f - kFrag1) / kFrag2, f2 = (f - kFrag1) % kFrag2;
-        const int pair = f2 >> 1, h = f2 & 1;
-        const int d2 = h == 0 ? (g >> 1) : 2;
-        if (n < kOut && (h == 0 || g < 2))
-            v = w[(size_t)nbox * kW1 + ((size_t)b * kDepth + dep) * kW2 + (n * kHid + 8 * (g & 1) + i) * kTaps +
-                  tap_of(p, pair / 3, pair % 3, d2)];
+    const int f = r / 512, lane = (r / 8) % 64, c = r % 8;
+    const int kh = f / (5 * kQHalf), e1 = (f / kQHalf) % 5, q = kh * kQHalf + f % kQHalf;
+    const int m = lane & 15, g = lane >> 4;
+    const int d = m >> 2, o = m & 3;
+    const int p = 4 * q + g;
+    const int d0 = p / 5 - 2 - d;                          // offset along a0 of plane p / 5 from depth d
+    float acc = 0.f;
+    if (o < kOut && d < T && c < kIn && d0 >= -2 && d0 <= 2) {
+        int s[3];                                          // offset along the grid axes
+        const int* pm = tl.perm[b];
+        s[pm[0]] = d0; s[pm[1]] = e1 - 2; s[pm[2]] = p % 5 - 2;
+        // layer-2 taps u with s - u a layer-1 tap, in increasing tap order
+        int lo[3], hi[3];
+        for (int q2 = 0; q2 < 3; ++q2) { lo[q2] = max(-1, s[q2] - 1); hi[q2] = min(1, s[q2] + 1); }
+        const uint16_t* w1 = w + (size_t)b * kW1 + (size_t)c * kTaps;
+        const uint16_t* w2 = w + (size_t)nbox * kW1 + ((size_t)b * kDepth + d) * kW2 + (size_t)o * kHid * kTaps;
+        for (int k = 0; k < kHid; ++k) {
+            for (int u0 = lo[0]; u0 <= hi[0]; ++u0)
+                for (int u1 = lo[1]; u1 <= hi[1]; ++u1)
+                    for (int u2 = lo[2]; u2 <= hi[2]; ++u2) {
+                        const int u = (u0 + 1) * 9 + (u1 + 1) * 3 + (u2 + 1);
+                        const int v = (s[0] - u0 + 1) * 9 + (s[1] - u1 + 1) * 3 + (s[2] - u2 + 1);
+                        const float prod = bf16_to_f(w2[k * kTaps + u]) * bf16_to_f(w1[k * kIn * kTaps + v]);
+                        acc = acc + prod;
+                    }
+        }
     }
-    packed[idx] = v;
+    packed[idx] = __builtin_bit_cast(uint16_t, (__bf16)acc);
 }
 
 // floor(c / d) for 0 <= c < 4096 and 1 <= d <= 64 from r = 1/d: (c + 0.5)/d
@@ -231,42 +246,52 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const uint4& b, const f32
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-constexpr int kInIter = (kInMax + 255) / 256;           // input cells per thread (upper bound)
-
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FDTD_SHELL_WAVES))) void shell_mfma(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf) {
-    // the input tile [cell][8 x BF16], then (after layer 1) the hidden tile
-    // [2 planes][cell][8 x BF16] in the same words
-    __shared__ uint4 s_buf[kTileLds];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FDTD_SHELL_WAVES))) void shell_conv(ShellArgs<T> a, ShellTiles tl, const uint16_t* packed, float* ybuf) {
+    __shared__ uint4 s_buf[kTileLds];         // the input tile [cell][8 x BF16], then the K-half partials
     // hardware workgroup k runs on XCD k % 8; each XCD gets a contiguous
     // range of tiles so the halos of neighbouring tiles meet in its L2
     const int nb = gridDim.x, qq = nb >> 3, rr = nb & 7, xcd = blockIdx.x & 7, jj = blockIdx.x >> 3;
     const int tile = (xcd < rr ? xcd * (qq + 1) : rr * (qq + 1) + (xcd - rr) * qq) + jj;
-    int b = 0;
-    while (b < a.nbox - 1 && tile >= tl.tstart[b + 1]) ++b;
+    // tiles alternate between the two boxes of an axis (same tile grid), so
+    // the two sides of a tile column run together on one XCD: the lines
+    // they share (the wrap of a row, or its two ends on the z faces) are
+    // fetched from HBM once
+    int ax = 0;
+    while (ax < 2 && tile >= tl.tstart[2 * ax + 2]) ++ax;
+    const int b = 2 * ax + ((tile - tl.tstart[2 * ax]) & 1);
     const ShellBox& B = a.box[b];
     const int p0 = tl.perm[b][0], p1 = tl.perm[b][1], p2 = tl.perm[b][2];
-    const int te0 = tl.te[b][0];
-    const int ie2 = kTileA2 + 4, he1 = kTileA1 + 2, he0 = te0 + 2;
-    const int tix = tile - tl.tstart[b];
-    const int t2 = tix % tl.nt[b][2], t1 = (tix / tl.nt[b][2]) % tl.nt[b][1], t0 = tix / (tl.nt[b][2] * tl.nt[b][1]);
+    const int tix = (tile - tl.tstart[2 * ax]) >> 1;
+    const int t2 = tix % tl.nt[b][2], t1 = tix / tl.nt[b][2];
     int o[3];                                  // tile origin, grid axes
-    o[p0] = B.o0[p0] + t0 * te0;
+    o[p0] = B.o0[p0];
     o[p1] = B.o0[p1] + t1 * kTileA1;
     o[p2] = B.o0[p2] + t2 * kTileA2;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kh = wave & 1, rh = wave >> 1;
+    const int g = lane >> 4, n = lane & 15;
+    // 0. this wave's 25 A fragments (loaded first: they need nothing else)
+    bf16x8 A[5 * kQHalf];
+    {
+        const uint4* fr = reinterpret_cast<const uint4*>(packed) + ((size_t)b * 2 + kh) * (5 * kQHalf) * 64 + lane;
+#pragma unroll
+        for (int f = 0; f < 5 * kQHalf; ++f) A[f] = __builtin_bit_cast(bf16x8, fr[f * 64]);
+    }
     const size_t N = (size_t)a.sx * a.sy * a.sz;
     // 1. input tile -> LDS (BF16, 8 per cell), loaded in grid order (z
     // fastest) in passes whose loads are all issued before the first
     // conversion (threads past the tile reload its last cell)
     {
         int ext[3], str[3];                    // input extents and LDS strides per grid axis
-        ext[p0] = te0 + 4; ext[p1] = kTileA1 + 4; ext[p2] = ie2;
-        str[p0] = kInA0; str[p1] = ie2; str[p2] = 1;
-        const int nin = ext[0] * ext[1] * ext[2];
+        ext[p0] = kInPlanes; ext[p1] = kInR; ext[p2] = kInC;
+        str[p0] = kInA0; str[p1] = kInC; str[p2] = 1;
+        constexpr int nin = kInPlanes * kInR * kInC;
         const float r2 = 1.0f / ext[2], r1 = 1.0f / ext[1];
         const T* E0 = a.E; const T* E1 = a.E + N; const T* E2 = a.E + 2 * N;
         const T* H0 = a.H; const T* H1 = a.H + N; const T* H2 = a.H + 2 * N;
-        constexpr int U = sizeof(T) == 4 ? (kInIter + 1) / 2 : 2;   // cells per pass
+        constexpr int U = sizeof(T) == 4 ? 4 : 2;   // cells per pass
         for (int c0 = 0; c0 < nin; c0 += U * 256) {
             float v[U][6];
             int dst[U];
@@ -289,133 +314,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FDTD_SHELL_
             }
         }
     }
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int g = lane >> 4, n = lane & 15;
-    const bf16x8* frag = reinterpret_cast<const bf16x8*>(packed) + (size_t)b * (kFrag1 + kDepth * kFrag2) * 64 + lane;
     __syncthreads();
-    // 2. layer 1: wave w takes a0 rows h0 = w/2, w/2 + 2, w/2 + 4 (< T+2) and
-    // along a1 the kR1 = (kTileA1 + 2) / 2 rows h1 = kR1 (w % 2) + [0, kR1);
-    // B fragment (h0 + d0, y) feeds rows y - d1.  Results stay in registers
-    // (BF16, channels 4g .. 4g+3 of cell n) until every wave is done with
-    // the input tile.
-    constexpr int kR1 = (kTileA1 + 2) / 2, kH0 = 3;
-    static_assert(2 * kR1 == kTileA1 + 2, "two waves split the hidden rows along a1");
-    uint32_t hreg[kH0][kR1][2];
+    // 2. the GEMM: input row r (of the wave's kCR + 4), pair group q feeds
+    // output rows r - e1
+    f32x4 acc[kCR];
+#pragma unroll
+    for (int r = 0; r < kCR; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
     {
-        bf16x8 W1[kFrag1];
+        int off[kQHalf];
 #pragma unroll
-        for (int f = 0; f < kFrag1; ++f) W1[f] = frag[f * 64];
-        const int h1s = kR1 * (wave & 1);
-        const int lofs = n + min(g, 2);        // this lane's cell along a2 (group 3: any valid cell)
+        for (int k = 0; k < kQHalf; ++k) {
+            const int p = 4 * (kh * kQHalf + k) + g;
+            off[k] = (p / 5) * kInA0 + rh * kCR * kInC + n + p % 5;
+        }
 #pragma unroll
-        for (int k = 0; k < kH0; ++k) {
-            const int h0 = (wave >> 1) + 2 * k;
-            f32x4 acc[kR1];
+        for (int r = 0; r < kCR + 4; ++r) {
 #pragma unroll
-            for (int r = 0; r < kR1; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-            if (h0 < he0) {
+            for (int k = 0; k < kQHalf; ++k) {
+                const uint4 bv = s_buf[off[k] + r * kInC];
 #pragma unroll
-                for (int d0 = 0; d0 < 3; ++d0) {
-                    const uint4* rowp = s_buf + (h0 + d0) * kInA0 + h1s * ie2 + lofs;
-#pragma unroll
-                    for (int y = 0; y < kR1 + 2; ++y) {
-                        const uint4 bv = rowp[y * ie2];
-#pragma unroll
-                        for (int d1 = 0; d1 < 3; ++d1) {
-                            const int r = y - d1;
-                            if (r >= 0 && r < kR1) acc[r] = mfma(W1[d0 * 3 + d1], bv, acc[r]);
-                        }
-                    }
+                for (int e1 = 0; e1 < 5; ++e1) {
+                    const int ro = r - e1;
+                    if (ro >= 0 && ro < kCR) acc[ro] = mfma(A[e1 * kQHalf + k], bv, acc[ro]);
                 }
-            }
-#pragma unroll
-            for (int r = 0; r < kR1; ++r) {
-                hreg[k][r][0] = pack_bf16(acc[r][0], acc[r][1]);
-                hreg[k][r][1] = pack_bf16(acc[r][2], acc[r][3]);
             }
         }
     }
+    // 3. sum the K halves (the odd wave's partials through LDS) and store
+    // outputs 0..2 of cell (depth g, row, column n) to the correction, per
+    // component on the grid: ycorr[component][grid index]
     __syncthreads();
-    {
-        // hidden cell (h0, h1, n) channels 4g .. 4g+3 -> plane g / 2, half g % 2
-        char* hb = reinterpret_cast<char*>(s_buf) + (size_t)(g >> 1) * kHidPlane * 16 + n * 16 + (g & 1) * 8;
-        const int h1s = kR1 * (wave & 1);
+    f32x4* part = reinterpret_cast<f32x4*>(s_buf) + (size_t)rh * kCR * 64 + lane;
+    if (kh == 1) {
 #pragma unroll
-        for (int k = 0; k < kH0; ++k) {
-            const int h0 = (wave >> 1) + 2 * k;
-            if (h0 >= he0) continue;
-#pragma unroll
-            for (int r = 0; r < kR1; ++r)
-                *reinterpret_cast<uint2*>(hb + ((h0 * he1 + h1s + r) * kRow) * 16) = make_uint2(hreg[k][r][0], hreg[k][r][1]);
-        }
+        for (int r = 0; r < kCR; ++r) part[r * 64] = acc[r];
     }
     __syncthreads();
-    // 3. layer 2: wave w takes output a0 row o0 = w (< T), i.e. depth o0 of
-    // the box (a tile is the box's whole depth) and its head, and its 8 rows
-    // along a1; B fragment (o0 + d0, y, half) feeds rows y - d1.  Lanes 14,
-    // 15 of each MFMA read a valid cell and their columns are never used.
-    const int o0 = wave;
-    if (o0 < te0) {
-        bf16x8 W2[kFrag2];                     // rows n >= 3 are zero: only 3 lanes of 16 load
+    if (kh == 1) return;
+    int ext[3], gst[3];
+    for (int q = 0; q < 3; ++q) ext[q] = B.o1[q] - B.o0[q];
+    gst[0] = a.sy * a.sz; gst[1] = a.sz; gst[2] = 1;
+    const int l1 = o[p1] - B.o0[p1] + rh * kCR, l2 = o[p2] - B.o0[p2] + n;
+    if (g < ext[p0] && l2 < ext[p2]) {
+        float* yc = ybuf + (size_t)(B.o0[p0] + g) * gst[p0] + (size_t)(o[p1] + rh * kCR) * gst[p1] +
+                    (size_t)(o[p2] + n) * gst[p2];
 #pragma unroll
-        for (int f = 0; f < kFrag2; ++f) {
-            const uint4 v = n < kOut ? reinterpret_cast<const uint4*>(frag)[(kFrag1 + o0 * kFrag2 + f) * 64]
-                                     : make_uint4(0u, 0u, 0u, 0u);
-            W2[f] = __builtin_bit_cast(bf16x8, v);
-        }
-        constexpr int kR2 = kTileA1;
-        f32x4 acc[kR2];
-#pragma unroll
-        for (int r = 0; r < kR2; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const int nn = min(n, kTileA2 - 1);
-        const uint4* plane = s_buf + (g & 1) * kHidPlane + nn;
-#pragma unroll
-        for (int d0 = 0; d0 < 3; ++d0) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int d2 = h == 0 ? (g >> 1) : 2;
-                const uint4* rowp = plane + (o0 + d0) * he1 * kRow + d2;
-#pragma unroll
-                for (int y = 0; y < kR2 + 2; ++y) {
-                    const uint4 bv = rowp[y * kRow];
-#pragma unroll
-                    for (int d1 = 0; d1 < 3; ++d1) {
-                        const int r = y - d1;
-                        if (r >= 0 && r < kR2) acc[r] = mfma(W2[(d0 * 3 + d1) * 2 + h], bv, acc[r]);
-                    }
-                }
-            }
-        }
-        // outputs 0..2 of cell (o0, r, n) (tile axes) to the correction, in
-        // box order: ycorr[component][ostart[b] + box-local C index]
-        int ext[3], bst[3];
-        for (int q = 0; q < 3; ++q) ext[q] = B.o1[q] - B.o0[q];
-        bst[0] = ext[1] * ext[2]; bst[1] = ext[2]; bst[2] = 1;
-        const int l0 = o[p0] - B.o0[p0] + o0, l1 = o[p1] - B.o0[p1], l2 = o[p2] - B.o0[p2] + n;
-        if (g == 0 && n < kTileA2 && l0 < ext[p0] && l2 < ext[p2]) {
-            const int out = tl.ostart[a.nbox];
-            float* yc = ybuf + tl.ostart[b] + l0 * bst[p0] + l1 * bst[p1] + l2 * bst[p2];
-#pragma unroll
-            for (int r = 0; r < kR2; ++r) {
-                if (l1 + r < ext[p1]) {
-                    yc[r * bst[p1]] = acc[r][0];
-                    yc[out + r * bst[p1]] = acc[r][1];
-                    yc[2 * out + r * bst[p1]] = acc[r][2];
-                }
+        for (int r = 0; r < kCR; ++r) {
+            if (l1 + r < ext[p1]) {
+                const f32x4 v = acc[r] + part[r * 64];
+                yc[(size_t)r * gst[p1]] = v[0];
+                yc[N + (size_t)r * gst[p1]] = v[1];
+                yc[2 * N + (size_t)r * gst[p1]] = v[2];
             }
         }
     }
 }
 
 // E += correction on the shell cells, one thread per cell in box order
-// (z fastest within a box), the order of the correction's scratch.
+// (z fastest within a box), the correction read at the cell's grid offset.
 template <class T>
 __global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, const float* ycorr) {
     int cell = blockIdx.x * 256 + threadIdx.x;
     const int out = tl.ostart[a.nbox];
     if (cell >= out) return;
-    const float y0 = ycorr[cell], y1 = ycorr[out + cell], y2 = ycorr[2 * out + cell];
     int b = 0;
     while (b < a.nbox - 1 && cell >= tl.ostart[b + 1]) ++b;
     cell -= tl.ostart[b];
@@ -424,11 +385,10 @@ __global__ __launch_bounds__(256) void shell_add(ShellArgs<T> a, ShellTiles tl, 
     const int lz = cell % nz, ly = (cell / nz) % ny, lx = cell / (nz * ny);
     const size_t N = (size_t)a.sx * a.sy * a.sz;
     const size_t gi = ((size_t)(B.o0[0] + lx) * a.sy + (B.o0[1] + ly)) * a.sz + (B.o0[2] + lz);
-    a.Eout[gi] += (T)y0;
-    a.Eout[N + gi] += (T)y1;
-    a.Eout[2 * N + gi] += (T)y2;
+    a.Eout[gi] += (T)ycorr[gi];
+    a.Eout[N + gi] += (T)ycorr[N + gi];
+    a.Eout[2 * N + gi] += (T)ycorr[2 * N + gi];
 }
-
 
 // ---- training with MFMA (v_mfma_f32_16x16x4_f32) --------------------------
 //
@@ -711,7 +671,7 @@ __global__ __launch_bounds__(256) void shell_wgrad2_mfma(ShellArgs<T> a) {
 }
 
 struct ShellPlan {
-    int nbox;
+    int nbox, T;
     ShellBox box[6];
     int hcells, out_cells;
     ShellTiles tiles;
@@ -721,6 +681,7 @@ struct ShellPlan {
 
 ShellPlan make_shell(const Plan& P, int T) {
     ShellPlan S;
+    S.T = T;
     const int s[3] = {P.sx, P.sy, P.sz};
     int nb = 0;
     for (int r = 0; r < 3; ++r) {
@@ -750,7 +711,7 @@ ShellPlan make_shell(const Plan& P, int T) {
     S.hcells = off;
     S.out_cells = outc;
     // MFMA tiles: a0 = the box's thin axis (T), a2 = z for x and y faces, y
-    // for z faces (14), a1 = the remaining axis (8)
+    // for z faces (16), a1 = the remaining axis (16)
     int ts = 0;
     for (int b = 0; b < nb; ++b) {
         const int r = b / 2;
@@ -775,7 +736,7 @@ ShellPlan make_shell(const Plan& P, int T) {
     S.tiles.ostart[nb] = oc;
     S.tiles.tstart[nb] = ts;
     S.ntiles = ts;
-    S.scratch_floats = std::max((size_t)S.hcells * kHid, (size_t)3 * S.out_cells);
+    S.scratch_floats = std::max((size_t)S.hcells * kHid, (size_t)3 * P.N);
     return S;
 }
 
@@ -796,16 +757,17 @@ int check_shell(const Plan& P, int T) {
     return 0;
 }
 
-// Scratch layout: [0, scratch_floats) the correction, [3][out_cells] FP32
-// in box order, between its computation and its addition to E (the backward
-// keeps hidden values there), then the packed weight fragments.
+// Scratch layout: [0, scratch_floats) the correction, [3][N] FP32 on the
+// grid (written and read on the shell cells only), between its computation
+// and its addition to E (the backward keeps hidden values there), then the
+// packed weight fragments.
 uint16_t* packed_weights(const ShellPlan& S, float* scratch) {
     return reinterpret_cast<uint16_t*>(scratch + S.scratch_floats);
 }
 
 void launch_shell_pack(const ShellPlan& S, const uint16_t* w, float* scratch, hipStream_t st) {
-    hipLaunchKernelGGL(shell_pack, dim3((S.nbox * kPackBox + 255) / 256), dim3(256), 0, st, w, S.nbox, S.tiles,
-                       packed_weights(S, scratch));
+    hipLaunchKernelGGL(shell_compose, dim3((S.nbox * kPackBox + 255) / 256), dim3(256), 0, st, w, S.nbox, S.T,
+                       S.tiles, packed_weights(S, scratch));
 }
 
 // The correction (weights already packed in scratch) into the scratch, and
@@ -819,14 +781,14 @@ ShellArgs<T> shell_args(const Plan& P, const ShellPlan& S, const uint16_t* w, co
     return a;
 }
 
-void launch_shell_mfma(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H,
+void launch_shell_conv(const Plan& P, const ShellPlan& S, const uint16_t* w, const void* E, const void* H,
                        float* scratch, hipStream_t st) {
     const uint16_t* pk = packed_weights(S, scratch);
     if (P.d.dtype == FDTD_F32)
-        hipLaunchKernelGGL(shell_mfma<float>, dim3(S.ntiles), dim3(256), 0, st, shell_args<float>(P, S, w, E, H),
+        hipLaunchKernelGGL(shell_conv<float>, dim3(S.ntiles), dim3(256), 0, st, shell_args<float>(P, S, w, E, H),
                            S.tiles, pk, scratch);
     else
-        hipLaunchKernelGGL(shell_mfma<double>, dim3(S.ntiles), dim3(256), 0, st, shell_args<double>(P, S, w, E, H),
+        hipLaunchKernelGGL(shell_conv<double>, dim3(S.ntiles), dim3(256), 0, st, shell_args<double>(P, S, w, E, H),
                            S.tiles, pk, scratch);
 }
 
@@ -862,41 +824,41 @@ int fdtd_shell_apply(const void* plan, int32_t thickness, const uint16_t* w, voi
     const ShellPlan S = make_shell(P, thickness);
     hipStream_t st = (hipStream_t)stream;
     launch_shell_pack(S, w, hidden, st);
-    launch_shell_mfma(P, S, w, E, H, hidden, st);
+    launch_shell_conv(P, S, w, E, H, hidden, st);
     launch_shell_add(P, S, E, hidden, st);
     return check_launch("fdtd_shell_apply");
 }
 
 int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness, const uint16_t* w, float* hidden,
-                      int32_t n, int64_t src_index, const void* src_wave, void* stream) {
+                      int32_t n, int32_t period, int64_t src_index, const void* src_wave, void* stream,
+                      int32_t* moved) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
     if (int rc = check_shell(P, thickness)) return rc;
     if (!w || !hidden) return fail(2, "w and hidden are required");
     if (n < 0) return fail(3, "n = %d < 0", n);
+    if (period != 1 && period != 2 && period != 4) return fail(3, "correction period %d is not 1, 2 or 4", period);
+    if (n % period) return fail(3, "n = %d is not a multiple of the correction period %d", n, period);
     if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
     if (P.d.poynting == FDTD_POYNTING_TWO_PASS) return fail(3, "two-pass Poynting is not supported with the shell");
     if (P.d.bloch_mask) return fail(3, "the shell correction is for real fields (no Bloch boundaries)");
     if (int rc = check_fields(P, f, true)) return rc;
+    if (moved) *moved = 0;
     const ShellPlan S = make_shell(P, thickness);
     hipStream_t st = (hipStream_t)stream;
     fdtd_fields cur = *f;
     const size_t esz = P.d.dtype == FDTD_F32 ? 4 : 8;
     const bool swap = (f->H_out && f->H_out != f->H);
     if (n > 0) launch_shell_pack(S, w, hidden, st);
-    // step k: E update (+ the correction of step k-1, added before the
-    // update) and source; correction y(E, H) into the scratch; H update from
-    // the uncorrected E.  The last step's correction is added at the end.
     const auto amp_of = [&](int k) -> const void* {
         return src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
     };
-    const auto half_e = [&](int k) {
-        const float* pending = k > 0 ? hidden : nullptr;
+    const auto half_e = [&](int k, bool pending) {
+        const float* y = pending ? hidden : nullptr;
         return (P.d.dtype == FDTD_F32)
-                   ? launch_e<float>(P, &cur, st, (const float*)amp_of(k), src_index, pending, thickness, S.out_cells)
-                   : launch_e<double>(P, &cur, st, (const double*)amp_of(k), src_index, pending, thickness,
-                                      S.out_cells);
+                   ? launch_e<float>(P, &cur, st, (const float*)amp_of(k), src_index, y, thickness, P.N)
+                   : launch_e<double>(P, &cur, st, (const double*)amp_of(k), src_index, y, thickness, P.N);
     };
     const auto half_h = [&]() {
         const int rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
@@ -906,54 +868,90 @@ int fdtd_shell_step_n(const void* plan, const fdtd_fields* f, int32_t thickness,
         }
         return rc;
     };
-    // With a second E buffer and a plan that has the fused pass: E_0 and its
-    // correction, then per step one pass for H_{k-1} + S and E_k (+ the
-    // correction of step k-1) and the correction of step k, then H_{n-1}.
-    // Same order of operations per cell as below; the count of passes is
+    const auto done = [&]() {
+        if (moved)
+            *moved = (cur.E != f->E ? 1 : 0) | (cur.psi_e != f->psi_e ? 2 : 0) | (cur.H != f->H ? 4 : 0);
+        return check_launch("fdtd_shell_step_n");
+    };
+    if (period > 1) {
+        // step k: E update (+ the correction of step k-1 when k-1 is a
+        // multiple of the period) and source; the correction y(E, H) into
+        // the scratch when k is a multiple of the period; H update from the
+        // uncorrected E.  With the two-step pass (E may end in E_out,
+        // reported through *moved): E_0 and its correction, then per pair
+        // of steps k, k+1 one pass (H_{k-1}, E_k with the pending
+        // correction added to E^{k-1} when there is one, H_k, E_{k+1}) and
+        // the correction of E_{k+1} when it is due; the last pass stops
+        // after H_{n-1}.
+        if (n == 0) return done();
+        const bool pass2 = moved && P.d.dtype == FDTD_F32 && !P.d.nonuniform && he_buffers(P, f) &&
+                           he_supported(P) && he2_supported(P) &&
+                           (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
+        if (pass2) {
+            if (int rc = half_e(0, false)) return rc;
+            launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
+            for (int k = 1; k < n; k += 2) {
+                const bool last_e = k + 1 < n;
+                const bool pending = (k - 1) % period == 0;
+                if (int rc = launch_he2(P, &cur, st, (const float*)amp_of(k), src_index, last_e,
+                                        pending ? hidden : nullptr, thickness, P.N))
+                    return rc;
+                std::swap(cur.H, cur.H_out);
+                std::swap(cur.psi_h, cur.psi_h_out);
+                std::swap(cur.E, cur.E_out);
+                std::swap(cur.psi_e, cur.psi_e_out);
+                if (last_e && (k + 1) % period == 0) launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
+            }
+            return done();
+        }
+        for (int k = 0; k < n; ++k) {
+            if (int rc = half_e(k, k > 0 && (k - 1) % period == 0)) return rc;
+            if (k % period == 0) launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
+            if (int rc = half_h()) return rc;
+        }
+        return done();
+    }
+    // period 1: step k: E update (+ the correction of step k-1, added
+    // before the update) and source; correction y(E, H) into the scratch; H
+    // update from the uncorrected E.  The last step's correction is added at
+    // the end.  With a second E buffer and a plan that has the fused pass:
+    // E_0 and its correction, then per step one pass for H_{k-1} + S and E_k
+    // (+ the correction of step k-1) and the correction of step k, then
+    // H_{n-1}.  Same order of operations per cell; the count of passes is
     // kept even so that E ends in f->E.
     if (n >= 2 && he_buffers(P, f) && he_supported(P)) {
-        if (int rc = half_e(0)) return rc;
-        launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+        if (int rc = half_e(0, false)) return rc;
+        launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
         int k = 1;
         if ((n - 1) % 2 == 1) {
             if (int rc = half_h()) return rc;
-            if (int rc = half_e(1)) return rc;
-            launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+            if (int rc = half_e(1, true)) return rc;
+            launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
             k = 2;
         }
         for (; k < n; ++k) {
             const int rc = (P.d.dtype == FDTD_F32)
                                ? launch_he<float>(P, &cur, st, (const float*)amp_of(k), src_index, hidden,
-                                                  thickness, S.out_cells)
+                                                  thickness, P.N)
                                : launch_he<double>(P, &cur, st, (const double*)amp_of(k), src_index, hidden,
-                                                   thickness, S.out_cells);
+                                                   thickness, P.N);
             if (rc) return rc;
             std::swap(cur.H, cur.H_out);
             std::swap(cur.psi_h, cur.psi_h_out);
             std::swap(cur.E, cur.E_out);
-            launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
+            launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
         }
         if (int rc = half_h()) return rc;
         launch_shell_add(P, S, cur.E, hidden, st);
-        return check_launch("fdtd_shell_step_n");
+        return done();
     }
     for (int k = 0; k < n; ++k) {
-        const void* amp = src_wave ? (const char*)src_wave + (size_t)k * esz : nullptr;
-        const float* pending = k > 0 ? hidden : nullptr;
-        int rc = (P.d.dtype == FDTD_F32)
-                     ? launch_e<float>(P, &cur, st, (const float*)amp, src_index, pending, thickness, S.out_cells)
-                     : launch_e<double>(P, &cur, st, (const double*)amp, src_index, pending, thickness, S.out_cells);
-        if (rc) return rc;
-        launch_shell_mfma(P, S, w, cur.E, cur.H, hidden, st);
-        rc = (P.d.dtype == FDTD_F32) ? launch_h<float>(P, &cur, st) : launch_h<double>(P, &cur, st);
-        if (rc) return rc;
-        if (swap) {
-            std::swap(cur.H, cur.H_out);
-            std::swap(cur.psi_h, cur.psi_h_out);
-        }
+        if (int rc = half_e(k, k > 0)) return rc;
+        launch_shell_conv(P, S, w, cur.E, cur.H, hidden, st);
+        if (int rc = half_h()) return rc;
     }
     if (n > 0) launch_shell_add(P, S, cur.E, hidden, st);
-    return check_launch("fdtd_shell_step_n");
+    return done();
 }
 
 int64_t fdtd_shell_partials_elems(void) { return (int64_t)kWgGroups * std::max(kW1, kDepth * kW2); }

@@ -135,11 +135,17 @@ BASE_PML = {"ln_R_per_layer": -2.2, "m": 2.0, "cfs_alpha": 0.15, "epsilon_eff": 
 class ShellNet:
     """Weights of the shell correction: an FP32 master copy (for training)
     and the BF16 copy the kernels read.  ``base_pml`` are the CPML parameters
-    of the T-cell shell (defaults of simulation.py:161-168 otherwise)."""
+    of the T-cell shell (defaults of simulation.py:161-168 otherwise).
+    ``period``: the correction is applied after every step (1) or after
+    every second or fourth step (2, 4: steps 0, period, 2 period, ...),
+    which lets the solver run two steps per pass between corrections."""
 
     def __init__(self, thickness: int = 4, weights: Optional[torch.Tensor] = None, seed: int = 0,
-                 w1_scale: float = 0.05, device="cuda", base_pml: Optional[Dict] = None):
+                 w1_scale: float = 0.05, device="cuda", base_pml: Optional[Dict] = None, period: int = 2):
+        if period not in (1, 2, 4):
+            raise ValueError("correction period %d is not 1, 2 or 4" % period)
         self.T = thickness
+        self.period = period
         self.base_pml = dict(BASE_PML if base_pml is None else base_pml)
         if weights is None:
             g = torch.Generator().manual_seed(seed)
@@ -161,7 +167,7 @@ class ShellNet:
         return [dict(f, **self.base_pml) for f in cpml_faces(self.T)]
 
     def save(self, path: str) -> None:
-        numpy.savez_compressed(path, thickness=self.T, weights=self.master.detach().cpu().numpy(),
+        numpy.savez_compressed(path, thickness=self.T, period=self.period, weights=self.master.detach().cpu().numpy(),
                                base_pml_keys=numpy.array(list(self.base_pml.keys())),
                                base_pml_values=numpy.array(list(self.base_pml.values()), dtype=numpy.float64))
 
@@ -171,19 +177,25 @@ class ShellNet:
             base = {str(k): float(v) for k, v in zip(z["base_pml_keys"], z["base_pml_values"])}
             if z["weights"].size != N_WEIGHTS:
                 raise ValueError("%s holds %d weights, this net has %d" % (path, z["weights"].size, N_WEIGHTS))
-            return cls(int(z["thickness"]), weights=torch.as_tensor(z["weights"]), device=device, base_pml=base)
+            period = int(z["period"]) if "period" in z.files else 1
+            return cls(int(z["thickness"]), weights=torch.as_tensor(z["weights"]), device=device, base_pml=base,
+                       period=period)
 
 
 class ShellRunner:
     """Applies a ShellNet to one Simulation (the thin domain with CPML-T).
 
     One step of the shell solver: E update and source (E_s); H update from
-    E_s; E = E_s + correction(E_s, H before the update).  ``step`` makes one
-    such step call by call, ``step_n`` makes n natively (the correction is
-    then added by the next step's E update)."""
+    E_s; on a corrected step (every step, or steps 0, period, 2 period, ...
+    counted from the runner's creation) E = E_s + correction(E_s, H before
+    the update).  ``step`` makes one such step call by call, ``step_n`` makes
+    n natively (the correction is then added by the next step's E update;
+    with period 2 or 4 each pair of steps runs as one two-step pass where the
+    plan has it)."""
 
     def __init__(self, sim: Simulation, net: ShellNet):
         self.sim, self.net = sim, net
+        self.k = 0                     # steps made (the correction's phase)
         self.lib = sim._lib
         nh = int(self.lib.fdtd_shell_hidden_elems(sim.plan, net.T))
         if nh <= 0:
@@ -202,6 +214,10 @@ class ShellRunner:
                                              ctypes.c_void_p(self.hidden.data_ptr()), self._st()),
                    "fdtd_shell_apply")
 
+    def corrected(self, k: int) -> bool:
+        """Whether step k (counted from the start) gets the correction."""
+        return k % self.net.period == 0
+
     def step(self, src_index: Optional[int] = None, amp=None, keep: Optional[List[torch.Tensor]] = None) -> None:
         """One shell step call by call; ``keep`` receives a copy of E after
         the E update and source (the correction's input)."""
@@ -211,6 +227,12 @@ class ShellRunner:
             sim._E[src_index] += amp
         if keep is not None:
             keep.append(sim._E.clone())
+        corr = self.corrected(self.k)
+        self.k += 1
+        if not corr:
+            sim.update_H([])
+            sim.flush()
+            return
         Ec = sim._E.clone()
         self.apply(Ec, sim._H[sim._h])
         sim.update_H([])
@@ -219,8 +241,14 @@ class ShellRunner:
 
     def step_n(self, n: int, src_index: Optional[int] = None, src_wave: Optional[torch.Tensor] = None) -> None:
         """``n`` shell steps launched natively (fdtd_shell_step_n); source as
-        Simulation.step_n."""
+        Simulation.step_n.  n and the steps made so far must be multiples
+        of the correction period."""
+        P = self.net.period
+        if n % P or self.k % P:
+            raise ValueError("with correction period %d, step_n takes a multiple of %d steps from a multiple of %d"
+                             % (P, P, P))
         sim = self.sim
+        sim.flush()
         wave_ptr = None
         if src_wave is not None:
             wave = src_wave.to(device=sim.device, dtype=sim._tdtype).contiguous()
@@ -229,13 +257,19 @@ class ShellRunner:
             self._wave_keepalive = wave
             wave_ptr = ctypes.c_void_p(wave.data_ptr())
         f = sim._fields()
+        moved = ctypes.c_int32(0)
         _lib.check(self.lib.fdtd_shell_step_n(sim.plan, ctypes.byref(f), self.net.T,
                                               ctypes.c_void_p(self.net.bf16.data_ptr()),
-                                              ctypes.c_void_p(self.hidden.data_ptr()), int(n),
+                                              ctypes.c_void_p(self.hidden.data_ptr()), int(n), self.net.period,
                                               int(src_index if src_index is not None else -1), wave_ptr,
-                                              self._st()), "fdtd_shell_step_n")
-        if len(sim._H) == 2 and n % 2 == 1:
+                                              self._st(), ctypes.byref(moved)), "fdtd_shell_step_n")
+        self.k += n
+        if len(sim._H) == 2 and moved.value & 4:
             sim._h = 1 - sim._h
+        if moved.value & 1:
+            sim._swap_e()
+        if moved.value & 2:
+            sim._psi_e, sim._psi_e_out = sim._psi_e_out, sim._psi_e
 
     def backward(self, E, H, Ebar, Ebar_out, Hbar_out, wgrad) -> None:
         if self.hbar is None:
@@ -331,6 +365,7 @@ class ShellTrainer:
             n0, n1 = s * self.K, min(s * self.K + self.K, sc.steps)
             sim.restore(cps[s])
             Es, Hs, Em = [E.clone()], [sim._H[sim._h].clone()], []
+            runner.k = n0
             for n in range(n0, n1):
                 runner.step(idx, wave[n], keep=Em)
                 Es.append(E.clone())
@@ -338,9 +373,12 @@ class ShellTrainer:
             for n in reversed(range(n0, n1)):
                 k = n - n0
                 # step n: Em = Estep(E_k, H_k) + source; H_k+1 = Hstep(H_k, Em);
-                # E_k+1 = Em + y(Em, H_k).  Eb: adjoint of E_k+1 (+ the loss).
+                # E_k+1 = Em + y(Em, H_k) on a corrected step, Em otherwise.
+                # Eb: adjoint of E_k+1 (+ the loss).
                 Ev(Eb).add_((2.0 / Z) * (Ev(Es[k + 1]) - wide[n]))
-                Eb_mid.copy_(Eb)                 # adjoint of E_k+1, the correction's output
+                corr = runner.corrected(n)
+                if corr:
+                    Eb_mid.copy_(Eb)             # adjoint of E_k+1, the correction's output
                 b.E_prev, b.E_next = Es[k].data_ptr(), Em[k].data_ptr()
                 b.H_prev, b.H_next = Hs[k].data_ptr(), Hs[k + 1].data_ptr()
                 b.Eb, b.Hb = Eb.data_ptr(), Hb.data_ptr()
@@ -350,7 +388,8 @@ class ShellTrainer:
                 _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint H")
                 # the correction: Eb += its Em part, Hb += its H_k part (the H step's
                 # Jacobian in H is the identity, so adding after its adjoint is exact)
-                runner.backward(Em[k], Hs[k], Eb_mid, Eb, Hb, wgrad)
+                if corr:
+                    runner.backward(Em[k], Hs[k], Eb_mid, Eb, Hb, wgrad)
                 b.parts = 2                      # the E step: Eb -> adjoint of E_k, Hb += its part
                 _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint E")
                 cur = 1 - cur
@@ -411,12 +450,12 @@ def train_scenarios(n: int, seed0: int = 100, interior=(64, 64, 48), steps: int 
 def validation_scenarios(n: int = 4, seed0: int = 500, interior=(128, 128, 96), steps: int = 400):
     """Held-out scenarios for model selection, with their padded references."""
     scs = [make_scenario(seed0 + s, interior=interior, steps=steps) for s in range(n)]
-    return [(sc, run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=5)) for sc in scs]
+    return [(sc, run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=10)) for sc in scs]
 
 
 def validation_db(net: ShellNet, val) -> float:
     """Mean reflection (dB vs the padded runs) of the net on ``val``."""
-    return float(numpy.mean([reflection_db(run_learned(sc, net, sample_every=5), wide) for sc, wide in val]))
+    return float(numpy.mean([reflection_db(run_learned(sc, net, sample_every=10), wide) for sc, wide in val]))
 
 
 def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4,
@@ -466,12 +505,12 @@ def evaluate_main(net: ShellNet, seeds: Sequence[int], interior=(256, 256, 128),
     base = ShellNet(net.T, weights=torch.zeros_like(net.master), base_pml=net.base_pml)
     for sd in seeds:
         sc = make_scenario(sd, interior=interior, steps=steps)
-        wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=5)
+        wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=10)
         row = {"seed": sd, "wavelength": round(sc.wavelength, 1), "src": list(sc.src)}
-        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=5), wide)
-        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=5), wide)
+        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=10), wide)
+        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=10), wide)
         for T in (4, 8, 16):
-            row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=5), wide)
+            row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=10), wide)
         rows.append(row)
         log(json.dumps(row))
     return rows

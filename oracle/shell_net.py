@@ -7,13 +7,19 @@ of T cells per face, split into six boxes (x faces first, then y, then z),
   E_out = E + y on the shell cells,
 with t over the 27 offsets of a periodic 3x3x3 stencil, weights in BF16, and
 d(i) the depth of i: its index along the box's thin axis (0..T-1; four
-output heads are stored whatever T).
+output heads are stored whatever T).  The net has no activation, so per box
+and depth the two layers are one 5x5x5 convolution:
+  Wc[b][d][o][c][s] = sum_k sum_u W2[b][d][o][k][u] W1[b][k][c][s - u]
+  y_o(i) = sum_{c,s} Wc[b][d(i)][o][c][s] x_c(i + s).
 
-``forward(..., exact_f32=True)`` reproduces the GPU arithmetic: inputs and
-hidden values rounded to BF16, products and sums in float32, in the GPU's
-summation order.  ``forward(..., exact_f32=False)`` and ``backward`` work in
-float64 without rounding (the backward treats the BF16 roundings as identity,
-as the GPU backward does).
+``forward(..., exact_f32=True)`` reproduces the GPU arithmetic: the composed
+weights summed in float32 in the GPU's order (k outer, u inner) and rounded
+once to BF16, inputs rounded to BF16, products and sums in float32 (the
+GPU's matrix cores sum in another order).  ``forward(..., exact_f32=False)``
+and ``backward`` work in float64 on the two layers without rounding (the
+backward treats the BF16 roundings as identity, as the GPU backward does);
+``forward_composed`` is the float64 composed form, which the tests hold
+against the two-layer form.
 """
 from __future__ import annotations
 
@@ -85,15 +91,50 @@ def _shift(a, d):
     return numpy.roll(a, (-d[0], -d[1], -d[2]), axis=(-3, -2, -1))
 
 
+def compose(w, exact_f32=True):
+    """Wc[b][d][o][c][s0][s1][s2], offsets s - 2 along the grid axes: the
+    two layers of each box and depth as one 5x5x5 kernel.  exact_f32: float32
+    sums in the GPU's order (k outer, u inner; the products of BF16 values are
+    exact), rounded to BF16 once; else float64."""
+    W1, W2 = split_weights(w)
+    dt = numpy.float32 if exact_f32 else numpy.float64
+    W1 = W1.astype(dt)
+    W2 = W2.astype(dt)
+    Wc = numpy.zeros((6, N_DEPTH, N_OUT, N_IN, 5, 5, 5), dt)
+    offs = offsets()
+    for k in range(N_HID):
+        for u, du in enumerate(offs):
+            a = W2[:, :, :, k, u][:, :, :, None]             # (b, d, o, 1)
+            for v, dv in enumerate(offs):
+                s = (du[0] + dv[0] + 2, du[1] + dv[1] + 2, du[2] + dv[2] + 2)
+                Wc[:, :, :, :, s[0], s[1], s[2]] += a * W1[:, None, None, k, :, v]
+    return bf16_round(Wc) if exact_f32 else Wc
+
+
+def _composed(x, Wc, shape, T, dt):
+    y = numpy.zeros((3,) + shape, dt)
+    masks = [(b, d, m) for b, box in enumerate(boxes(shape, T)) for d, m in enumerate(depth_masks(shape, box, b))
+             if m.any()]
+    for s0 in range(5):
+        for s1 in range(5):
+            for s2 in range(5):
+                xs = _shift(x, (s0 - 2, s1 - 2, s2 - 2))
+                for b, d, m in masks:
+                    y[:, m] += (Wc[b, d, :, :, s0, s1, s2] @ xs[:, m]).astype(dt)
+    return y
+
+
 def forward(E, H, w, T, exact_f32=True):
     """E, H: (3, sx, sy, sz); w: flat weights already rounded to BF16 values
     (float).  Returns (E_out, y) with y the correction on the shell."""
     shape = E.shape[1:]
-    W1, W2 = split_weights(w)
-    dt = numpy.float32 if exact_f32 else numpy.float64
-    x = numpy.concatenate([E, H]).astype(dt)
     if exact_f32:
-        x = bf16_round(x)
+        x = bf16_round(numpy.concatenate([E, H]))
+        y = _composed(x, compose(w, True), shape, T, numpy.float32)
+        return (E.astype(numpy.float32) + y).astype(numpy.float32), y
+    W1, W2 = split_weights(w)
+    dt = numpy.float64
+    x = numpy.concatenate([E, H]).astype(dt)
     xs = [_shift(x, d) for d in offsets()]
     y = numpy.zeros((3,) + shape, dt)
     for b, box in enumerate(boxes(shape, T)):
@@ -104,8 +145,6 @@ def forward(E, H, w, T, exact_f32=True):
                 for c in range(N_IN):
                     acc = acc + dt(W1[b, k, c, t]) * xs[t][c]
             h[k] = acc
-        if exact_f32:
-            h = bf16_round(h)
         hs = [_shift(h, d) for d in offsets()]
         for dep, m in enumerate(depth_masks(shape, box, b)):
             if not m.any():
@@ -118,6 +157,13 @@ def forward(E, H, w, T, exact_f32=True):
             y[:, m] = yb[:, m]
     Eout = (E.astype(dt) + y).astype(dt)
     return Eout, y
+
+
+def forward_composed(E, H, w, T):
+    """float64 correction through the composed 5x5x5 kernels (no rounding)."""
+    shape = E.shape[1:]
+    x = numpy.concatenate([E, H]).astype(numpy.float64)
+    return _composed(x, compose(w, False), shape, T, numpy.float64)
 
 
 def backward(E, H, w, T, Ebar):
