@@ -232,7 +232,7 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
     timed("learned4", 4, net.faces(), net)
     timed("cpml4_base", 4, net.faces())
     refl = L.evaluate_main(net, seeds, interior=interior, steps=600, log=lambda s: print(s, file=sys.stderr))
-    out = {"speed": rows, "reflection_db": refl,
+    out = {"speed": rows, "reflection_db": refl, "correction_period": net.period,
            "learned4_vs_cpml16_interior_speedup": rows["learned4"]["interior_gcell_s"] / rows["cpml16"]["interior_gcell_s"]}
     print(json.dumps(out), flush=True)
     return out
@@ -278,14 +278,19 @@ def learned_summary(steps: int = 100, warmup: int = 20, seeds=(1000, 1001, 1002,
     refl = []
     for sd in seeds:
         scr = L.make_scenario(sd, interior=interior, steps=refl_steps)
-        wide = L.run_interior(scr, L.WIDE_PAD, L.cpml_faces(L.WIDE_PAD), sample_every=10)
+        wide = L.run_interior(scr, L.WIDE_PAD, L.cpml_faces(L.WIDE_PAD), sample_every=4)
         refl.append({"seed": sd, "wavelength": round(scr.wavelength, 1),
-                     "learned4": L.reflection_db(L.run_learned(scr, net, sample_every=10), wide),
-                     "cpml4_base": L.reflection_db(L.run_learned(scr, base, sample_every=10), wide),
-                     "cpml16": L.reflection_db(L.run_interior(scr, 16, L.cpml_faces(16), sample_every=10), wide)})
+                     "learned4": L.reflection_db(L.run_learned(scr, net, sample_every=4), wide),
+                     "cpml4_base": L.reflection_db(L.run_learned(scr, base, sample_every=4), wide),
+                     "cpml16": L.reflection_db(L.run_interior(scr, 16, L.cpml_faces(16), sample_every=4), wide)})
         del wide
         torch.cuda.empty_cache()
     out = {"interior": list(interior), "timed_steps": steps, "speed": speed,
+           "correction_period": net.period,
+           "schedule": ("the correction every %d steps between two-step passes (he2_march, the first after a "
+                        "correction adding it), shell_conv (composed 5x5x5 MFMA convolution) after the pass that "
+                        "reaches a corrected step" % net.period) if net.period > 1 else
+                       "one fused pass per step and the correction after every step",
            "learned4_vs_cpml16_interior_speedup": ratio,
            "reflection_db": refl, "reflection_steps": refl_steps,
            "worst_learned4_db": max(r["learned4"] for r in refl),

@@ -81,7 +81,7 @@ def cpml_faces(T: int) -> List[Dict]:
     return [{"axis": a, "polarity": p, "thickness": T} for a in "xyz" for p in "np"]
 
 
-def run_interior(sc: Scenario, pad: int, pmls: List[Dict], sample_every: int = 10,
+def run_interior(sc: Scenario, pad: int, pmls: List[Dict], sample_every: int = 4,
                  correction=None, float_type=numpy.float32) -> torch.Tensor:
     """Runs the padded scenario; returns the interior E every
     ``sample_every`` steps as a (samples, 3, ix, iy, iz) tensor.
@@ -285,7 +285,7 @@ class ShellRunner:
 import ctypes  # noqa: E402
 
 
-def run_learned(sc: Scenario, net: ShellNet, sample_every: int = 10, float_type=numpy.float32) -> torch.Tensor:
+def run_learned(sc: Scenario, net: ShellNet, sample_every: int = 4, float_type=numpy.float32) -> torch.Tensor:
     """The scenario in the thin domain (CPML-T shell + the net's correction);
     interior E every ``sample_every`` steps."""
     pad = net.T
@@ -298,6 +298,37 @@ def run_learned(sc: Scenario, net: ShellNet, sample_every: int = 10, float_type=
         runner.step_n(sample_every, idx, wave[n0:n0 + sample_every])
         out.append(sim._E.view(3, *shape)[(slice(None),) + sl].clone())
     return torch.stack(out)
+
+
+def stability_trace(net: ShellNet, steps: int = 12000, every: int = 400, interior=(48, 48, 48),
+                    seed: int = 7) -> List[float]:
+    """Interior energy every ``every`` steps of the thin domain (CPML-T shell
+    and the net) after a pulse with no static part (a derivative of a
+    Gaussian in a random component at the centre), so the energy of a stable
+    absorber keeps falling towards round-off; a correction that feeds a
+    growing mode shows as energy that rises again."""
+    rng = numpy.random.default_rng(seed)
+    sc = Scenario(tuple(interior), numpy.ones(interior), (int(rng.integers(0, 3)),) + tuple(s // 2 for s in interior),
+                  16.0, steps)
+    shape, eps, idx, sl = embed(sc, net.T)
+    sim = Simulation(eps, net.faces(), dt=sc.dt, do_poynting=False)
+    runner = ShellRunner(sim, net)
+    n = numpy.arange(steps, dtype=numpy.float64)
+    w, n0 = 6.0, 30.0
+    wave = torch.as_tensor(-(n - n0) / w * numpy.exp(-0.5 * ((n - n0) / w) ** 2), dtype=sim._tdtype,
+                           device=sim.device)
+    out = []
+    for k0 in range(0, steps - every + 1, every):
+        runner.step_n(every, idx, wave[k0:k0 + every])
+        out.append(float(torch.sum(sim._E.view(3, *shape)[(slice(None),) + sl].double() ** 2)))
+    return out
+
+
+def is_stable(trace: Sequence[float]) -> bool:
+    """A stability trace whose last value is no higher than (5 % above) the
+    lowest of its first half: over the second half nothing grows."""
+    half = len(trace) // 2
+    return trace[-1] <= 1.05 * min(trace[:half + 1])
 
 
 class ShellTrainer:
@@ -320,6 +351,8 @@ class ShellTrainer:
         self.v = torch.zeros_like(net.master)
         self.t = 0
         self.next_case = 0                     # scenarios are taken in turn across train() calls
+        self.stab = None                       # the source-free run of the stability term (stability_setup)
+        self.last_stab = None
         self.cases = []
         for sc in scenarios:
             wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=1)
@@ -336,19 +369,40 @@ class ShellTrainer:
         runner = ShellRunner(sim, self.net)
         shape = sim.shape
         wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device=sim.device)
-        E = sim._E
         Ev = lambda t: t.view(3, *shape)[(slice(None),) + sl]
 
+        def per_step(n, E, Eb):
+            # loss term of step n (on E after the step) and its adjoint into Eb
+            if Eb is not None:
+                Ev(Eb).add_((2.0 / Z) * (Ev(E) - wide[n]))
+                return None
+            return torch.sum((Ev(E) - wide[n]).double() ** 2)
+
+        loss, wgrad = self._trajectory(sim, runner, sc.steps, idx, wave, per_step=per_step)
+        return loss / Z, wgrad
+
+    def _trajectory(self, sim, runner, n_steps, idx, wave, per_step=None, final_w: float = 0.0):
+        """Runs ``n_steps`` shell steps from the simulation's state (source
+        idx, wave if given), with checkpoints every K steps, and reverses
+        them: the gradient of sum_n per_step(n, E after step n) + final_w
+        (|E_n|^2 + |H_n|^2) with respect to the weights."""
+        E = sim._E
         cps, loss = [], torch.zeros((), dtype=torch.float64, device=sim.device)
-        for n in range(sc.steps):
+        runner.k = 0
+        for n in range(n_steps):
             if n % self.K == 0:
                 cps.append(sim.checkpoint())
-            runner.step(idx, wave[n])
-            loss += torch.sum((Ev(E) - wide[n]).double() ** 2)
-        # reverse
+            runner.step(idx, wave[n] if wave is not None else None)
+            if per_step is not None:
+                loss += per_step(n, E, None)
         N3 = 3 * sim.N
         Eb = torch.zeros(N3, dtype=torch.float32, device=sim.device)
         Hb = torch.zeros_like(Eb)
+        if final_w:
+            Hn = sim._H[sim._h]
+            loss += final_w * (torch.sum(E.double() ** 2) + torch.sum(Hn.double() ** 2))
+            Eb.add_(E, alpha=2.0 * final_w)
+            Hb.add_(Hn, alpha=2.0 * final_w)
         Eb_mid = torch.zeros_like(Eb)
         npsi = max(sim.psi_elems, 1)
         pe = [torch.zeros(npsi, dtype=torch.float32, device=sim.device) for _ in range(2)]
@@ -362,20 +416,21 @@ class ShellTrainer:
         st = ctypes.c_void_p(torch.cuda.current_stream(sim.device).cuda_stream)
         cur = 0
         for s in reversed(range(len(cps))):
-            n0, n1 = s * self.K, min(s * self.K + self.K, sc.steps)
+            n0, n1 = s * self.K, min(s * self.K + self.K, n_steps)
             sim.restore(cps[s])
             Es, Hs, Em = [E.clone()], [sim._H[sim._h].clone()], []
             runner.k = n0
             for n in range(n0, n1):
-                runner.step(idx, wave[n], keep=Em)
+                runner.step(idx, wave[n] if wave is not None else None, keep=Em)
                 Es.append(E.clone())
                 Hs.append(sim._H[sim._h].clone())
             for n in reversed(range(n0, n1)):
                 k = n - n0
                 # step n: Em = Estep(E_k, H_k) + source; H_k+1 = Hstep(H_k, Em);
                 # E_k+1 = Em + y(Em, H_k) on a corrected step, Em otherwise.
-                # Eb: adjoint of E_k+1 (+ the loss).
-                Ev(Eb).add_((2.0 / Z) * (Ev(Es[k + 1]) - wide[n]))
+                # Eb: adjoint of E_k+1 (+ the loss), Hb: adjoint of H_k+1.
+                if per_step is not None:
+                    per_step(n, Es[k + 1], Eb)
                 corr = runner.corrected(n)
                 if corr:
                     Eb_mid.copy_(Eb)             # adjoint of E_k+1, the correction's output
@@ -394,7 +449,82 @@ class ShellTrainer:
                 _lib.check(sim._lib.fdtd_adjoint_step(sim.plan, ctypes.byref(b), st), "adjoint E")
                 cur = 1 - cur
             del Es, Hs, Em
-        return float(loss) / Z, wgrad
+        return float(loss), wgrad
+
+    def stability_setup(self, interior=(48, 48, 48), steps: int = 2000, harvest: int = 6000, seed: int = 7) -> None:
+        """The runs the stability term uses, on the thin domain of an eps = 1
+        interior: a pulse with no static part (a derivative of a Gaussian,
+        over the first ~60 steps, in a seeded component at the centre) run
+        for ``harvest`` steps, whose last state (rescaled to unit energy)
+        starts a source-free run of ``steps`` steps.  After a stable
+        absorber that state is the round-off floor, which stays flat; when
+        the correction feeds a growing mode the harvest has let it dominate,
+        and the short run shows its growth."""
+        rng = numpy.random.default_rng(seed)
+        sc = Scenario(tuple(interior), numpy.ones(interior), (int(rng.integers(0, 3)),) + tuple(s // 2 for s in interior),
+                      16.0, harvest)
+        shape, eps, idx, sl = embed(sc, self.net.T)
+        sim = Simulation(eps, self.net.faces(), dt=sc.dt, do_poynting=False)
+        n = numpy.arange(harvest, dtype=numpy.float64)
+        w, n0 = 6.0, 30.0
+        wave = torch.as_tensor(-(n - n0) / w * numpy.exp(-0.5 * ((n - n0) / w) ** 2), dtype=sim._tdtype,
+                               device=sim.device)
+        self.stab = {"sim": sim, "zero": sim.checkpoint(), "steps": steps, "harvest": harvest, "src": (idx, wave),
+                     "start": None}
+        self.stability_harvest()
+
+    def stability_harvest(self) -> None:
+        """Runs the pulse with the current weights and keeps its last state,
+        rescaled to unit energy, as the stability term's start."""
+        S = self.stab
+        sim = S["sim"]
+        sim.restore(S["zero"])
+        runner = ShellRunner(sim, self.net)
+        idx, wave = S["src"]
+        P = self.net.period
+        chunk = 400 - 400 % P
+        for k0 in range(0, S["harvest"] - S["harvest"] % chunk, chunk):
+            runner.step_n(chunk, idx, wave[k0:k0 + chunk])
+        st = sim.checkpoint()
+        e = float(torch.sum(st["E"].double() ** 2) + torch.sum(st["H"].double() ** 2))
+        scale = 1.0 / math.sqrt(max(e, 1e-300))
+        for v in st.values():
+            v.mul_(scale)
+        S["start"] = st
+
+    def stability_loss_and_grad(self, window: int = 200) -> Tuple[float, torch.Tensor]:
+        """log(A / B) for the source-free run of stability_setup, with A and
+        B the sums of |E|^2 over its last ``window`` steps and over the
+        ``window`` steps before its middle, and the gradient of that
+        logarithm: the growth of the run's second half, whose gradient for
+        a growing mode is its growth rate's, whatever the mode's start."""
+        S = self.stab
+        sim, steps = S["sim"], S["steps"]
+        sim.restore(S["start"])
+        runner = ShellRunner(sim, self.net)
+        late0, mid0 = steps - window, steps // 2 - window
+        acc = {"A": torch.zeros((), dtype=torch.float64, device=sim.device), "B": None}
+        acc["B"] = torch.zeros_like(acc["A"])
+        scale = {}
+
+        def per_step(n, E, Eb):
+            late, mid = n >= late0, mid0 <= n < mid0 + window
+            if Eb is None:
+                if late:
+                    acc["A"] += torch.sum(E.double() ** 2)
+                elif mid:
+                    acc["B"] += torch.sum(E.double() ** 2)
+                return torch.zeros((), dtype=torch.float64, device=sim.device)
+            if not scale:
+                scale["A"], scale["B"] = float(acc["A"]), float(acc["B"])
+            if late:
+                Eb.add_(E, alpha=2.0 / scale["A"])
+            elif mid:
+                Eb.add_(E, alpha=-2.0 / scale["B"])
+            return None
+
+        _, g = self._trajectory(sim, runner, steps, None, None, per_step=per_step)
+        return math.log(scale["A"] / scale["B"]), g
 
     def adam(self, g: torch.Tensor) -> None:
         b1, b2 = self.betas
@@ -407,9 +537,12 @@ class ShellTrainer:
         self.net.sync()
 
     def train(self, iterations: int, log: Optional[Callable[[int, float], None]] = None,
-              decay_every: int = 0, decay: float = 0.5, batch: int = 1) -> List[float]:
+              decay_every: int = 0, decay: float = 0.5, batch: int = 1, stab_weight: float = 0.0,
+              stab_every: int = 1) -> List[float]:
         """``iterations`` Adam steps, each on the mean loss and gradient of
-        ``batch`` scenarios taken in turn; returns the mean losses."""
+        ``batch`` scenarios taken in turn, plus ``stab_weight`` times the
+        stability term (stability_setup) when set; returns the mean
+        losses."""
         hist = []
         lr0 = self.lr
         for it in range(iterations):
@@ -421,6 +554,12 @@ class ShellTrainer:
                 self.next_case += 1
                 Ls += L / batch
                 gs = g / batch if gs is None else gs.add_(g, alpha=1.0 / batch)
+            if stab_weight and self.stab is not None and self.t % stab_every == 0:
+                # hinge: only a second half that grows by more than 2 % pushes back
+                Lst, gst = self.stability_loss_and_grad()
+                self.last_stab = Lst
+                if Lst > math.log(1.02):
+                    gs.add_(gst, alpha=stab_weight * stab_every)
             self.adam(gs)
             hist.append(Ls)
             if log:
@@ -450,25 +589,49 @@ def train_scenarios(n: int, seed0: int = 100, interior=(64, 64, 48), steps: int 
 def validation_scenarios(n: int = 4, seed0: int = 500, interior=(128, 128, 96), steps: int = 400):
     """Held-out scenarios for model selection, with their padded references."""
     scs = [make_scenario(seed0 + s, interior=interior, steps=steps) for s in range(n)]
-    return [(sc, run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=10)) for sc in scs]
+    return [(sc, run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=4)) for sc in scs]
 
 
 def validation_db(net: ShellNet, val) -> float:
     """Mean reflection (dB vs the padded runs) of the net on ``val``."""
-    return float(numpy.mean([reflection_db(run_learned(sc, net, sample_every=10), wide) for sc, wide in val]))
+    return float(numpy.mean([reflection_db(run_learned(sc, net, sample_every=4), wide) for sc, wide in val]))
 
 
 def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4,
-               val_every: int = 8, interior=(64, 64, 48), steps: int = 300) -> ShellNet:
-    """Train a 4-cell shell net from W2 = 0 (the base CPML-4); every
-    ``val_every`` Adam steps measure the held-out reflection and keep the
-    best weights (the base itself when nothing beats it)."""
-    net = ShellNet(4, seed=1)
+               val_every: int = 8, interior=(64, 64, 48), steps: int = 300, period: int = 2,
+               stab_weight: float = 0.0, stab_every: int = 2, init: Optional[str] = None,
+               harvest: int = 12000, check: int = 24000) -> ShellNet:
+    """Train a 4-cell shell net (correction every ``period`` steps) from
+    W2 = 0 (the base CPML-4); every ``val_every`` Adam steps measure the
+    held-out reflection and keep the best weights whose long source-free
+    run does not grow (stability_trace), the base itself when nothing beats
+    it.  ``stab_weight``: weight of the stability term (every ``stab_every``
+    steps; its pulse run ``harvest`` steps long).  ``check``: length of the
+    stability trace of model selection.  ``init``: start from these weights
+    instead of W2 = 0."""
+    net = ShellNet(4, seed=1, period=period)
+    if init:
+        net0 = ShellNet.load(init)
+        if net0.period != period:
+            raise ValueError("%s has period %d, not %d" % (init, net0.period, period))
+        net.master.copy_(net0.master)
+        net.sync()
     tr = ShellTrainer(net, train_scenarios(n_scen, interior=interior, steps=steps), checkpoint_every=60, lr=lr)
+    if stab_weight:
+        tr.stability_setup(harvest=harvest)
     val = validation_scenarios()
     best_db = validation_db(net, val)
     best = net.master.clone()
-    log("validation: base CPML-4 %.2f dB" % best_db)
+    if init and not is_stable(stability_trace(net, steps=check)):
+        # an unstable start is no candidate: the record starts at its base (W2 = 0)
+        keep = net.master.clone()
+        best[6 * W1_SIZE:] = 0
+        net.master.copy_(best)
+        net.sync()
+        best_db = validation_db(net, val)
+        net.master.copy_(keep)
+        net.sync()
+    log("validation: start %.2f dB" % best_db)
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
 
     def save_best():
@@ -482,14 +645,20 @@ def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, bat
         n = min(val_every, iterations - it)
         lr_it = lr * 0.5 ** (it // decay_every)
         tr.lr = lr_it
-        tr.train(n, batch=batch,
-                 log=lambda k, L: log("it %d loss %.2f dB lr %.1e (%.0f s)" % (it + k, 10 * numpy.log10(L), tr.lr,
-                                                                              time.time() - t0)))
+        if stab_weight:
+            tr.stability_harvest()
+        tr.train(n, batch=batch, stab_weight=stab_weight, stab_every=stab_every,
+                 log=lambda k, L: log("it %d loss %.2f dB stab %s lr %.1e (%.0f s)" % (
+                     it + k, 10 * numpy.log10(L), "%.2e" % tr.last_stab if tr.last_stab is not None else "-",
+                     tr.lr, time.time() - t0)))
         db = validation_db(net, val)
-        if db < best_db:
+        trace = stability_trace(net, steps=check)
+        stable = is_stable(trace)
+        if db < best_db and stable:
             best_db, best = db, net.master.clone()
             save_best()                        # the best so far survives an interrupted run
-        log("validation after %d: %.2f dB (best %.2f)" % (it + n, db, best_db))
+        log("validation after %d: %.2f dB, long run %s (%.1e -> %.1e) (best %.2f)" % (
+            it + n, db, "stable" if stable else "GROWS", trace[len(trace) // 2], trace[-1], best_db))
     net.master.copy_(best)
     net.sync()
     net.save(out)
@@ -505,12 +674,12 @@ def evaluate_main(net: ShellNet, seeds: Sequence[int], interior=(256, 256, 128),
     base = ShellNet(net.T, weights=torch.zeros_like(net.master), base_pml=net.base_pml)
     for sd in seeds:
         sc = make_scenario(sd, interior=interior, steps=steps)
-        wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=10)
+        wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=4)
         row = {"seed": sd, "wavelength": round(sc.wavelength, 1), "src": list(sc.src)}
-        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=10), wide)
-        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=10), wide)
+        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=4), wide)
+        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=4), wide)
         for T in (4, 8, 16):
-            row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=10), wide)
+            row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=4), wide)
         rows.append(row)
         log(json.dumps(row))
     return rows
@@ -525,11 +694,15 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4, help="scenarios per Adam step")
     ap.add_argument("--interior", default="64,64,48", help="training interior (cells)")
     ap.add_argument("--steps", type=int, default=300, help="steps per training scenario")
+    ap.add_argument("--period", type=int, default=2, help="correction every 1, 2 or 4 steps")
+    ap.add_argument("--stab-weight", type=float, default=0.0, help="weight of the stability term")
+    ap.add_argument("--init", default=None, help="start from these weights (same period)")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
     a = ap.parse_args(argv)
     interior = tuple(int(v) for v in a.interior.split(","))
-    net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps)
+    net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps,
+                      period=a.period, stab_weight=a.stab_weight, init=a.init)
            if a.train else ShellNet.load(a.weights))
     if a.evaluate:
         rows = evaluate_main(net, range(1000, 1000 + a.evaluate))
