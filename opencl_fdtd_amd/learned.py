@@ -487,7 +487,10 @@ class ShellTrainer:
             runner.step_n(chunk, idx, wave[k0:k0 + chunk])
         st = sim.checkpoint()
         e = float(torch.sum(st["E"].double() ** 2) + torch.sum(st["H"].double() ** 2))
-        scale = 1.0 / math.sqrt(max(e, 1e-300))
+        if not math.isfinite(e) or e <= 0.0:
+            raise FloatingPointError("the stability run's fields are not finite (energy %r): the correction "
+                                     "grows too fast for the harvest" % e)
+        scale = 1.0 / math.sqrt(e)
         for v in st.values():
             v.mul_(scale)
         S["start"] = st
@@ -697,12 +700,15 @@ def main(argv=None):
     ap.add_argument("--period", type=int, default=2, help="correction every 1, 2 or 4 steps")
     ap.add_argument("--stab-weight", type=float, default=0.0, help="weight of the stability term")
     ap.add_argument("--init", default=None, help="start from these weights (same period)")
+    ap.add_argument("--stab-harvest", type=int, default=12000, help="pulse run before the stability term's run")
+    ap.add_argument("--stab-check", type=int, default=24000, help="stability trace of model selection (steps)")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
     a = ap.parse_args(argv)
     interior = tuple(int(v) for v in a.interior.split(","))
     net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps,
-                      period=a.period, stab_weight=a.stab_weight, init=a.init)
+                      period=a.period, stab_weight=a.stab_weight, init=a.init, harvest=a.stab_harvest,
+                      check=a.stab_check)
            if a.train else ShellNet.load(a.weights))
     if a.evaluate:
         rows = evaluate_main(net, range(1000, 1000 + a.evaluate))

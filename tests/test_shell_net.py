@@ -306,7 +306,7 @@ def test_stability_detectors_agree():
     w = rng.standard_normal(L.N_WEIGHTS)
     n1 = 6 * L.W1_SIZE
     w[:n1] *= 0.05
-    w[n1:] *= 0.01
+    w[n1:] *= 0.001
     for name, wt, grows in (("base", numpy.zeros(L.N_WEIGHTS), False), ("random", w, True)):
         net = L.ShellNet(4, weights=torch.as_tensor(wt, dtype=torch.float32), period=4)
         trace = L.stability_trace(net, steps=8000)
@@ -345,8 +345,8 @@ def test_trained_shell_gain_over_its_base():
     """The shipped net against the method without it (the same CPML-4, zero
     weights, same correction schedule) on unseen 256x256x128 scenarios, 600
     steps: reflection (interior field error over the run, every 4th step, vs
-    the scenario padded by 48 cells of CPML) improves in 3 of 4 scenarios,
-    by at least 0.8 dB on average, and stays at or below -40 dB."""
+    the scenario padded by 48 cells of CPML) improves in every scenario, by
+    at least 2 dB on average, and stays at or below -40 dB."""
     import torch
     from opencl_fdtd_amd import learned as L
     net = L.ShellNet.load(L.WEIGHTS)
@@ -363,8 +363,8 @@ def test_trained_shell_gain_over_its_base():
         print("seed %d: base %.2f dB, learned %.2f dB" % (seed, db_base, db_net))
         assert db_net <= -40.0
         gains.append(db_base - db_net)
-    assert sum(g > 0 for g in gains) >= 3
-    assert float(numpy.mean(gains)) >= 0.8
+    assert min(gains) > 0.0
+    assert float(numpy.mean(gains)) >= 2.0
 
 
 @pytest.mark.gpu
