@@ -54,6 +54,7 @@ void rec_fill(fdtd_launch_record& r, const EArgs<T>& a) {
     r.buf[FDTD_REC_F_IN] = r.buf[FDTD_REC_F_OUT] = a.s[1].E[0];
     r.buf[FDTD_REC_G_IN] = a.s[1].H[0];
     r.buf[FDTD_REC_SRC] = a.src_amp;
+    r.buf[FDTD_REC_PTR0] = a.ycorr[0];      // pending shell correction (null: none)
     r.src_x = a.src_x; r.src_p = a.src_p; r.src_comp = a.src_amp ? a.src_comp : -1;
 }
 
@@ -84,6 +85,7 @@ void rec_fill(fdtd_launch_record& r, const HEArgs<T>& a) {
     r.buf[FDTD_REC_PSIH_OUT] = a.hs.psi_out;
     r.buf[FDTD_REC_S] = a.S.b;
     r.buf[FDTD_REC_SRC] = a.src_amp;
+    r.buf[FDTD_REC_PTR0] = a.ycorr.b;
     r.src_x = a.src_x; r.src_p = a.src_p; r.src_comp = a.src_amp ? a.src_comp : -1;
 }
 
@@ -100,6 +102,7 @@ void rec_fill(fdtd_launch_record& r, const HE2Args<T>& a) {
     r.buf[FDTD_REC_PSIH_OUT] = a.psi_h_out;
     r.buf[FDTD_REC_S] = a.S.b;
     r.buf[FDTD_REC_SRC] = a.src_amp;
+    r.buf[FDTD_REC_PTR0] = a.ycorr.b;
     r.src_x = a.src_x; r.src_p = a.src_p; r.src_comp = a.src_amp ? a.src_comp : -1;
 }
 

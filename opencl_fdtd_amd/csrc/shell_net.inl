@@ -670,6 +670,18 @@ __global__ __launch_bounds__(256) void shell_wgrad2_mfma(ShellArgs<T> a) {
     }
 }
 
+#ifdef FDTD_RECORD_LAUNCHES
+// launch recording of the shell's kernels: the fields the correction reads
+// (E_IN, H_IN) and the E it is added to (E_OUT)
+template <class T>
+void rec_fill(fdtd_launch_record& r, const ShellArgs<T>& a) {
+    r.src_comp = -1;
+    r.buf[FDTD_REC_E_IN] = a.E;
+    r.buf[FDTD_REC_H_IN] = a.H;
+    r.buf[FDTD_REC_E_OUT] = a.Eout;
+}
+#endif
+
 struct ShellPlan {
     int nbox, T;
     ShellBox box[6];

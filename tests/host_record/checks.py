@@ -43,6 +43,10 @@ def _load(path):
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_step_n_observed.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.POINTER(L.Observe), P,
                                          ctypes.POINTER(ctypes.c_int32)]
+    lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P,
+                                      P, ctypes.POINTER(ctypes.c_int32)]
+    lib.fdtd_shell_hidden_elems.argtypes = [P, ctypes.c_int32]
+    lib.fdtd_shell_hidden_elems.restype = ctypes.c_int64
     lib.fdtd_update_he2_preferred.argtypes = [P]
     lib.fdtd_update_he_supported.argtypes = [P]
     lib.fdtd_record_count.restype = ctypes.c_int64
@@ -256,6 +260,91 @@ def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None):
         assert loc["E"] == f.E and loc["psi_e"] == f.psi_e
 
 
+SHELL_KERNELS = ("shell_compose", "shell_conv", "shell_add")
+
+
+def check_shell_schedule(recs, f, n, period, scratch, wave, esz, moved, pingpong):
+    """fdtd_shell_step_n: the half-steps as step_n's (E H E H ..., each
+    launch reading what the previous wrote, the moved bits), and between
+    them the correction: shell_compose first; right after the E half-step
+    of each corrected step k (k a multiple of the period) one shell_conv
+    that reads the newest E and H and writes the scratch; the E update of
+    step k + 1 (an e_march, a he_march, or the first E of a he2_march)
+    carries the scratch as its pending correction, and no other E update
+    does; with period 1 a final shell_add of the scratch into the newest
+    E."""
+    plain = [r for r in recs if _name(r) not in SHELL_KERNELS]
+    check_schedule(plain, f, n, moved, wave, esz, False, pingpong)
+    corrected = lambda k: k % period == 0
+    if n == 0:
+        assert not recs, [_name(r) for r in recs]
+        return
+    assert _name(recs[0]) == "shell_compose", _name(recs[0])
+    loc = {"E": f.E, "H": f.H}
+    n_e = 0                   # E half-steps made so far
+    last = None               # the last half-step made ("E" or "H")
+    convs, adds = [], 0
+    for r in recs[1:]:
+        name, b = _name(r), r.buf
+        if name == "shell_conv":
+            assert last == "E" and corrected(n_e - 1), ("correction after step", n_e - 1, last)
+            assert b[E_IN] == loc["E"] and b[H_IN] == loc["H"], "the correction reads the newest E and H"
+            convs.append(n_e - 1)
+            continue
+        if name == "shell_add":
+            assert period == 1 and r is recs[-1], "shell_add ends a period-1 call"
+            assert b[E_OUT] == loc["E"]
+            adds += 1
+            continue
+        if name == "shell_compose":
+            raise AssertionError("weights composed twice")
+        halves = _halves(r)
+        k = n_e                                  # step of the launch's first E half-step
+        if "E" in halves:
+            want = scratch if (k >= 1 and corrected(k - 1)) else None
+            assert b[PTR0] == want, (name, "step", k, "pending", b[PTR0], want)
+        if name == "h_march":
+            loc["H"] = b[H_OUT]
+        elif name != "e_march":
+            loc["E"], loc["H"] = b[E_OUT], b[H_OUT]
+        n_e += halves.count("E")
+        last = halves[-1]
+    assert convs == [k for k in range(n) if corrected(k)] if period == 1 else \
+        convs == [k for k in range(0, n - 1) if corrected(k)], (convs, n, period)
+    assert adds == (1 if period == 1 else 0), adds
+
+
+def _shell(lib, L, plan, shape, esz, poynting):
+    """fdtd_shell_step_n on fake device addresses: periods 1, 2 and 4, with
+    and without the fused passes' buffers and the moved bits."""
+    T = 4 if min(shape) >= 12 else 2
+    nh = lib.fdtd_shell_hidden_elems(plan, T)
+    assert nh > 0, lib.fdtd_last_error()
+    checked = 0
+    for period, with_out, use_moved in itertools.product((1, 2, 4), (False, True), (False, True)):
+        for n in ((0, 1, 2, 3, 5) if period == 1 else (0, period, 2 * period, 3 * period)):
+            f = L.Fields()
+            addrs = iter(range(0x100000, 0x10000000, 0x100000))
+            for name, _ in L.Fields._fields_:
+                setattr(f, name, next(addrs))
+            f.F = f.G = f.G_out = f.psi_f = f.psi_g = f.psi_g_out = f.F_out = None
+            f.oS = None
+            if not with_out:
+                f.E_out = f.psi_e_out = None
+            w, scratch, wave = 0x71000000, 0x72000000, 0x70000000
+            moved = ctypes.c_int32(-1)
+            rc = lib.fdtd_shell_step_n(plan, ctypes.byref(f), T, w, scratch, n, period, 1, wave, None,
+                                       ctypes.byref(moved) if use_moved else None)
+            assert rc == 0, (shape, n, period, lib.fdtd_last_error())
+            recs = _records(lib)
+            for r in recs:
+                if _name(r) not in SHELL_KERNELS:
+                    check_decomposition(r)
+            check_shell_schedule(recs, f, n, period, scratch, wave, esz, moved.value if use_moved else None, True)
+            checked += 1
+    return checked
+
+
 def _desc(L, shape, faces, dtype, poynting, bloch_mask=0, bloch_mode=0):
     d = L.Desc()
     d.shape[:] = list(shape)
@@ -372,6 +461,8 @@ def run(lib_path: str, quick: bool = False) -> int:
                 check_schedule(recs, f, n, None if homekeep else moved.value, wave, esz, bool(bloch), pingpong)
                 checked += 1
             checked += _observed(lib, L, plan, shape, esz, poynting, bool(bloch), poynting in (0, 1))
+            if not bloch and poynting != 2 and min(shape) >= 8:
+                checked += _shell(lib, L, plan, shape, esz, poynting)
         finally:
             lib.fdtd_plan_destroy(plan)
     return checked
