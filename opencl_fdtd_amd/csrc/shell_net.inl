@@ -1,8 +1,9 @@
 // The learned absorbing shell (part of the fdtd_hip.hip translation unit).
 //
-// A thin shell of T cells on every face runs the CPML-T update; after each E
-// update (and source) a small convolutional net adds a correction to E on the
-// shell cells:
+// A thin shell of T cells on every face runs the CPML-T update; after the E
+// update (and source) of every corrected step (every step, or every second
+// or fourth: the caller's period) a small convolutional net adds a
+// correction to E on the shell cells:
 //   x = (Ex, Ey, Ez, Hx, Hy, Hz)                         6 input channels
 //   h_k(j) = sum_{c,t} W1[k][c][t] x_c(j + t)            16 hidden channels
 //   y_a(i) = sum_{k,t} W2[d(i)][a][k][t] h_k(i + t)      3 outputs
@@ -11,13 +12,15 @@
 // head per depth).
 // t runs over the 27 offsets of a 3x3x3 stencil (periodic wrap, as the step).
 // There is no activation: the correction must be linear in the fields, as
-// Maxwell's equations are.  Weights and activations are BF16 (inputs rounded
-// to BF16), products are exact in FP32 and accumulated in FP32.
+// Maxwell's equations are.  Weights are BF16.  The inference evaluates the
+// two layers as their composed 5x5x5 kernel (composed in FP32, rounded once
+// to BF16) on inputs rounded to BF16, with exact products accumulated in
+// FP32; the backward (training) works in FP32 on the two layers.
 //
 // The shell is split into 6 boxes, one per face (x faces own the edges and
-// corners they touch, then y faces), each with its own weights.  Hidden values
-// live per box on the box grown by one cell, in a caller-provided FP32
-// scratch [cell][16].
+// corners they touch, then y faces), each with its own weights.  For the
+// backward, hidden values live per box on the box grown by one cell, in a
+// caller-provided FP32 scratch [cell][16].
 
 #ifndef FDTD_SHELL_WAVES
 #define FDTD_SHELL_WAVES 3           // shell_conv: minimum waves per SIMD (3 workgroups of 4 waves per CU)
