@@ -541,7 +541,7 @@ class ShellTrainer:
 
     def train(self, iterations: int, log: Optional[Callable[[int, float], None]] = None,
               decay_every: int = 0, decay: float = 0.5, batch: int = 1, stab_weight: float = 0.0,
-              stab_every: int = 1) -> List[float]:
+              stab_every: int = 1, stab_hinge: float = math.log(1.02)) -> List[float]:
         """``iterations`` Adam steps, each on the mean loss and gradient of
         ``batch`` scenarios taken in turn, plus ``stab_weight`` times the
         stability term (stability_setup) when set; returns the mean
@@ -558,10 +558,10 @@ class ShellTrainer:
                 Ls += L / batch
                 gs = g / batch if gs is None else gs.add_(g, alpha=1.0 / batch)
             if stab_weight and self.stab is not None and self.t % stab_every == 0:
-                # hinge: only a second half that grows by more than 2 % pushes back
+                # hinge: only a second half that grows by more than exp(stab_hinge) pushes back
                 Lst, gst = self.stability_loss_and_grad()
                 self.last_stab = Lst
-                if Lst > math.log(1.02):
+                if Lst > stab_hinge:
                     gs.add_(gst, alpha=stab_weight * stab_every)
             self.adam(gs)
             hist.append(Ls)
@@ -603,7 +603,7 @@ def validation_db(net: ShellNet, val) -> float:
 def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4,
                val_every: int = 8, interior=(64, 64, 48), steps: int = 300, period: int = 2,
                stab_weight: float = 0.0, stab_every: int = 2, init: Optional[str] = None,
-               harvest: int = 12000, check: int = 24000) -> ShellNet:
+               harvest: int = 12000, check: int = 24000, stab_hinge: float = math.log(1.02)) -> ShellNet:
     """Train a 4-cell shell net (correction every ``period`` steps) from
     W2 = 0 (the base CPML-4); every ``val_every`` Adam steps measure the
     held-out reflection and keep the best weights whose long source-free
@@ -650,7 +650,7 @@ def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, bat
         tr.lr = lr_it
         if stab_weight:
             tr.stability_harvest()
-        tr.train(n, batch=batch, stab_weight=stab_weight, stab_every=stab_every,
+        tr.train(n, batch=batch, stab_weight=stab_weight, stab_every=stab_every, stab_hinge=stab_hinge,
                  log=lambda k, L: log("it %d loss %.2f dB stab %s lr %.1e (%.0f s)" % (
                      it + k, 10 * numpy.log10(L), "%.2e" % tr.last_stab if tr.last_stab is not None else "-",
                      tr.lr, time.time() - t0)))
@@ -702,13 +702,15 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (same period)")
     ap.add_argument("--stab-harvest", type=int, default=12000, help="pulse run before the stability term's run")
     ap.add_argument("--stab-check", type=int, default=24000, help="stability trace of model selection (steps)")
+    ap.add_argument("--stab-hinge", type=float, default=0.02,
+                    help="growth of the stability run's second half (relative) above which the term acts")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--evaluate", type=int, default=0, help="number of unseen 256x256x128 scenarios")
     a = ap.parse_args(argv)
     interior = tuple(int(v) for v in a.interior.split(","))
     net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps,
                       period=a.period, stab_weight=a.stab_weight, init=a.init, harvest=a.stab_harvest,
-                      check=a.stab_check)
+                      check=a.stab_check, stab_hinge=math.log1p(a.stab_hinge))
            if a.train else ShellNet.load(a.weights))
     if a.evaluate:
         rows = evaluate_main(net, range(1000, 1000 + a.evaluate))

@@ -345,8 +345,9 @@ def test_trained_shell_gain_over_its_base():
     """The shipped net against the method without it (the same CPML-4, zero
     weights, same correction schedule) on unseen 256x256x128 scenarios, 600
     steps: reflection (interior field error over the run, every 4th step, vs
-    the scenario padded by 48 cells of CPML) improves in every scenario, by
-    at least 2 dB on average, and stays at or below -40 dB."""
+    the scenario padded by 48 cells of CPML) improves in every scenario (by
+    more than 0.5 dB), by at least 3 dB on average, and stays at or below
+    -40 dB."""
     import torch
     from opencl_fdtd_amd import learned as L
     net = L.ShellNet.load(L.WEIGHTS)
@@ -363,8 +364,8 @@ def test_trained_shell_gain_over_its_base():
         print("seed %d: base %.2f dB, learned %.2f dB" % (seed, db_base, db_net))
         assert db_net <= -40.0
         gains.append(db_base - db_net)
-    assert min(gains) > 0.0
-    assert float(numpy.mean(gains)) >= 2.0
+    assert min(gains) > 0.5
+    assert float(numpy.mean(gains)) >= 3.0
 
 
 @pytest.mark.gpu
