@@ -343,9 +343,10 @@ class ShellTrainer:
     ``checkpoint_every`` steps recomputed from checkpoints."""
 
     def __init__(self, net: ShellNet, scenarios: Sequence[Scenario], checkpoint_every: int = 60,
-                 lr: float = 1e-3, betas=(0.9, 0.999)):
+                 lr: float = 1e-3, betas=(0.9, 0.999), loss_every: int = 1):
         self.net = net
         self.K = checkpoint_every
+        self.loss_every = loss_every           # the loss compares E after every loss_every-th step
         self.lr, self.betas = lr, betas
         self.m = torch.zeros_like(net.master)
         self.v = torch.zeros_like(net.master)
@@ -355,7 +356,7 @@ class ShellTrainer:
         self.last_stab = None
         self.cases = []
         for sc in scenarios:
-            wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=1)
+            wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=loss_every)
             self.cases.append((sc, wide, float(torch.sum(wide.double() ** 2))))
 
     def _thin(self, sc):
@@ -371,12 +372,17 @@ class ShellTrainer:
         wave = torch.as_tensor(sc.wave(), dtype=torch.float32, device=sim.device)
         Ev = lambda t: t.view(3, *shape)[(slice(None),) + sl]
 
+        le = self.loss_every
+
         def per_step(n, E, Eb):
-            # loss term of step n (on E after the step) and its adjoint into Eb
+            # loss term of step n (on E after the step, every le-th step) and its adjoint into Eb
+            if (n + 1) % le:
+                return None if Eb is not None else torch.zeros((), dtype=torch.float64, device=sim.device)
+            w = wide[(n + 1) // le - 1]
             if Eb is not None:
-                Ev(Eb).add_((2.0 / Z) * (Ev(E) - wide[n]))
+                Ev(Eb).add_((2.0 / Z) * (Ev(E) - w))
                 return None
-            return torch.sum((Ev(E) - wide[n]).double() ** 2)
+            return torch.sum((Ev(E) - w).double() ** 2)
 
         loss, wgrad = self._trajectory(sim, runner, sc.steps, idx, wave, per_step=per_step)
         return loss / Z, wgrad
@@ -603,7 +609,8 @@ def validation_db(net: ShellNet, val) -> float:
 def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, batch: int = 4,
                val_every: int = 8, interior=(64, 64, 48), steps: int = 300, period: int = 2,
                stab_weight: float = 0.0, stab_every: int = 2, init: Optional[str] = None,
-               harvest: int = 12000, check: int = 24000, stab_hinge: float = math.log(1.02)) -> ShellNet:
+               harvest: int = 12000, check: int = 24000, stab_hinge: float = math.log(1.02),
+               loss_every: int = 1) -> ShellNet:
     """Train a 4-cell shell net (correction every ``period`` steps) from
     W2 = 0 (the base CPML-4); every ``val_every`` Adam steps measure the
     held-out reflection and keep the best weights whose long source-free
@@ -619,7 +626,8 @@ def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, bat
             raise ValueError("%s has period %d, not %d" % (init, net0.period, period))
         net.master.copy_(net0.master)
         net.sync()
-    tr = ShellTrainer(net, train_scenarios(n_scen, interior=interior, steps=steps), checkpoint_every=60, lr=lr)
+    tr = ShellTrainer(net, train_scenarios(n_scen, interior=interior, steps=steps), checkpoint_every=60, lr=lr,
+                      loss_every=loss_every)
     if stab_weight:
         tr.stability_setup(harvest=harvest)
     val = validation_scenarios()
@@ -702,6 +710,7 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (same period)")
     ap.add_argument("--stab-harvest", type=int, default=12000, help="pulse run before the stability term's run")
     ap.add_argument("--stab-check", type=int, default=24000, help="stability trace of model selection (steps)")
+    ap.add_argument("--loss-every", type=int, default=1, help="compare E after every n-th step (memory)")
     ap.add_argument("--stab-hinge", type=float, default=0.02,
                     help="growth of the stability run's second half (relative) above which the term acts")
     ap.add_argument("--weights", default=WEIGHTS)
@@ -710,7 +719,7 @@ def main(argv=None):
     interior = tuple(int(v) for v in a.interior.split(","))
     net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps,
                       period=a.period, stab_weight=a.stab_weight, init=a.init, harvest=a.stab_harvest,
-                      check=a.stab_check, stab_hinge=math.log1p(a.stab_hinge))
+                      check=a.stab_check, stab_hinge=math.log1p(a.stab_hinge), loss_every=a.loss_every)
            if a.train else ShellNet.load(a.weights))
     if a.evaluate:
         rows = evaluate_main(net, range(1000, 1000 + a.evaluate))
