@@ -130,6 +130,14 @@ N_WEIGHTS = 6 * (W1_SIZE + N_DEPTH * W2_SIZE)
 # kappa_max = sqrt(6), CFS alpha 0.15), far better at 4 cells than the
 # defaults of simulation.py:161-168.
 BASE_PML = {"ln_R_per_layer": -2.2, "m": 2.0, "cfs_alpha": 0.15, "epsilon_eff": 6.0, "ma": 1}
+# Thinner shells' bases, tuned the same way (mean reflection on the held-out
+# 128 x 128 x 96 scenarios over a grid of ln R, m, CFS alpha and epsilon_eff;
+# for 3 cells the runner-up, whose worst case is better).
+THIN_BASE_PML = {
+    2: {"ln_R_per_layer": -2.2, "m": 1.5, "cfs_alpha": 0.05, "epsilon_eff": 12.0, "ma": 1},
+    3: {"ln_R_per_layer": -2.2, "m": 2.0, "cfs_alpha": 0.15, "epsilon_eff": 12.0, "ma": 1},
+    4: BASE_PML,
+}
 
 
 class ShellNet:
@@ -610,20 +618,21 @@ def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, bat
                val_every: int = 8, interior=(64, 64, 48), steps: int = 300, period: int = 2,
                stab_weight: float = 0.0, stab_every: int = 2, init: Optional[str] = None,
                harvest: int = 12000, check: int = 24000, stab_hinge: float = math.log(1.02),
-               loss_every: int = 1) -> ShellNet:
-    """Train a 4-cell shell net (correction every ``period`` steps) from
-    W2 = 0 (the base CPML-4); every ``val_every`` Adam steps measure the
+               loss_every: int = 1, thickness: int = 4) -> ShellNet:
+    """Train a shell net of ``thickness`` cells (correction every ``period``
+    steps) from W2 = 0 (its base CPML, THIN_BASE_PML); every ``val_every`` Adam steps measure the
     held-out reflection and keep the best weights whose long source-free
     run does not grow (stability_trace), the base itself when nothing beats
     it.  ``stab_weight``: weight of the stability term (every ``stab_every``
     steps; its pulse run ``harvest`` steps long).  ``check``: length of the
     stability trace of model selection.  ``init``: start from these weights
     instead of W2 = 0."""
-    net = ShellNet(4, seed=1, period=period)
+    net = ShellNet(thickness, seed=1, period=period, base_pml=THIN_BASE_PML[thickness])
     if init:
         net0 = ShellNet.load(init)
-        if net0.period != period:
-            raise ValueError("%s has period %d, not %d" % (init, net0.period, period))
+        if net0.period != period or net0.T != thickness:
+            raise ValueError("%s has period %d and thickness %d, not %d and %d" % (
+                init, net0.period, net0.T, period, thickness))
         net.master.copy_(net0.master)
         net.sync()
     tr = ShellTrainer(net, train_scenarios(n_scen, interior=interior, steps=steps), checkpoint_every=60, lr=lr,
@@ -678,18 +687,19 @@ def train_main(iterations: int, lr: float, n_scen: int, out: str, log=print, bat
 
 def evaluate_main(net: ShellNet, seeds: Sequence[int], interior=(256, 256, 128), steps: int = 600,
                   log=print) -> List[Dict]:
-    """Reflection (dB, vs the 48-cell padded run) of CPML-4 with the shell's
-    base parameters (the net zeroed), the learned 4-cell shell, CPML-8 and
-    CPML-16 on unseen scenarios of the given interior."""
+    """Reflection (dB, vs the 48-cell padded run) of CPML-T with the shell's
+    base parameters (the net zeroed: ``base``), the learned T-cell shell
+    (``learned``), CPML-T with the defaults, CPML-8 and CPML-16 on unseen
+    scenarios of the given interior."""
     rows = []
     base = ShellNet(net.T, weights=torch.zeros_like(net.master), base_pml=net.base_pml)
     for sd in seeds:
         sc = make_scenario(sd, interior=interior, steps=steps)
         wide = run_interior(sc, WIDE_PAD, cpml_faces(WIDE_PAD), sample_every=4)
         row = {"seed": sd, "wavelength": round(sc.wavelength, 1), "src": list(sc.src)}
-        row["cpml4_base"] = reflection_db(run_learned(sc, base, sample_every=4), wide)
-        row["learned4"] = reflection_db(run_learned(sc, net, sample_every=4), wide)
-        for T in (4, 8, 16):
+        row["base"] = reflection_db(run_learned(sc, base, sample_every=4), wide)
+        row["learned"] = reflection_db(run_learned(sc, net, sample_every=4), wide)
+        for T in sorted({net.T, 8, 16}):
             row["cpml%d" % T] = reflection_db(run_interior(sc, T, cpml_faces(T), sample_every=4), wide)
         rows.append(row)
         log(json.dumps(row))
@@ -698,7 +708,7 @@ def evaluate_main(net: ShellNet, seeds: Sequence[int], interior=(256, 256, 128),
 
 def main(argv=None):
     import argparse
-    ap = argparse.ArgumentParser(description="train / evaluate the learned 4-cell absorbing shell")
+    ap = argparse.ArgumentParser(description="train / evaluate a learned thin absorbing shell")
     ap.add_argument("--train", type=int, default=0, help="training iterations")
     ap.add_argument("--lr", type=float, default=5e-6)
     ap.add_argument("--scenarios", type=int, default=48)
@@ -706,6 +716,7 @@ def main(argv=None):
     ap.add_argument("--interior", default="64,64,48", help="training interior (cells)")
     ap.add_argument("--steps", type=int, default=300, help="steps per training scenario")
     ap.add_argument("--period", type=int, default=2, help="correction every 1, 2 or 4 steps")
+    ap.add_argument("--thickness", type=int, default=4, choices=sorted(THIN_BASE_PML), help="shell cells")
     ap.add_argument("--stab-weight", type=float, default=0.0, help="weight of the stability term")
     ap.add_argument("--init", default=None, help="start from these weights (same period)")
     ap.add_argument("--stab-harvest", type=int, default=12000, help="pulse run before the stability term's run")
@@ -719,14 +730,16 @@ def main(argv=None):
     interior = tuple(int(v) for v in a.interior.split(","))
     net = (train_main(a.train, a.lr, a.scenarios, a.weights, batch=a.batch, interior=interior, steps=a.steps,
                       period=a.period, stab_weight=a.stab_weight, init=a.init, harvest=a.stab_harvest,
-                      check=a.stab_check, stab_hinge=math.log1p(a.stab_hinge), loss_every=a.loss_every)
+                      check=a.stab_check, stab_hinge=math.log1p(a.stab_hinge), loss_every=a.loss_every,
+                      thickness=a.thickness)
            if a.train else ShellNet.load(a.weights))
     if a.evaluate:
         rows = evaluate_main(net, range(1000, 1000 + a.evaluate))
-        gain = [r["cpml4_base"] - r["learned4"] for r in rows]
-        print(json.dumps({"scenarios": len(rows), "mean_gain_db": float(numpy.mean(gain)),
+        gain = [r["base"] - r["learned"] for r in rows]
+        print(json.dumps({"thickness": net.T, "scenarios": len(rows), "mean_gain_db": float(numpy.mean(gain)),
                           "improved": int(sum(g > 0 for g in gain)),
-                          "worst_learned_db": max(r["learned4"] for r in rows)}), flush=True)
+                          "worst_base_db": max(r["base"] for r in rows),
+                          "worst_learned_db": max(r["learned"] for r in rows)}), flush=True)
 
 
 if __name__ == "__main__":
