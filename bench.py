@@ -579,6 +579,11 @@ def main():
         "config": {"workload": w.name, "shape": list(w.shape), "pml": "CPML-8 on 6 faces",
                    "poynting": w.do_poynting, "cells": w.cells,
                    "steps_per_pass": 2 if two else 1,
+                   "schedule": ("step_n calls chain (fdtd_step_n_chain): the warm-up call leaves its last H "
+                                "half-step pending and the timed call runs it in its first pass, leaving its own "
+                                "last one pending, as update_H defers into update_E; the timed region holds the "
+                                "E and H half-steps of exactly `steps` steps") if sim._defer else
+                               "each step_n call runs E first and H last",
                    "s_stored": ("only for the second H half-step of each two-step pass (the first is not "
                                 "computed); the S after step_n is bit-identical to the per-step loop's")
                                if two else "every H half-step",

@@ -26,6 +26,9 @@
  *   fdtd_step_n        <- n iterations of the caller loop fdtd.py:162-175
  *   fdtd_step_n_moved     (E, F, source, H, G, S) without host round trips
  *   fdtd_step_n_observed  (the same with a plane flux and field copies per step)
+ *   fdtd_step_n_chain     (the same, the H half-step of the loop's last
+ *                         iteration left for the next call, as sim.update_H
+ *                         defers into the next sim.update_E)
  *   fdtd_update_he     <- sim.update_H + sim.update_S of one iteration of that
  *                         loop followed by sim.update_E and the soft source of
  *                         the next, in one pass over the grid
@@ -55,7 +58,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 6
+#define FDTD_ABI_VERSION 7
 #define FDTD_MAX_FACES 12
 
 enum { FDTD_F32 = 0, FDTD_F64 = 1 };
@@ -174,6 +177,25 @@ int fdtd_step_n(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
  * buffer roles accordingly. */
 int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
                       const void* src_wave, void* stream, int32_t* moved);
+
+/* fdtd_step_n_moved (same loop, same results, same *moved bits) chained
+ * over calls.  flags:
+ *   FDTD_CHAIN_H_PENDING  the H half-step of the previous call's last step
+ *                         has not run: it runs first, in one pass with E_0;
+ *   FDTD_CHAIN_DEFER_H    leave the H half-step of the last step (H_{n-1},
+ *                         with S) unrun.
+ * *moved also receives bit 3 (FDTD_MOVED_H_PENDING) when an H half-step is
+ * pending on return (H, psi_H and S then hold the step before); the caller
+ * passes FDTD_CHAIN_H_PENDING to the next call or runs fdtd_update_h.
+ * Consecutive calls with both flags run every H half-step fused with the
+ * next E half-step: n steps in n / 2 two-step passes.  No counterpart in
+ * the reference, whose caller loop (fdtd.py:162-175) has no call boundary
+ * on the device. */
+#define FDTD_CHAIN_H_PENDING 1
+#define FDTD_CHAIN_DEFER_H 2
+#define FDTD_MOVED_H_PENDING 8
+int fdtd_step_n_chain(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,
+                      const void* src_wave, int32_t flags, void* stream, int32_t* moved);
 
 /* What fdtd_step_n_observed records after every step k = 0 .. n-1.  No
  * counterpart in the reference: a caller there reads sim.S (or E, H) on the

@@ -11,18 +11,20 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 MAX_FACES = 12
 
 F32, F64 = 0, 1
 POYNTING_OFF, POYNTING_FUSED, POYNTING_TWO_PASS = 0, 1, 2
 BLOCH_PHASE, BLOCH_DECOUPLED = 0, 1
+CHAIN_H_PENDING, CHAIN_DEFER_H, MOVED_H_PENDING = 1, 2, 8
 
 # Every symbol include/fdtd.h declares.
 EXPORTS = (
     "fdtd_abi_version", "fdtd_last_error", "fdtd_plan_create", "fdtd_plan_destroy", "fdtd_plan_describe",
     "fdtd_psi_elems", "fdtd_coef_elems", "fdtd_update_e", "fdtd_update_h",
     "fdtd_update_s", "fdtd_update_j", "fdtd_add_point", "fdtd_step_n", "fdtd_step_n_moved", "fdtd_step_n_observed",
+    "fdtd_step_n_chain",
     "fdtd_update_he", "fdtd_update_he_supported", "fdtd_update_he2", "fdtd_update_he2_supported",
     "fdtd_update_he2_preferred",
     "fdtd_adjoint_step", "fdtd_plane_sum", "fdtd_reduce_partials", "fdtd_energy", "fdtd_energy_noalloc",
@@ -108,6 +110,8 @@ def load() -> ctypes.CDLL:
     lib.fdtd_add_point.argtypes = [P, P, ctypes.c_int64, P, P]
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
     lib.fdtd_step_n_moved.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.POINTER(ctypes.c_int32)]
+    lib.fdtd_step_n_chain.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.c_int32, P,
+                                      ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_step_n_observed.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.POINTER(Observe), P,
                                          ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_update_he.argtypes = [P, F, ctypes.c_int64, P, P]

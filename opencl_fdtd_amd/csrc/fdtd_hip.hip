@@ -2878,12 +2878,18 @@ int copy_field(void* dst, const void* src, size_t bytes, hipStream_t st) {
 // (pass counts kept even); otherwise they may end in f->E_out / f->psi_e_out
 // and *moved says where (bit 0: E, bit 1: psi_e).  obs: the observations of
 // fdtd_step_n_observed after every step (one step per pass then).
+// flags (fdtd_step_n_chain): FDTD_CHAIN_H_PENDING runs the previous
+// call's last H half-step first, FDTD_CHAIN_DEFER_H leaves this call's.
 int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
-                void* stream, bool home, int32_t* moved, const fdtd_observe* obs = nullptr) {
+                void* stream, bool home, int32_t* moved, const fdtd_observe* obs = nullptr, int32_t flags = 0) {
     if (!plan) return fail(1, "plan is NULL");
     const Plan& P = *(const Plan*)plan;
     if (moved) *moved = 0;
     if (n < 0) return fail(3, "n = %d < 0", n);
+    if (flags & ~(FDTD_CHAIN_H_PENDING | FDTD_CHAIN_DEFER_H)) return fail(3, "unknown chain flags 0x%x", flags);
+    const bool pend_in = (flags & FDTD_CHAIN_H_PENDING) != 0;
+    const bool defer = (flags & FDTD_CHAIN_DEFER_H) != 0 && n > 0;
+    if ((pend_in || defer) && (home || obs)) return fail(3, "chain flags need fdtd_step_n_chain");
     if (src_wave && (src_index < 0 || src_index >= 3 * P.N))
         return fail(3, "src_index %lld out of [0, %lld)", (long long)src_index, (long long)(3 * P.N));
     if (int rc = check_fields(P, f, true)) return rc;
@@ -2943,23 +2949,27 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     // both counts are kept even (E and psi_e end in f->E and f->psi_e, and
     // H swaps n times in all, as unfused): 2 * (m / 4) he2_march passes,
     // then of the rest an even number of he_march and at most one unfused.
-    // Otherwise m / 2 he2_march passes and at most one he_march.
+    // Otherwise m / 2 he2_march passes and at most one he_march.  With a
+    // pending H_{-1} the pairs start at (H_{-1}, E_0): m = n; with a
+    // deferred H_{n-1} the schedule ends after the last pair.
     const bool pair_bufs = !P.d.bloch_mask ||
                            (f->F_out && f->F_out != f->F && f->G_out && f->G_out != f->G &&
                             (P.d.n_face == 0 || (f->psi_g_out && f->psi_g_out != f->psi_g)));
-    const bool fuse = n >= 2 && he_buffers(P, f) && he_supported(P) && pair_bufs;
+    const bool fuse = n >= (pend_in ? 1 : 2) && he_buffers(P, f) && he_supported(P) && pair_bufs;
     const bool fuse2 = fuse && !obs && he2_supported(P) && P.he2_pref &&
                        (P.d.n_face == 0 || (f->psi_e_out && f->psi_e_out != f->psi_e));
     int k = 0;
     if (fuse) {
-        if (int rc = half_e(0)) return rc;
-        if (int rc = seen_e(0)) return rc;
-        k = 1;
-        const int m = n - 1;
+        if (!pend_in) {
+            if (int rc = half_e(0)) return rc;
+            if (int rc = seen_e(0)) return rc;
+            k = 1;
+        }
+        const int m = pend_in ? n : n - 1;
         const int n2 = fuse2 ? (home ? 2 * (m / 4) : m / 2) : 0;
         // shortest schedule with an odd pair count: the last pair and the
         // final H half-step run as one pass (he2_march without its last E)
-        const bool heh = fuse2 && !home && (m - 2 * n2) == 1;
+        const bool heh = fuse2 && !home && !defer && (m - 2 * n2) == 1;
         const int r = m - 2 * n2 - (heh ? 1 : 0);
         if (home && r % 2 == 1) {
             if (int rc = half_h()) return rc;
@@ -2991,14 +3001,18 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
             std::swap(cur.E, cur.E_out);
             std::swap(cur.psi_e, cur.psi_e_out);
         }
-        if (!heh) {
+        if (!heh && !defer) {
             if (int rc = half_h()) return rc;
             if (int rc = seen_h(n - 1)) return rc;
         }
     } else {
+        if (pend_in && n > 0) {
+            if (int rc = half_h()) return rc;
+        }
         for (k = 0; k < n; ++k) {
             if (int rc = half_e(k)) return rc;
             if (int rc = seen_e(k)) return rc;
+            if (defer && k == n - 1) break;
             if (int rc = half_h()) return rc;
             if (int rc = seen_h(k)) return rc;
         }
@@ -3006,7 +3020,8 @@ int step_n_impl(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_i
     // a two-step pass swaps H once for two H half-steps, so with an odd
     // number of them the newest H is not where n's parity puts it
     if (moved)
-        *moved = (cur.E != f->E ? 1 : 0) | (cur.psi_e != f->psi_e ? 2 : 0) | (cur.H != f->H ? 4 : 0);
+        *moved = (cur.E != f->E ? 1 : 0) | (cur.psi_e != f->psi_e ? 2 : 0) | (cur.H != f->H ? 4 : 0) |
+                 ((defer || (pend_in && n == 0)) ? FDTD_MOVED_H_PENDING : 0);
     return 0;
 }
 
@@ -3023,6 +3038,12 @@ int fdtd_step_n_moved(const void* plan, const fdtd_fields* f, int32_t n, int64_t
                       void* stream, int32_t* moved) {
     if (!moved) return fail(2, "moved is NULL");
     return step_n_impl(plan, f, n, src_index, src_wave, stream, false, moved);
+}
+
+int fdtd_step_n_chain(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index, const void* src_wave,
+                      int32_t flags, void* stream, int32_t* moved) {
+    if (!moved) return fail(2, "moved is NULL");
+    return step_n_impl(plan, f, n, src_index, src_wave, stream, false, moved, nullptr, flags);
 }
 
 int fdtd_step_n_observed(const void* plan, const fdtd_fields* f, int32_t n, int64_t src_index,

@@ -534,7 +534,12 @@ class Simulation:
         enqueued without host synchronisation: E (+F), ``E[src_index] +=
         src_wave[k]``, H (+G) with S.  ``src_wave`` is a device tensor (or
         array) of ``n`` source values in the field dtype.  E and psi_E may end
-        in either of their two buffers (``sim.E`` follows)."""
+        in either of their two buffers (``sim.E`` follows).  With ``defer_h``
+        (the default where the fused pass exists) the last step's H half-step
+        is left pending as ``update_H``'s is: the next ``step_n`` or
+        ``update_E`` runs it in its first pass, and reading any field,
+        ``flush`` or any other update runs it first, so consecutive calls
+        run every H half-step fused with the next E half-step."""
         return self._step_n(n, src_index, src_wave, events, None)
 
     def step_n_observed(self, n: int, src_index: Optional[int] = None, src_wave=None, flux=None,
@@ -578,7 +583,12 @@ class Simulation:
         return self._step_n(n, src_index, src_wave, events, obs)
 
     def _step_n(self, n, src_index, src_wave, events, obs) -> Event:
-        self.flush()
+        # chained (fdtd_step_n_chain): a pending H half-step runs in the
+        # call's first pass and the call's last one is left pending, as
+        # update_H is, for the next call (or flush) to run
+        chain = obs is None and self._defer
+        if not chain:
+            self.flush()
         stream = self._stream()
         for e in events or ():
             stream.wait_event(e.torch_event if isinstance(e, Event) else e)
@@ -592,11 +602,17 @@ class Simulation:
                 raise Exception("src_wave has {} values, need {}".format(wave.numel(), n))
             self._wave_keepalive = wave
             wave_ptr = wave.data_ptr()
-        f = self._fields()
+        f = self._fields(flush=not chain)
         moved = ctypes.c_int32(0)
         idx = int(src_index if src_index is not None else -1)
         with torch.cuda.device(self.device):
-            if obs is None:
+            if chain:
+                flags = _lib.CHAIN_DEFER_H | (_lib.CHAIN_H_PENDING if self._pending_h else 0)
+                _lib.check(self._lib.fdtd_step_n_chain(self._plan, ctypes.byref(f), int(n), idx, wave_ptr, flags,
+                                                       ctypes.c_void_p(stream.cuda_stream), ctypes.byref(moved)),
+                           "fdtd_step_n_chain")
+                self._pending_h = bool(moved.value & _lib.MOVED_H_PENDING)
+            elif obs is None:
                 _lib.check(self._lib.fdtd_step_n_moved(self._plan, ctypes.byref(f), int(n), idx, wave_ptr,
                                                        ctypes.c_void_p(stream.cuda_stream), ctypes.byref(moved)),
                            "fdtd_step_n_moved")

@@ -41,6 +41,8 @@ def _load(path):
     lib.fdtd_psi_elems.restype = ctypes.c_int64
     lib.fdtd_step_n_moved.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_step_n.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, P]
+    lib.fdtd_step_n_chain.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.c_int32, P,
+                                      ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_step_n_observed.argtypes = [P, F, ctypes.c_int32, ctypes.c_int64, P, ctypes.POINTER(L.Observe), P,
                                          ctypes.POINTER(ctypes.c_int32)]
     lib.fdtd_shell_step_n.argtypes = [P, F, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P,
@@ -175,8 +177,10 @@ def _halves(r):
     return HALF[name]
 
 
-def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None):
+def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None, flags=0):
     """Half-steps E H E H ..., each launch reading what the previous wrote.
+    flags (fdtd_step_n_chain): a pending H half-step first (1), the last H
+    half-step left pending (2; *moved bit 3 reports a pending one).
     obs (fdtd_step_n_observed): dict with "flux" (S + comp * N * esz, the
     plane_sum input, or None) and "E_keep" / "H_keep" (destination lists or
     None); after the k-th E half-step E is copied to E_keep[k], after the
@@ -246,7 +250,11 @@ def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None):
                 assert b[SRC] == wave + first_e * esz, (name, first_e)
                 k_src += _halves(r).count("E")
         seq += list(_halves(r))
-    assert "".join(seq) == "EH" * n, ("".join(seq), n)
+    pend_out = bool(flags & 2 and n > 0) or bool(flags & 1 and n == 0)
+    expect = ("H" if flags & 1 and n > 0 else "") + "EH" * n
+    if flags & 2 and n > 0:
+        expect = expect[:-1]
+    assert "".join(seq) == expect, ("".join(seq), n, flags)
     if wave and n:
         assert k_src == n, (k_src, n)
     if obs is not None:
@@ -256,6 +264,7 @@ def check_schedule(recs, f, n, moved, wave, esz, bloch, pingpong, obs=None):
         assert bool(moved & 1) == (loc["E"] == f.E_out) and (loc["E"] in (f.E, f.E_out)), moved
         assert bool(moved & 2) == (loc["psi_e"] == f.psi_e_out) or f.psi_e_out is None
         assert bool(moved & 4) == (loc["H"] == f.H_out and f.H_out != f.H), moved
+        assert bool(moved & 8) == pend_out, (moved, n, flags)
     else:
         assert loc["E"] == f.E and loc["psi_e"] == f.psi_e
 
@@ -460,6 +469,17 @@ def run(lib_path: str, quick: bool = False) -> int:
                     check_decomposition(r)
                 check_schedule(recs, f, n, None if homekeep else moved.value, wave, esz, bool(bloch), pingpong)
                 checked += 1
+                if homekeep:
+                    continue
+                for flags in (1, 2, 3):
+                    moved = ctypes.c_int32(-1)
+                    rc = lib.fdtd_step_n_chain(plan, ctypes.byref(f), n, 1, wave, flags, None, ctypes.byref(moved))
+                    assert rc == 0, (shape, n, flags, lib.fdtd_last_error())
+                    recs = _records(lib)
+                    for r in recs:
+                        check_decomposition(r)
+                    check_schedule(recs, f, n, moved.value, wave, esz, bool(bloch), pingpong, flags=flags)
+                    checked += 1
             checked += _observed(lib, L, plan, shape, esz, poynting, bool(bloch), poynting in (0, 1))
             if not bloch and poynting != 2 and min(shape) >= 8:
                 checked += _shell(lib, L, plan, shape, esz, poynting)

@@ -164,7 +164,7 @@ def test_fp64_reductions():
     acc = torch.zeros(1, dtype=torch.float64, device="cuda")
     for axis, idx in ((0, 3), (1, 12), (2, 0)):
         acc.zero_()
-        assert sim._lib.fdtd_plane_sum(sim.plan, ctypes.c_void_p(sim._S.data_ptr()), axis, axis, idx,
+        assert sim._lib.fdtd_plane_sum(sim.plan, ctypes.c_void_p(sim.S.data_ptr()), axis, axis, idx,
                                        ctypes.c_void_p(acc.data_ptr()), st) == 0
         S = sim.S.get().astype(numpy.float64).reshape(3, *shape)
         want = numpy.take(S[axis], idx, axis=axis).sum()

@@ -225,6 +225,47 @@ def test_step_n_two_step_schedule_equals_unfused():
     assert len(homes) > 1     # the schedule did move E / psi_E
 
 
+@pytest.mark.parametrize("shape, T, dtype, poynting", [
+    ((120, 120, 73), 8, numpy.float32, True),
+    ((23, 30, 73), 8, numpy.float32, False),
+    ((11, 13, 9), 3, numpy.float64, True),
+])
+def test_step_n_calls_chain(shape, T, dtype, poynting):
+    """Consecutive step_n calls with nothing read between them chain
+    (fdtd_step_n_chain): each call leaves its last H half-step pending and
+    the next call (or update_E) runs it in its first pass.  Mixed with
+    zero-step calls and the per-call loop, E, H, S and both psi blocks equal
+    those of the same calls without deferral, bit for bit."""
+    from opencl_fdtd_amd import Simulation
+    print("case step_n chain", shape, T, dtype.__name__, poynting, flush=True)
+    eps, pmls, init, _ = _case(shape, T, seed=43, pec=True)
+    mk = lambda d: Simulation([numpy.asarray(e) for e in eps], [dict(p) for p in pmls], initial_fields=init,
+                              float_type=dtype, do_poynting=poynting, defer_h=d)
+    a, b = mk(True), mk(False)
+    idx = 1 * a.N + (shape[0] // 2) * shape[1] * shape[2] + (shape[1] // 3) * shape[2] + shape[2] // 2
+    wave = numpy.sin(0.17 * numpy.arange(64)).astype(dtype)
+    for s in (a, b):
+        k = 0
+        for op, n in (("n", 5), ("n", 6), ("n", 1), ("n", 0), ("loop", 2), ("n", 4), ("n", 3), ("n", 8)):
+            if op == "n":
+                s.step_n(n, idx, wave[k:k + n])
+                k += n
+                if s is a:
+                    assert s.pending_h                # left for the next call
+                continue
+            for _ in range(n):
+                s.update_E([])
+                s.E[idx] += float(wave[k])
+                s.update_H([])
+                if s.update_S:                        # None without Poynting
+                    s.update_S([])
+                k += 1
+    torch.cuda.synchronize()
+    assert a.pending_h and not b.pending_h
+    _assert_same(_state(a), _state(b), "chained step_n vs unchained")
+    assert not a.pending_h                            # the reads ran it
+
+
 def test_step_n_c_abi_keeps_e_home():
     """fdtd_step_n (the C-ABI entry point without the move report) keeps the
     pass counts even: E and psi_E end in the caller's buffers, with the same
@@ -490,6 +531,7 @@ def test_step_n_observed_equals_single_steps(shape, T, dtype, mode):
     ref_E, ref_H, values = [], [], []
     for k in range(n):
         b.step_n(1, idx, wave[k:k + 1])
+        b.flush()                 # S of the step (step_n leaves its last H half-step pending)
         _lib.check(b._lib.fdtd_plane_sum(b.plan, ctypes.c_void_p(b._S.data_ptr()), comp, axis, index,
                                          ctypes.c_void_p(acc_b.data_ptr()), st), "fdtd_plane_sum")
         Sk = fetch(b, "S").reshape(3, *shape)[comp]
