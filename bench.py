@@ -238,7 +238,7 @@ def learned_report(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 
     return out
 
 
-def learned_summary(steps: int = 100, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), refl_steps: int = 600):
+def learned_summary(steps: int = 300, warmup: int = 20, seeds=(1000, 1001, 1002, 1003), refl_steps: int = 600):
     """The metric's second half, sized for the default run: interior
     Gcell-updates/s of the learned 4-cell shell and of CPML-16 on the same
     256 x 256 x 128 interior, with Poynting off and on, and the reflection
