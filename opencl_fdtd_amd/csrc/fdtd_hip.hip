@@ -103,6 +103,9 @@ namespace {
 #ifndef FDTD_HE2_LDSCARRY
 #define FDTD_HE2_LDSCARRY 1          // two-step pass: H, E, eps of the previous plane read back from LDS
 #endif
+#ifndef FDTD_HE2_LDSNB
+#define FDTD_HE2_LDSNB 1             // two-step pass: E^n at y+1, z+1 read from LDS (written one plane ahead)
+#endif
 #ifndef FDTD_HE2_EARLY
 #define FDTD_HE2_EARLY 0             // two-step pass: next plane's loads issued before the last stage
 #endif
@@ -1774,6 +1777,12 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
     __shared__ T le[2][3][HB];         // E^{n+1} by plane parity
     __shared__ T lb[7][HB];            // oS products of H^{n+3/2}, then H^{n+3/2} (x, y, z)
     __shared__ T ctab[kCoefTab];
+    // NB: E^n of plane x+1 (loaded anyway for the first H update of plane x)
+    // is put in LDS by plane parity, so the first H update of plane x+1 reads
+    // its y+1 and z+1 neighbours there; only the tile's last row (and, with z
+    // segments, its last column) loads them from global memory
+    constexpr bool NB = FDTD_HE2_LDSNB != 0;
+    __shared__ T lx[NB ? 2 : 1][3][NB ? HB : 1];
     const Geom<T>& g = a.g;
     int pb, ck;
     block_coords(g.pblocks, g.nchunk, &pb, &ck);
@@ -1821,6 +1830,15 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
         qy = ldc<false>(a.E, 1, ob + c.op);
         qz = ldc<false>(a.E, 2, ob + c.op);
     }
+    // NB: the neighbours a cell takes from global memory (outside the tile)
+    const bool gy = tc.r == tc.nr - 1;
+    const bool gz = tc.seg && tc.cz == tc.ncz - 1;
+    if constexpr (NB) {
+        lx[0][0][t] = qx;
+        lx[0][1][t] = qy;
+        lx[0][2][t] = qz;
+        lds_barrier();
+    }
     // carried from plane xa-1: H^{n+1/2}, E^{n+1}, the psi pairs after the
     // first half-steps, eps, (S2) oS of the first H half-step; from xa-2:
     // H^{n+3/2} and its oS products
@@ -1861,7 +1879,27 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
                 L.y2 = ldc<false>(a.ycorr, 2, of);
             }
         }
-        h_fetch<T, false>(g, hA, c, xf, pbytes, L.h);
+        if constexpr (NB) {
+            // h_fetch without the y+1 / z+1 neighbours inside the tile
+            const uint32_t ob = (uint32_t)xf * pbytes;
+            const uint32_t on = (xf == g.sx - 1) ? 0u : ob + pbytes;
+            L.h.exn = ldc<FDTD_NT_H>(hA.E, 0, on + c.op);
+            L.h.eyn = ldc<FDTD_NT_H>(hA.E, 1, on + c.op);
+            L.h.ezn = ldc<FDTD_NT_H>(hA.E, 2, on + c.op);
+            if (gy) {
+                L.h.ex_y = ldc<false>(hA.E, 0, ob + c.op_y);
+                L.h.ez_y = ldc<false>(hA.E, 2, ob + c.op_y);
+            }
+            if (gz) {
+                L.h.ex_z = ldc<false>(hA.E, 0, ob + c.op_z);
+                L.h.ey_z = ldc<false>(hA.E, 1, ob + c.op_z);
+            }
+            L.h.hx = ldc<FDTD_NT_H>(hA.Hin, 0, ob + c.op);
+            L.h.hy = ldc<FDTD_NT_H>(hA.Hin, 1, ob + c.op);
+            L.h.hz = ldc<FDTD_NT_H>(hA.Hin, 2, ob + c.op);
+        } else {
+            h_fetch<T, false>(g, hA, c, xf, pbytes, L.h);
+        }
     };
     Loads cur;
     if (EARLY) fetch(xa, false, cur);
@@ -1876,6 +1914,19 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
         ka.hx = slab_hit(g, 0, xa, g.sx);
         kb.hx = slab_hit(g, 0, xb, g.sx);
         if (!EARLY) fetch(xa, ea, cur);
+        if constexpr (NB) {
+            // E^n of plane xa at y+1 and z+1, written one iteration earlier
+            // (or before the march) into the slot of this iteration's parity
+            const int s0 = it & 1;
+            if (!gy) {
+                cur.h.ex_y = lx[s0][0][min(typ, HB - 1)];
+                cur.h.ez_y = lx[s0][2][min(typ, HB - 1)];
+            }
+            if (!gz) {
+                cur.h.ex_z = lx[s0][0][tzp];
+                cur.h.ey_z = lx[s0][1][tzp];
+            }
+        }
         const HIn<T>& in = cur.h;
         const PsiPair<T>& qh = cur.qh;
         const PsiPair<T>& qe = cur.qe;
@@ -1906,6 +1957,13 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
         la[ba][0][t] = g1x;
         la[ba][1][t] = g1y;
         la[ba][2][t] = g1z;
+        if constexpr (NB) {
+            // E^n of plane xa + 1 for the next iteration (its slot was last
+            // read before the previous iteration's first barrier)
+            lx[(it + 1) & 1][0][t] = in.exn;
+            lx[(it + 1) & 1][1][t] = in.eyn;
+            lx[(it + 1) & 1][2][t] = in.ezn;
+        }
         if (S2) {
             la[ba][3][t] = oAxy;
             la[ba][4][t] = oAzy;
