@@ -1767,8 +1767,10 @@ struct HE2Args {
 // SHELL: add the pending learned-shell correction to E^n before the first
 // E update, on every cell that update evaluates (the halo rows' E^{n+1}
 // feeds the second H half-step); the first H update and S use E^n as it is.
+// NB: E^n at y+1, z+1 read from LDS (the tile's last row, and with z
+// segments its last column, still load them from global memory).
 template <class T, bool NU, bool S2, bool E2 = true, bool LC = (FDTD_HE2_LDSCARRY != 0),
-          bool EARLY = (FDTD_HE2_EARLY != 0), bool POY = true, bool SHELL = false>
+          bool EARLY = (FDTD_HE2_EARLY != 0), bool POY = true, bool SHELL = false, bool NB = false>
 __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 3) void he2_march(HE2Args<T> a) {
     static_assert(POY || !S2, "the first half-step's S needs Poynting");
     constexpr int HB = FDTD_HE2_HB;
@@ -1781,7 +1783,6 @@ __global__ __launch_bounds__(FDTD_HE2_HB, FDTD_HE2_HB >= 1024 ? FDTD_HE_WAVES : 
     // is put in LDS by plane parity, so the first H update of plane x+1 reads
     // its y+1 and z+1 neighbours there; only the tile's last row (and, with z
     // segments, its last column) loads them from global memory
-    constexpr bool NB = FDTD_HE2_LDSNB != 0;
     __shared__ T lx[NB ? 2 : 1][3][NB ? HB : 1];
     const Geom<T>& g = a.g;
     int pb, ck;
@@ -2649,30 +2650,44 @@ int launch_he2(const Plan& P, const fdtd_fields* f, hipStream_t st, const float*
     }
     const dim3 grid(P.lhe2.pblocks * P.lhe2.nchunk);
     const bool nu = P.d.nonuniform != 0;
+    constexpr bool LC = FDTD_HE2_LDSCARRY != 0, EA = FDTD_HE2_EARLY != 0, S2 = FDTD_HE2_S2 != 0;
+    // E^n neighbours from LDS (measured, per step: rows of 73 in tiles of
+    // 14 rows 2.8 % faster; 64-cell z segments of rows of 160 1.1 % faster;
+    // rows of 136 in tiles of 7 rows 1 % slower, where a quarter of the
+    // waves hold a last row and the pass spills more)
+    const bool whole = FDTD_HE2_LDSNB != 0 && (P.lhe2.tzb > 1 || P.lhe2.tyo + 3 >= 10);
+#define FDTD_HE2_GO(NU_, S2_, E2_, POY_, SH_)                                                               \
+    do {                                                                                                   \
+        if (whole)                                                                                         \
+            hipLaunchKernelGGL((he2_march<T, NU_, S2_, E2_, LC, EA, POY_, SH_, true>), grid, dim3(FDTD_HE2_HB), \
+                               0, st, a);                                                                  \
+        else                                                                                               \
+            hipLaunchKernelGGL((he2_march<T, NU_, S2_, E2_, LC, EA, POY_, SH_, false>), grid,                 \
+                               dim3(FDTD_HE2_HB), 0, st, a);                                               \
+    } while (0)
     if (ycorr) {
         // the learned shell's passes (uniform grids), Poynting fused or off
-        constexpr bool LC = FDTD_HE2_LDSCARRY != 0, EA = FDTD_HE2_EARLY != 0;
         if (nu) return fail(3, "the shell correction's two-step pass needs a uniform grid");
         const bool poy = P.d.poynting == FDTD_POYNTING_FUSED;
-        if (poy && last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, true, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (poy) hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, true, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, false, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, false, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        if (poy && last_e) FDTD_HE2_GO(false, false, true, true, true);
+        else if (poy) FDTD_HE2_GO(false, false, false, true, true);
+        else if (last_e) FDTD_HE2_GO(false, false, true, false, true);
+        else FDTD_HE2_GO(false, false, false, false, true);
         return check_launch("fdtd_update_he2");
     }
     if (P.d.poynting == FDTD_POYNTING_FUSED) {
-        if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (nu) hipLaunchKernelGGL((he2_march<T, true, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, true>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else hipLaunchKernelGGL((he2_march<T, false, FDTD_HE2_S2 != 0, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        if (nu && last_e) FDTD_HE2_GO(true, S2, true, true, false);
+        else if (nu) FDTD_HE2_GO(true, S2, false, true, false);
+        else if (last_e) FDTD_HE2_GO(false, S2, true, true, false);
+        else FDTD_HE2_GO(false, S2, false, true, false);
     } else {
         // Poynting off: no S (and no S of the first half-step)
-        constexpr bool LC = FDTD_HE2_LDSCARRY != 0, EA = FDTD_HE2_EARLY != 0;
-        if (nu && last_e) hipLaunchKernelGGL((he2_march<T, true, false, true, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (nu) hipLaunchKernelGGL((he2_march<T, true, false, false, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else if (last_e) hipLaunchKernelGGL((he2_march<T, false, false, true, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
-        else hipLaunchKernelGGL((he2_march<T, false, false, false, LC, EA, false>), grid, dim3(FDTD_HE2_HB), 0, st, a);
+        if (nu && last_e) FDTD_HE2_GO(true, false, true, false, false);
+        else if (nu) FDTD_HE2_GO(true, false, false, false, false);
+        else if (last_e) FDTD_HE2_GO(false, false, true, false, false);
+        else FDTD_HE2_GO(false, false, false, false, false);
     }
+#undef FDTD_HE2_GO
     return check_launch("fdtd_update_he2");
 }
 
