@@ -2281,9 +2281,18 @@ Launch make_tile_launch(int sx, int sy, int sz, int block, int yh, int zh, int m
         best.nchunk = (sx + best.xchunk - 1) / best.xchunk;
     };
     if ((int64_t)(1 + yh) * sz <= block) consider(block / sz - yh, sz, 1, sz);
-    if (!whole_only)
+    if (!whole_only) {
         for (int tzo = FDTD_SEG_MIN; tzo <= 256; tzo *= 2)
             if (tzo < sz && tzo + zh <= block) consider(block / (tzo + zh) - yh, tzo, (sz + tzo - 1) / tzo, tzo + zh);
+        // rows cut into k equal segments (rows of 136: two of 68, 11 owned
+        // rows per tile instead of 4 whole rows or 12 rows of 64 + 64 + 8)
+        for (int k = 2; k <= 16; ++k) {
+            const int tzo = (sz + k - 1) / k;
+            if (tzo < FDTD_SEG_MIN) break;
+            if (tzo < sz && tzo <= 256 && tzo + zh <= block)
+                consider(block / (tzo + zh) - yh, tzo, (sz + tzo - 1) / tzo, tzo + zh);
+        }
+    }
     return best;
 }
 
